@@ -1,0 +1,234 @@
+"""Benchmark: batched Sinkhorn barycenter + unrolled gradient on one MI355X.
+
+Workload (one "step"): compute_gradient_both for all 64 histograms of the
+batched standard-domain configuration -- 128x128 FP32 grid, K=5 atoms,
+gamma=1, L=100 Sinkhorn iterations, quadratic loss -- the forward L
+iterations and both unrolled backwards, with inputs already resident in HBM.
+Metric: Sinkhorn-iterations/s = P * L * steps / seconds.
+
+Also measured in the same run: the plain form (same kernels, one solve per
+launch sequence, the reference's serial P loop), the dominant kernel's
+launch duration for the roofline, and the reference CPU path on a bounded
+sample (cpu_baseline), which also gives the error of the GPU result
+against the reference's FP64 output.
+
+python bench.py [--gpus 1] [--steps 20] [--warmup 3]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import wassersteindictionarylearning_amd as wdl  # noqa: E402
+from wassersteindictionarylearning_amd import _hip, synthetic  # noqa: E402
+
+METRIC = ("Sinkhorn barycenter+grad iters/s (batched vs plain, 1 GPU) at ≤1e-5 err; "
+          "p99 ms")
+PEAK_FP32_MFMA = 157.3   # TFLOP/s, MI355X dense FP32 matrix
+PEAK_HBM = 8000.0        # GB/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def band_halfwidth(table64) -> int:
+    t32 = np.asarray(table64).astype(np.float32)
+    nz = np.nonzero(t32)[0]
+    return int(nz.max()) if nz.size else 0
+
+
+def pass_flops(nrows, ncols, D):
+    """MFMA FLOPs one 1-D pass executes: 16-wide contraction chunks whose FP32
+    factor is not all zero, for every 16 x 16 output block (rows_gemm)."""
+    ncb = (ncols + 15) // 16
+    rb = (nrows + 15) // 16
+    chunks = 0
+    for cb in range(ncb):
+        c0 = 16 * cb
+        lo = max(0, c0 - D) // 16
+        hi = min(ncb - 1, (c0 + 15 + D) // 16)
+        chunks += hi - lo + 1
+    return chunks * rb * 16 * 16 * 16 * 2
+
+
+def conv_flops(n, D):
+    """(executed, dense 4n^3) FLOPs of one image-convolution."""
+    return 2 * pass_flops(n, n, D), 4 * n ** 3
+
+
+def cpu_baseline(c, gpu, budget_s=20.0):
+    """Reference CPU path (oracle/_ref, compiled from the reference sources) on
+    a bounded sample of the same workload, and the GPU error against it."""
+    from oracle import ref
+    if not ref.available():
+        return None, None
+    n, K, L, g, loss = c["n"], c["K"], c["L"], c["gamma"], c["loss"]
+    out = {"cores": ref.num_threads(), "kind": "reference"}
+    errs = {"grad_lambda": [], "grad_dic": [], "bary": []}
+    t_sse, done = 0.0, 0
+    for p in range(c["P"]):
+        t0 = time.perf_counter()
+        r = ref.gradient_both(c["atoms"], c["weights"][p], c["obs"][p], n, n, g, L, loss,
+                              variant="sse")
+        t_sse += time.perf_counter() - t0
+        done += 1
+        for k, gk in (("grad_lambda", gpu["grad_lambda"][p]), ("grad_dic", gpu["grad_dic"][p]),
+                      ("bary", gpu["bary"][p])):
+            errs[k].append(float(np.linalg.norm(gk.ravel() - r[k].ravel())
+                                 / np.linalg.norm(r[k].ravel())))
+        if t_sse > budget_s / 2 or done >= 8:
+            break
+    out["value"] = done * L / t_sse
+    out["unit"] = "Sinkhorn-iters/s"
+    out["sample"] = ("compute_gradient_both with GaussianSSE2DKernel (AVX, OpenMP) on %d of the %d "
+                     "histograms at full size (128x128, K=5, L=100), %.1f s" % (done, c["P"], t_sse))
+    # the app's default kernel (Gaussian2DKernel, main_dictionary_learning.cpp:176), one histogram
+    t0 = time.perf_counter()
+    ref.gradient_both(c["atoms"], c["weights"][0], c["obs"][0], n, n, g, L, loss, variant="plain")
+    t_plain = time.perf_counter() - t0
+    out["plain_kernel_value"] = L / t_plain
+    err = {k: max(v) for k, v in errs.items()}
+    err["histograms_checked"] = done
+    err["against"] = "reference compute_gradient_both (FP64, CPU)"
+    return out, err
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain-form timing")
+    ap.add_argument("--dense", action="store_true", help="do not skip all-zero factor chunks")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the P solves would shard by histogram with one K*N all-reduce per "
+              "evaluation; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    c = synthetic.make_problem("batched_std")
+    n, P, K, L, g, loss = c["n"], c["P"], c["K"], c["L"], c["gamma"], c["loss"]
+    N = n * n
+    log("workload batched_std n=%d P=%d K=%d L=%d gamma=%g" % (n, P, K, L, g))
+    ctx = wdl.Context(0)
+    flags = wdl.FLAG_DENSE if args.dense else 0
+    s = wdl.SinkhornSolver(ctx, n, n, K, P, L, g, loss, flags)
+    s.set_observed(c["obs"])
+    s.set_problem(c["atoms"], c["weights"])
+    stream = ctx.stream
+
+    for _ in range(args.warmup):
+        s.run(wdl.RUN_BATCHED)
+    ctx.synchronize()
+    log("warmup done")
+
+    ev = [_hip.Event() for _ in range(args.steps + 1)]
+    _hip.synchronize()
+    ev[0].record(stream)
+    for k in range(args.steps):
+        s.run(wdl.RUN_BATCHED)
+        ev[k + 1].record(stream)
+    ev[-1].synchronize()
+    ctx.synchronize()
+    total_ms = ev[0].elapsed_ms(ev[-1])
+    step_ms = np.array([ev[k].elapsed_ms(ev[k + 1]) for k in range(args.steps)])
+    value = P * L * args.steps / (total_ms / 1e3)
+    log("batched: %.3f ms/step, %.0f iters/s" % (total_ms / args.steps, value))
+
+    res = s.results()
+    res["grad_dic"] = s.grad_dic()
+
+    plain = None
+    if not args.no_plain:
+        e0, e1 = _hip.Event(), _hip.Event()
+        e0.record(stream)
+        s.run(wdl.RUN_PLAIN)
+        e1.record(stream)
+        e1.synchronize()
+        pms = e0.elapsed_ms(e1)
+        plain = {"value": P * L / (pms / 1e3), "ms_per_step": pms,
+                 "form": "same kernels, one solve per launch sequence, P solves in turn, no graph"}
+        log("plain: %.1f ms/step" % pms)
+
+    # Dominant kernel (backward stage: both unrolled chains, 4 image-convolutions
+    # per (histogram, atom)) timed alone on the same stream.
+    reps = 20
+    e0, e1 = _hip.Event(), _hip.Event()
+    step = L // 2
+    s.run_stage(wdl.STAGE_BACKWARD, step)
+    e0.record(stream)
+    for _ in range(reps):
+        s.run_stage(wdl.STAGE_BACKWARD, step)
+    e1.record(stream)
+    e1.synchronize()
+    bwd_ms = e0.elapsed_ms(e1) / reps
+    e0.record(stream)
+    for _ in range(reps):
+        s.run_stage(wdl.STAGE_FORWARD, step)
+    e1.record(stream)
+    e1.synchronize()
+    fwd_ms = e0.elapsed_ms(e1) / reps
+    kern = wdl.HipGaussian2DKernel(ctx, g, n, n)
+    D = 1 << 20 if args.dense else band_halfwidth(kern.kernel1d())
+    f_exec, f_dense = conv_flops(n, D)
+    convs_bwd = 4 * P * K
+    achieved = convs_bwd * f_exec / (bwd_ms * 1e-3) / 1e12
+    # algorithmic HBM bytes of one backward launch (FP32): per (p,i) atom chain
+    # reads b, c, phi, v_i, K v's (L2 hits), A, GD and writes GD, v; weight
+    # chain reads phi, c, b, r_i, K r's, A and writes r.  Counting each
+    # distinct array once: 10 reads + 3 writes of N floats per (p,i).
+    bytes_bwd = P * K * N * 4 * 13
+    roofline = {
+        "bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_FP32_MFMA,
+        "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_MFMA, 4), "traffic": None,
+        "kernel": "k_backward (both unrolled chains, one Sinkhorn step, all 64 histograms)",
+        "launch_ms": round(bwd_ms, 4), "forward_launch_ms": round(fwd_ms, 4),
+        "flops_per_launch": convs_bwd * f_exec, "flop_count": "executed MFMA (banded)",
+        "dense_4n3_flops_per_launch": convs_bwd * f_dense,
+        "dense_4n3_equiv_tflops": round(convs_bwd * f_dense / (bwd_ms * 1e-3) / 1e12, 2),
+        "hbm_algorithmic_bytes_per_launch": bytes_bwd,
+        "hbm_achieved_gbs": round(bytes_bwd / (bwd_ms * 1e-3) / 1e9, 1),
+        "band_halfwidth": D,
+    }
+    log("backward launch %.4f ms -> %.1f TFLOP/s executed" % (bwd_ms, achieved))
+    whole_flops = P * (6 * L - 3) * K * f_dense
+    line = {
+        "metric": METRIC, "value": round(value, 1), "unit": "Sinkhorn-iters/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(total_ms / args.steps, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic: blob histograms (1-4 blobs, sigma 4-16 px, floor 1e-6), "
+                "uniform random atoms and weights, one seed",
+        "config": {"workload": "batched standard domain: compute_gradient_both for all P "
+                               "histograms per step", "n": n, "P": P, "K": K, "L": L,
+                   "gamma": g, "loss": "quadratic", "band_skip": not args.dense},
+        "latency_ms": {"p50": round(float(np.percentile(step_ms, 50)), 4),
+                       "p99": round(float(np.percentile(step_ms, 99)), 4),
+                       "what": "per-step (whole batch) latency"},
+        "dense_4n3_tflops_whole_step": round(whole_flops / (total_ms / args.steps * 1e-3) / 1e12, 2),
+        "plain": plain,
+        "batched_over_plain": (round(value / plain["value"], 2) if plain else None),
+        "roofline": roofline,
+    }
+    if not args.no_cpu:
+        cb, err = cpu_baseline(c, res)
+        line["cpu_baseline"] = cb
+        line["err_vs_fp64"] = err
+    s.close()
+    ctx.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

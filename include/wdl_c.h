@@ -1,0 +1,138 @@
+/*
+ * wdl_c.h -- C ABI of libwdl, the gfx950 (MI355X) Sinkhorn-barycenter hot path
+ * of Wasserstein dictionary learning.
+ *
+ * Every entry point returns an int status (WDL_OK = 0, < 0 on error; the text
+ * is in wdl_last_error()), except wdl_evaluate_both, which has the
+ * lbfgs_evaluate_t shape (lbfgs/lbfgs.h:378-384) and reports errors as NaN.
+ * Signatures use plain pointers and sizes only.  Host-pointer entry points
+ * take row-major H x W images, x fastest, nvectors of them back to back, in
+ * FP64 -- the layout the reference passes to its KernelType (kernels.h:669).
+ * Device-pointer entry points ("_device") take FP32 device memory and are
+ * ordered on the context's stream (wdl_ctx_stream).
+ *
+ * Citations are into matthieuheitz/WassersteinDictionaryLearning, cpp/.
+ */
+#ifndef WDL_C_H
+#define WDL_C_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WDL_OK 0
+#define WDL_ERR_ARG (-1)
+#define WDL_ERR_HIP (-2)
+#define WDL_ERR_UNSUPPORTED (-3)
+#define WDL_ERR_STATE (-4)
+#define WDL_ERR_NOMEM (-5)
+
+/* Loss indices of the reference CLI (main_dictionary_learning.cpp:189-197). */
+#define WDL_LOSS_TV 1
+#define WDL_LOSS_QUADRATIC 2
+#define WDL_LOSS_WASSERSTEIN 3
+#define WDL_LOSS_KL 4
+
+/* Flags for wdl_gauss2d_create / wdl_solver_create. */
+#define WDL_FLAG_LOG_DOMAIN 1   /* solver runs compute_gradient_both_log_signArray */
+#define WDL_FLAG_WARM_RESTART 2 /* solver keeps b_storage / b_temp (inverseWasserstein.h:380-387) */
+#define WDL_FLAG_DENSE 4        /* do not skip all-zero FP32 factor chunks */
+#define WDL_FLAG_NO_GRAPH 8     /* batched runs launch eagerly instead of replaying a hipGraph */
+
+/* Run modes of wdl_solver_run. */
+#define WDL_RUN_BATCHED 0 /* all P solves advance together, one launch per Sinkhorn stage */
+#define WDL_RUN_PLAIN 1   /* one solve per launch sequence, P in turn (inverseWasserstein.h:1052) */
+
+typedef struct wdl_ctx wdl_ctx;
+typedef struct wdl_gauss2d wdl_gauss2d;
+typedef struct wdl_solver wdl_solver;
+
+const char* wdl_last_error(void);
+const char* wdl_version(void);
+
+/* ---- context: one device, one stream --------------------------------------- */
+int wdl_ctx_create(int device, wdl_ctx** out);
+int wdl_ctx_destroy(wdl_ctx* ctx);
+/* The hipStream_t all work of this context is ordered on. */
+void* wdl_ctx_stream(wdl_ctx* ctx);
+int wdl_ctx_synchronize(wdl_ctx* ctx);
+
+/* ---- KernelType: the separable Gaussian operator ---------------------------
+ * Replaces Gaussian2DKernel (kernels.h:630-708), GaussianSSE2DKernel
+ * (kernels.h:820-907) and LogSignArrayGaussian2DKernel (kernels.h:1034-1202)
+ * as the object behind the reference's duck-typed KernelType.              */
+
+/* Gaussian2DKernel(gamma, W, H): kernels.h:632-641. */
+int wdl_gauss2d_create(wdl_ctx* ctx, int W, int H, double gamma, int flags, wdl_gauss2d** out);
+int wdl_gauss2d_destroy(wdl_gauss2d* k);
+/* kernel1d[d] = max(1e-250, exp(-d^2/gamma)), d < max(W,H): kernels.h:636-640. */
+int wdl_gauss2d_table(const wdl_gauss2d* k, double* out);
+/* Gaussian2DKernel::convolve / convolveAdjoint: kernels.h:669-702. */
+int wdl_gauss2d_convolve(wdl_gauss2d* k, const double* u, double* out, int nvectors);
+/* LogSignArrayGaussian2DKernel::log_convolve / log_convolveAdjoint: kernels.h:1073-1115. */
+int wdl_gauss2d_log_convolve(wdl_gauss2d* k, const double* log_u, double* out, int nvectors);
+/* LogSignArrayGaussian2DKernel::log_convolve_signArray / ...Adjoint: kernels.h:1117-1162.
+ * signs: bit i of byte i/8 (LSB first), 1 <=> value >= 0 (signArray.h:8-22);
+ * read as input signs, overwritten with output signs. */
+int wdl_gauss2d_log_convolve_signarray(wdl_gauss2d* k, const double* log_u,
+                                       unsigned char* signs, double* out, int nvectors);
+/* FP32 device form of convolve (no equivalent in the reference; the batched path's operator). */
+int wdl_gauss2d_convolve_device(wdl_gauss2d* k, const float* d_u, float* d_out, int nvectors);
+
+/* ---- Sinkhorn barycenter + unrolled gradient, P histograms per call --------
+ * Holds the state WassersteinRegression keeps per solve (b, conv_b, phi
+ * histories: inverseWasserstein.h:1214) for all P histograms at once.      */
+int wdl_solver_create(wdl_ctx* ctx, int W, int H, int K, int P, int L, double gamma, int loss,
+                      int flags, wdl_solver** out);
+int wdl_solver_destroy(wdl_solver* s);
+/* Device bytes wdl_solver_create would allocate (admission by footprint). */
+long long wdl_solver_footprint(int W, int H, int K, int P, int L, int flags);
+/* Problem::observed_pdf (P x N), inverseWasserstein.h:32-42. */
+int wdl_solver_set_observed(wdl_solver* s, const double* observed);
+int wdl_solver_set_observed_device(wdl_solver* s, const float* d_observed);
+/* Atoms (K x N) and barycentric weights (P x K); normalised on the host as
+ * Problem::normalize_values does per solve (inverseWasserstein.h:53-74). */
+int wdl_solver_set_problem(wdl_solver* s, const double* atoms, const double* lambdas);
+/* Already-normalised FP32 device atoms (K x N) and weights (P x K). */
+int wdl_solver_set_problem_device(wdl_solver* s, const float* d_atoms, const float* d_lambdas);
+/* Warm restart (flag WDL_FLAG_WARM_RESTART): b_storage (P x K x N) seeds b^0
+ * of every solve (inverseWasserstein.h:1057-1061); every run saves b^L into
+ * b_temp (:1081-1085); commit copies b_temp into b_storage (:418-422). */
+int wdl_solver_set_warm_state(wdl_solver* s, const double* b_storage);
+int wdl_solver_get_warm_state(wdl_solver* s, double* b_temp);
+int wdl_solver_warm_commit(wdl_solver* s);
+/* Enqueue the forward L iterations and both unrolled backwards for all P
+ * solves (compute_gradient_both, inverseWasserstein.h:544-674, or its log
+ * form :678-814).  Asynchronous on the context stream. */
+int wdl_solver_run(wdl_solver* s, int mode);
+/* Launch one stage kernel of a batched run over all P solves: stage 0 is
+ * forward iteration `step` (1..L), stage 1 is backward step `step` (L..1).
+ * No counterpart in the reference; for timing and profiling a single kernel.
+ * Stages read what earlier stages of a full run wrote. */
+#define WDL_STAGE_FORWARD 0
+#define WDL_STAGE_BACKWARD 1
+int wdl_solver_run_stage(wdl_solver* s, int stage, int step);
+/* Synchronise and fetch: loss per histogram (loss.h), dL/dlambda (P x K, the
+ * resultLa of each solve), dL/datoms summed over histograms (K x N) and the
+ * barycenters (P x N).  Any pointer may be NULL. */
+int wdl_solver_get_results(wdl_solver* s, double* loss, double* grad_lambda,
+                           double* grad_dic_sum, double* barycenters);
+/* resultDic of every solve (P x K x N). */
+int wdl_solver_get_grad_dic(wdl_solver* s, double* grad_dic);
+/* compute_gradient_both for every histogram: set_problem + run + get_results. */
+int wdl_gradient_both(wdl_solver* s, const double* atoms, const double* lambdas, double* loss,
+                      double* grad_lambda, double* grad_dic_sum, double* barycenters);
+
+/* ---- the L-BFGS callback ------------------------------------------------------
+ * WassersteinRegression::evaluate_both (inverseWasserstein.h:986-1145) with
+ * the per-histogram loop (:1052-1113) batched on the GPU.  `instance` is a
+ * wdl_solver*; x and g hold P*K weights then K*N atoms.  Pass it to lbfgs()
+ * as the lbfgs_evaluate_t.  scale_dictionary and exp_weight play the roles
+ * of WassersteinRegression::scaleDictionary and exp_weight. */
+int wdl_solver_set_scale(wdl_solver* s, double scale_dictionary, int exp_weight);
+double wdl_evaluate_both(void* instance, const double* x, double* g, int n, double step);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WDL_C_H */

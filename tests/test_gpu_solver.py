@@ -1,0 +1,182 @@
+"""Parity of the batched HIP Sinkhorn barycenter + unrolled gradient
+(compute_gradient_both for P histograms) and of evaluate_both with the FP64
+oracle and the reference's golden vectors.
+
+Tolerance (FP32 pipeline, FP64 gradient dot products): relative L2 error
+<= 1e-5 for the weight gradient, the dictionary gradient, the barycenters and
+the loss, as stated for the hot path."""
+import numpy as np
+import pytest
+
+import wassersteindictionarylearning_amd as wdl
+from oracle import oracle as O
+from tests._util import fixtures, rel
+from wassersteindictionarylearning_amd import synthetic
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = wdl.Context(0)
+    yield c
+    c.close()
+
+
+def solve_gpu(ctx, atoms, lam, obs, W, H, g, L, loss, flags=0, mode=wdl.RUN_BATCHED, b0=None):
+    P, K = lam.shape
+    s = wdl.SinkhornSolver(ctx, W, H, K, P, L, g, loss, flags)
+    s.set_observed(obs)
+    if b0 is not None:
+        s.set_warm_state(b0)
+    s.set_problem(atoms, lam)
+    s.run(mode)
+    r = s.results()
+    r["grad_dic"] = s.grad_dic()
+    if b0 is not None:
+        r["b_temp"] = s.get_warm_state()
+    s.close()
+    return r
+
+
+def oracle_all(atoms, lam, obs, W, H, g, L, loss, b0=None):
+    rs = [O.gradient_both(atoms, lam[p], obs[p], W, H, g, L, loss,
+                          b0=None if b0 is None else b0[p]) for p in range(lam.shape[0])]
+    return dict(grad_lambda=np.stack([r["grad_lambda"] for r in rs]),
+                grad_dic=np.stack([r["grad_dic"] for r in rs]),
+                bary=np.stack([r["bary"] for r in rs]),
+                b_last=np.stack([r["b_last"] for r in rs]),
+                loss=np.array([O.loss_value(loss, rs[p]["bary"], obs[p]) for p in range(len(rs))]))
+
+
+def check(r, o):
+    assert rel(r["bary"], o["bary"]) < TOL
+    assert rel(r["loss"], o["loss"]) < TOL
+    for p in range(o["grad_lambda"].shape[0]):
+        assert rel(r["grad_lambda"][p], o["grad_lambda"][p]) < TOL, p
+        assert rel(r["grad_dic"][p], o["grad_dic"][p]) < TOL, p
+    assert rel(r["grad_dic_sum"], o["grad_dic"].sum(0)) < TOL
+
+
+@pytest.mark.parametrize("W,H,K,P,L,g,loss", [
+    (12, 12, 3, 2, 6, 2.0, 2), (16, 12, 2, 3, 10, 1.0, 4), (20, 20, 3, 9, 8, 0.5, 1),
+    (64, 64, 4, 4, 25, 2.0, 2), (128, 128, 5, 2, 100, 1.0, 2)])
+def test_gradient_both_matches_oracle(ctx, W, H, K, P, L, g, loss):
+    print("gradient_both W=%d H=%d K=%d P=%d L=%d gamma=%g loss=%d" % (W, H, K, P, L, g, loss),
+          flush=True)
+    rng = np.random.default_rng(W + 7 * P + L)
+    obs = synthetic.blob_histograms(rng, W, P, (1, 3), (1.5, W / 6)) if W == H else \
+        rng.uniform(0.2, 1, (P, W * H))
+    obs = obs / obs.sum(1, keepdims=True)
+    atoms, lam = synthetic.init_atoms_weights(rng, K, P, W * H)
+    r = solve_gpu(ctx, atoms, lam, obs, W, H, g, L, loss)
+    check(r, oracle_all(atoms, lam, obs, W, H, g, L, loss))
+
+
+@pytest.mark.parametrize("name,d", fixtures("grad"))
+def test_gradient_both_golden(ctx, name, d):
+    W, H, L, g, loss = int(d["W"]), int(d["H"]), int(d["L"]), float(d["gamma"]), int(d["loss"])
+    print("gradient_both golden", name, flush=True)
+    K = d["lam"].size
+    warm = d["b0"].size > 0
+    flags = wdl.FLAG_WARM_RESTART if warm else 0
+    r = solve_gpu(ctx, d["atoms"], d["lam"][None], d["obs"][None], W, H, g, L, loss, flags,
+                  b0=d["b0"][None] if warm else None)
+    assert rel(r["grad_lambda"][0], d["grad_lambda"]) < TOL
+    assert rel(r["grad_dic"][0], d["grad_dic"]) < TOL
+    assert rel(r["bary"][0], d["bary"]) < TOL
+    if warm:
+        assert rel(r["b_temp"][0], d["b_last"]) < TOL
+
+
+def test_plain_equals_batched_and_graph_replays(ctx):
+    # Same kernels, one solve per launch sequence vs all at once: identical bits.
+    W = H = 32
+    K, P, L, g = 3, 10, 12, 1.0
+    print("plain vs batched P=%d" % P, flush=True)
+    rng = np.random.default_rng(3)
+    obs = synthetic.blob_histograms(rng, W, P, (1, 3), (2, 6))
+    atoms, lam = synthetic.init_atoms_weights(rng, K, P, W * H)
+    s = wdl.SinkhornSolver(ctx, W, H, K, P, L, g)
+    s.set_observed(obs)
+    s.set_problem(atoms, lam)
+    s.run(wdl.RUN_PLAIN)
+    plain = s.results()
+    outs = []
+    for rep in range(3):  # graph replays, with eager work and syncs in between
+        s.run(wdl.RUN_BATCHED)
+        outs.append(s.results())
+        s.run(wdl.RUN_PLAIN)
+        s.results()
+    s.close()
+    for o in outs:
+        for k in ("grad_lambda", "grad_dic_sum", "bary", "loss"):
+            np.testing.assert_array_equal(o[k], plain[k], err_msg=k)
+    ne = wdl.SinkhornSolver(ctx, W, H, K, P, L, g, flags=wdl.FLAG_NO_GRAPH)
+    ne.set_observed(obs)
+    ne.set_problem(atoms, lam)
+    ne.run()
+    eager = ne.results()
+    ne.close()
+    for k in ("grad_lambda", "grad_dic_sum", "bary", "loss"):
+        np.testing.assert_array_equal(eager[k], plain[k], err_msg=k)
+
+
+def test_each_histogram_same_alone_and_in_batch(ctx):
+    W = H = 24
+    K, P, L, g = 3, 8, 10, 1.0
+    print("alone vs batch P=%d" % P, flush=True)
+    rng = np.random.default_rng(5)
+    obs = synthetic.blob_histograms(rng, W, P, (1, 3), (2, 5))
+    atoms, lam = synthetic.init_atoms_weights(rng, K, P, W * H)
+    full = solve_gpu(ctx, atoms, lam, obs, W, H, g, L, 2)
+    for p in (0, 5):
+        one = solve_gpu(ctx, atoms, lam[p:p + 1], obs[p:p + 1], W, H, g, L, 2)
+        np.testing.assert_array_equal(one["grad_lambda"][0], full["grad_lambda"][p])
+        np.testing.assert_array_equal(one["bary"][0], full["bary"][p])
+        np.testing.assert_array_equal(one["grad_dic"][0], full["grad_dic"][p])
+
+
+@pytest.mark.parametrize("name,d", [f for f in fixtures("eval") if not f[1]["log_domain"]])
+def test_evaluate_both_golden(ctx, name, d):
+    W, H, K, P, L = (int(d[k]) for k in ("W", "H", "K", "P", "L"))
+    print("evaluate_both golden", name, flush=True)
+    warm = d["b_storage"].size > 0
+    s = wdl.SinkhornSolver(ctx, W, H, K, P, L, float(d["gamma"]), int(d["loss"]),
+                           wdl.FLAG_WARM_RESTART if warm else 0)
+    s.set_observed(d["obs"])
+    s.set_scale(float(d["scale"]), bool(d["exp_weight"]))
+    if warm:
+        s.set_warm_state(d["b_storage"])
+    f, g = s.evaluate_both(d["x"])
+    assert abs(f - float(d["f"])) <= TOL * abs(float(d["f"]))
+    assert rel(g, d["g"]) < TOL
+    if warm:
+        assert rel(s.get_warm_state(), d["b_temp"]) < TOL
+        s.warm_commit()
+        f2, _ = s.evaluate_both(d["x"])  # now seeded from b_temp
+        o2, _, _ = O.evaluate_both(d["x"], d["obs"], W, H, float(d["gamma"]), K, P, L,
+                                   int(d["loss"]), float(d["scale"]), bool(d["exp_weight"]),
+                                   b_storage=d["b_temp"])
+        assert abs(f2 - o2) <= TOL * abs(o2)
+    s.close()
+
+
+def test_evaluate_both_config_shape(ctx):
+    # The plain small configuration (64x64, K=4, L=25, gamma=2), 4 of its 16 histograms.
+    c = synthetic.make_problem("plain_small", P=4)
+    n, P, K, L, g = c["n"], c["P"], c["K"], c["L"], c["gamma"]
+    N = n * n
+    print("evaluate_both plain_small P=%d" % P, flush=True)
+    scale = (N / P) / c["scale_factor"]
+    x = O.encode_variables(c["weights"], c["atoms"], scale)
+    s = wdl.SinkhornSolver(ctx, n, n, K, P, L, g, c["loss"])
+    s.set_observed(c["obs"])
+    s.set_scale(scale, True)
+    f, gr = s.evaluate_both(x)
+    s.close()
+    fo, go, _ = O.evaluate_both(x, c["obs"], n, n, g, K, P, L, c["loss"], scale)
+    assert abs(f - fo) <= TOL * abs(fo)
+    assert rel(gr[:P * K], go[:P * K]) < TOL
+    assert rel(gr[P * K:], go[P * K:]) < TOL

@@ -1,0 +1,275 @@
+"""Pins the FP64 oracle (CPU only).
+
+Each oracle function is checked against the compiled reference's golden
+vectors (tests/golden, written from oracle/_ref by make_golden.py), against
+the compiled reference itself when oracle/_ref is built, and against what the
+mathematics fixes: a dense brute-force operator, closed forms, adjointness,
+finite differences of the L-step map.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from oracle import ref as R
+from tests._util import fixtures, maxrel, rel
+
+needs_ref = pytest.mark.skipif(not R.available(), reason="oracle/_ref not built")
+
+
+# ---------------------------------------------------------------- fixtures ---
+@pytest.mark.parametrize("name,d", fixtures("table"))
+def test_table_fixture(name, d):
+    t = O.kernel_table(int(d["W"]), int(d["H"]), float(d["gamma"]))
+    # numpy's exp and libm's exp may differ in the last ulp
+    np.testing.assert_allclose(t, d["out"], rtol=4e-16, atol=0)
+
+
+@pytest.mark.parametrize("name,d", fixtures("conv"))
+def test_convolve_fixture(name, d):
+    W, H, g = int(d["W"]), int(d["H"]), float(d["gamma"])
+    out = O.convolve(d["u"], W, H, g)
+    assert maxrel(out, d["out"]) < 1e-13
+    if "out_sse" in d:  # the AVX kernel agrees with the plain one
+        assert maxrel(d["out_sse"], d["out"]) < 1e-13
+
+
+@pytest.mark.parametrize("name,d", fixtures("lconv"))
+def test_log_convolve_fixture(name, d):
+    W, H, g = int(d["W"]), int(d["H"]), float(d["gamma"])
+    out = O.log_convolve(d["u"], W, H, g)
+    fin = np.isfinite(d["out"])
+    np.testing.assert_array_equal(fin, np.isfinite(out))
+    assert np.max(np.abs(out[fin] - d["out"][fin])) < 1e-11
+
+
+@pytest.mark.parametrize("name,d", fixtures("sconv"))
+def test_log_convolve_signarray_fixture(name, d):
+    W, H, g = int(d["W"]), int(d["H"]), float(d["gamma"])
+    out, s = O.log_convolve_signarray(d["u"], d["signs_in"], W, H, g)
+    np.testing.assert_array_equal(s, d["signs_out"])
+    fin = np.isfinite(d["out"])
+    np.testing.assert_array_equal(fin, np.isfinite(out))
+    assert np.max(np.abs(out[fin] - d["out"][fin])) < 1e-10
+
+
+def test_sign_helpers_fixture():
+    (_, d), = fixtures("signs")
+    lv, pos = O.abs_log(d["v"])
+    np.testing.assert_allclose(lv, d["log"], rtol=4e-16, atol=0)
+    np.testing.assert_array_equal(O.pack_signs(pos, 2), d["signs"])
+    np.testing.assert_allclose(O.abs_exp(lv, O.unpack_signs(d["signs"], 16)), d["back"],
+                               rtol=4e-16, atol=0)
+    # LSB-first layout (signArray.h:13): value 0 (=1) is bit 0 of byte 0.
+    assert d["signs"][0] & 1 == 1 and (d["signs"][0] >> 1) & 1 == 0
+
+
+@pytest.mark.parametrize("name,d", fixtures("grad"))
+def test_gradient_both_fixture(name, d):
+    W, H, L, g, loss = int(d["W"]), int(d["H"]), int(d["L"]), float(d["gamma"]), int(d["loss"])
+    b0 = d["b0"] if d["b0"].size else None
+    r = O.gradient_both(d["atoms"], d["lam"], d["obs"], W, H, g, L, loss, b0=b0)
+    for k in ("grad_lambda", "grad_dic", "bary", "b_last"):
+        assert rel(r[k], d[k]) < 1e-12, k
+
+
+@pytest.mark.parametrize("name,d", fixtures("glog"))
+def test_gradient_both_log_fixture(name, d):
+    W, H, L, g, loss = int(d["W"]), int(d["H"]), int(d["L"]), float(d["gamma"]), int(d["loss"])
+    r = O.gradient_both_log(d["atoms"], d["lam"], d["obs"], W, H, g, L, loss)
+    for k in ("grad_lambda", "grad_dic", "bary", "b_last"):
+        assert rel(r[k], d[k]) < 1e-12, k
+
+
+@pytest.mark.parametrize("name,d", fixtures("eval"))
+def test_evaluate_both_fixture(name, d):
+    W, H, K, P, L = (int(d[k]) for k in ("W", "H", "K", "P", "L"))
+    bs = d["b_storage"] if d["b_storage"].size else None
+    f, g, bt = O.evaluate_both(d["x"], d["obs"], W, H, float(d["gamma"]), K, P, L, int(d["loss"]),
+                               float(d["scale"]), bool(d["exp_weight"]), b_storage=bs,
+                               log_domain=bool(d["log_domain"]))
+    assert abs(f - float(d["f"])) <= 1e-12 * abs(float(d["f"]))
+    assert rel(g, d["g"]) < 1e-11
+    if bs is not None:
+        assert rel(bt, d["b_temp"]) < 1e-12
+
+
+# ------------------------------------------------- against the compiled ref ---
+@needs_ref
+@pytest.mark.parametrize("W,H,g", [(24, 24, 1.0), (31, 18, 0.5), (40, 40, 25.0)])
+def test_convolve_vs_ref(W, H, g):
+    u = np.random.default_rng(W * H).uniform(0, 1, 3 * W * H)
+    assert maxrel(O.convolve(u, W, H, g), R.convolve(u, W, H, g)) < 1e-13
+
+
+@needs_ref
+@pytest.mark.parametrize("loss", [1, 2, 4])
+def test_gradient_vs_ref(loss):
+    rng = np.random.default_rng(loss)
+    W, H, K, L, g = 20, 14, 3, 8, 1.0
+    atoms, lam = rng.uniform(0, 1, (K, W * H)), rng.uniform(0, 1, K)
+    obs = rng.uniform(0.5, 1, W * H)
+    obs /= obs.sum()
+    a = O.gradient_both(atoms, lam, obs, W, H, g, L, loss)
+    b = R.gradient_both(atoms, lam, obs, W, H, g, L, loss)
+    for k in ("grad_lambda", "grad_dic", "bary"):
+        assert rel(a[k], b[k]) < 1e-12, k
+    c = O.gradient_both_log(atoms, lam, obs, W, H, g, L, loss)
+    d = R.gradient_both(atoms, lam, obs, W, H, g, L, loss, variant="log")
+    for k in ("grad_lambda", "grad_dic", "bary"):
+        assert rel(c[k], d[k]) < 1e-11, k
+
+
+# ------------------------------------------------------------- mathematics ---
+def brute_operator(W, H, g):
+    """Dense N x N matrix exp(-|p-q|^2/gamma) floored at 1e-250 (kernels.h:656-662)."""
+    ys, xs = np.divmod(np.arange(W * H), W)
+    d2 = (xs[:, None] - xs[None, :]) ** 2 + (ys[:, None] - ys[None, :]) ** 2
+    # separable: exp(-dx^2/g) * exp(-dy^2/g), each factor floored as kernel1d is
+    fx = np.maximum(1e-250, np.exp(-((xs[:, None] - xs[None, :]) ** 2) / g))
+    fy = np.maximum(1e-250, np.exp(-((ys[:, None] - ys[None, :]) ** 2) / g))
+    return fx * fy, d2
+
+
+@pytest.mark.parametrize("W,H,g", [(5, 4, 1.0), (7, 9, 0.5), (6, 6, 30.0)])
+def test_convolve_brute_force(W, H, g):
+    M, _ = brute_operator(W, H, g)
+    u = np.random.default_rng(1).normal(size=(2, W * H))
+    np.testing.assert_allclose(O.convolve(u, W, H, g), u @ M.T, rtol=1e-12, atol=1e-15)
+
+
+def test_delta_response_and_adjoint():
+    W, H, g = 11, 13, 2.0
+    t = O.kernel_table(W, H, g)
+    u = np.zeros(W * H)
+    i0, j0 = 5, 3
+    u[i0 * W + j0] = 1.0
+    out = O.convolve(u, W, H, g).reshape(H, W)
+    expect = np.outer(t[np.abs(np.arange(H) - i0)], t[np.abs(np.arange(W) - j0)])
+    np.testing.assert_allclose(out, expect, rtol=1e-14)
+    rng = np.random.default_rng(3)
+    a, b = rng.normal(size=W * H), rng.normal(size=W * H)
+    assert abs(np.dot(O.convolve(a, W, H, g), b) - np.dot(a, O.convolve(b, W, H, g))) < 1e-12
+
+
+def test_log_convolve_matches_standard():
+    W, H, g = 9, 7, 1.5
+    x = np.random.default_rng(4).uniform(-3, 3, 2 * W * H)
+    np.testing.assert_allclose(O.log_convolve(x, W, H, g), np.log(O.convolve(np.exp(x), W, H, g)),
+                               rtol=0, atol=1e-12)
+    s = np.random.default_rng(5).integers(0, 2, x.size).astype(bool)
+    val = np.where(s, 1.0, -1.0) * np.exp(x)
+    ref_v = O.convolve(val, W, H, g)
+    out, bits = O.log_convolve_signarray(x, O.pack_signs(s), W, H, g)
+    np.testing.assert_allclose(O.abs_exp(out, O.unpack_signs(bits, x.size)), ref_v, rtol=1e-10,
+                               atol=1e-13)
+
+
+def test_forward_single_atom_closed_form():
+    # K = 1, lambda = 1: b stays 1, so the barycenter is K(p / K1) for any L (:565-592).
+    W, H, g = 10, 8, 1.0
+    rng = np.random.default_rng(6)
+    p = rng.uniform(0.1, 1, (1, W * H))
+    p /= p.sum()
+    q_expect = O.convolve(p[0] / O.convolve(np.ones(W * H), W, H, g), W, H, g)
+    for L in (1, 4):
+        r = O.gradient_both(p, np.ones(1), np.full(W * H, 1.0 / (W * H)), W, H, g, L)
+        np.testing.assert_allclose(r["bary"], q_expect, rtol=1e-13)
+
+
+def _loss_of(atoms, lam, obs, W, H, g, L, loss):
+    r = O.gradient_both(atoms, lam, obs, W, H, g, L, loss)
+    return O.loss_value(loss, r["bary"], obs)
+
+
+@pytest.mark.parametrize("loss", [2, 4])
+def test_gradient_finite_differences(loss):
+    # normalize_values renormalises lambda and the atoms, so the derivative of
+    # the loss along coordinate e_i is G_i - <w, G> (the projection of :1095-1108).
+    W, H, K, L, g = 6, 5, 2, 5, 1.0
+    rng = np.random.default_rng(7 + loss)
+    atoms = rng.uniform(0.2, 1, (K, W * H))
+    atoms /= atoms.sum(1, keepdims=True)
+    lam = rng.uniform(0.2, 1, K)
+    lam /= lam.sum()
+    obs = rng.uniform(0.2, 1, W * H)
+    obs /= obs.sum()
+    r = O.gradient_both(atoms, lam, obs, W, H, g, L, loss)
+    h = 1e-6
+    fd = np.zeros(K)
+    for i in range(K):
+        e = np.zeros(K)
+        e[i] = h
+        fd[i] = (_loss_of(atoms, lam + e, obs, W, H, g, L, loss)
+                 - _loss_of(atoms, lam - e, obs, W, H, g, L, loss)) / (2 * h)
+    GL = r["grad_lambda"]
+    assert rel(fd, GL - lam @ GL) < 1e-6
+    GD = r["grad_dic"]
+    proj = GD - (atoms * GD).sum(1, keepdims=True)
+    for (i, x) in [(0, 3), (1, 17), (1, 29)]:
+        e = np.zeros_like(atoms)
+        e[i, x] = h * atoms[i, x]
+        fdx = (_loss_of(atoms + e, lam, obs, W, H, g, L, loss)
+               - _loss_of(atoms - e, lam, obs, W, H, g, L, loss)) / (2 * h * atoms[i, x])
+        assert abs(fdx - proj[i, x]) < 1e-6 * np.abs(proj).max()
+
+
+def test_evaluate_both_finite_differences():
+    W, H, K, P, L, g, s = 5, 5, 2, 2, 4, 1.0, 3.0
+    rng = np.random.default_rng(11)
+    obs = rng.uniform(0.2, 1, (P, W * H))
+    obs /= obs.sum(1, keepdims=True)
+    x = np.concatenate([np.log(rng.uniform(0.2, 1, P * K)) * s, np.log(rng.uniform(0.2, 1, K * W * H))])
+    f, gr, _ = O.evaluate_both(x, obs, W, H, g, K, P, L, 2, s)
+    h = 1e-6
+    for j in [0, 3, P * K + 1, P * K + 30, len(x) - 1]:
+        e = np.zeros_like(x)
+        e[j] = h
+        fd = (O.evaluate_both(x + e, obs, W, H, g, K, P, L, 2, s)[0]
+              - O.evaluate_both(x - e, obs, W, H, g, K, P, L, 2, s)[0]) / (2 * h)
+        assert abs(fd - gr[j]) < 1e-6 * np.abs(gr).max()
+
+
+def test_domains_agree():
+    W, H, K, L, g = 12, 10, 3, 6, 1.0
+    rng = np.random.default_rng(12)
+    atoms, lam = rng.uniform(0.1, 1, (K, W * H)), rng.uniform(0.1, 1, K)
+    obs = rng.uniform(0.2, 1, W * H)
+    obs /= obs.sum()
+    for loss in (2, 4):
+        a = O.gradient_both(atoms, lam, obs, W, H, g, L, loss)
+        b = O.gradient_both_log(atoms, lam, obs, W, H, g, L, loss)
+        for k in ("grad_lambda", "grad_dic", "bary"):
+            assert rel(b[k], a[k]) < 1e-11, k
+
+
+def test_planted_sign_error_is_caught():
+    # A plausible slip (dropping the minus of -p/conv_b^2 in the weight
+    # recursion, :657) must fail the finite-difference check above; plant it
+    # by flipping the sign of the weight loop's second convolution (call 4L+1:
+    # 2L forward calls, 2L-1 dictionary-gradient calls before it).
+    W, H, K, L, g = 5, 4, 2, 3, 1.0
+    rng = np.random.default_rng(13)
+    atoms = rng.uniform(0.2, 1, (K, W * H))
+    atoms /= atoms.sum(1, keepdims=True)
+    lam = np.array([0.4, 0.6])
+    obs = rng.uniform(0.2, 1, W * H)
+    obs /= obs.sum()
+    good = O.gradient_both(atoms, lam, obs, W, H, g, L, 2)["grad_lambda"]
+    orig = O.convolve
+    calls = {"n": 0}
+
+    def wrong(u, W_, H_, g_):
+        calls["n"] += 1
+        return -orig(u, W_, H_, g_) if calls["n"] == 4 * L + 1 else orig(u, W_, H_, g_)
+
+    O.convolve = wrong
+    try:
+        bad = O.gradient_both(atoms, lam, obs, W, H, g, L, 2)["grad_lambda"]
+    finally:
+        O.convolve = orig
+    h = 1e-6
+    fd = np.array([(_loss_of(atoms, lam + h * np.eye(K)[i], obs, W, H, g, L, 2)
+                    - _loss_of(atoms, lam - h * np.eye(K)[i], obs, W, H, g, L, 2)) / (2 * h)
+                   for i in range(K)])
+    assert rel(fd, good - lam @ good) < 1e-6
+    assert rel(fd, bad - lam @ bad) > 1e-3

@@ -1,0 +1,282 @@
+"""MI355X (gfx950) Sinkhorn-barycenter hot path of Wasserstein dictionary learning.
+
+The product is the C-ABI shared library ``_lib/libwdl.so`` (HIP kernels,
+declared in ``include/wdl_c.h``) and the header-only C++ KernelType in
+``include/wdl/hip_kernel_type.hpp``.  This module is a thin ctypes layer over
+the same C ABI for Python callers, tests and the benchmark.  It never falls
+back to a CPU path: if the library is missing, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "_lib", "libwdl.so")
+
+WDL_OK = 0
+LOSS_TV, LOSS_QUADRATIC, LOSS_WASSERSTEIN, LOSS_KL = 1, 2, 3, 4
+FLAG_LOG_DOMAIN, FLAG_WARM_RESTART, FLAG_DENSE, FLAG_NO_GRAPH = 1, 2, 4, 8
+RUN_BATCHED, RUN_PLAIN = 0, 1
+STAGE_FORWARD, STAGE_BACKWARD = 0, 1
+
+_lib = None
+_vp = ctypes.c_void_p
+_d = ctypes.POINTER(ctypes.c_double)
+_u8 = ctypes.POINTER(ctypes.c_ubyte)
+_i = ctypes.c_int
+
+# name -> (restype, argtypes); mirrors include/wdl_c.h
+_SIGS = {
+    "wdl_last_error": (ctypes.c_char_p, []),
+    "wdl_version": (ctypes.c_char_p, []),
+    "wdl_ctx_create": (_i, [_i, ctypes.POINTER(_vp)]),
+    "wdl_ctx_destroy": (_i, [_vp]),
+    "wdl_ctx_stream": (_vp, [_vp]),
+    "wdl_ctx_synchronize": (_i, [_vp]),
+    "wdl_gauss2d_create": (_i, [_vp, _i, _i, ctypes.c_double, _i, ctypes.POINTER(_vp)]),
+    "wdl_gauss2d_destroy": (_i, [_vp]),
+    "wdl_gauss2d_table": (_i, [_vp, _d]),
+    "wdl_gauss2d_convolve": (_i, [_vp, _d, _d, _i]),
+    "wdl_gauss2d_log_convolve": (_i, [_vp, _d, _d, _i]),
+    "wdl_gauss2d_log_convolve_signarray": (_i, [_vp, _d, _u8, _d, _i]),
+    "wdl_gauss2d_convolve_device": (_i, [_vp, _vp, _vp, _i]),
+    "wdl_solver_create": (_i, [_vp, _i, _i, _i, _i, _i, ctypes.c_double, _i, _i,
+                               ctypes.POINTER(_vp)]),
+    "wdl_solver_destroy": (_i, [_vp]),
+    "wdl_solver_footprint": (ctypes.c_longlong, [_i, _i, _i, _i, _i, _i]),
+    "wdl_solver_set_observed": (_i, [_vp, _d]),
+    "wdl_solver_set_observed_device": (_i, [_vp, _vp]),
+    "wdl_solver_set_problem": (_i, [_vp, _d, _d]),
+    "wdl_solver_set_problem_device": (_i, [_vp, _vp, _vp]),
+    "wdl_solver_set_warm_state": (_i, [_vp, _d]),
+    "wdl_solver_get_warm_state": (_i, [_vp, _d]),
+    "wdl_solver_warm_commit": (_i, [_vp]),
+    "wdl_solver_run": (_i, [_vp, _i]),
+    "wdl_solver_run_stage": (_i, [_vp, _i, _i]),
+    "wdl_solver_get_results": (_i, [_vp, _d, _d, _d, _d]),
+    "wdl_solver_get_grad_dic": (_i, [_vp, _d]),
+    "wdl_gradient_both": (_i, [_vp, _d, _d, _d, _d, _d, _d]),
+    "wdl_solver_set_scale": (_i, [_vp, ctypes.c_double, _i]),
+    "wdl_evaluate_both": (ctypes.c_double, [_vp, _d, _d, _i, ctypes.c_double]),
+}
+
+
+class WdlError(RuntimeError):
+    pass
+
+
+def lib():
+    """Loads libwdl.so; raises if it has not been built (no CPU fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise WdlError("libwdl.so not built: run python -m wassersteindictionarylearning_amd._build")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(status: int, what: str):
+    if status != WDL_OK:
+        raise WdlError("%s failed (%d): %s" % (what, status, lib().wdl_last_error().decode()))
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(_d)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Context:
+    """One device, one HIP stream (wdl_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = _vp()
+        _check(lib().wdl_ctx_create(device, ctypes.byref(h)), "wdl_ctx_create")
+        self.handle = h
+        self.device = device
+
+    @property
+    def stream(self) -> int:
+        return lib().wdl_ctx_stream(self.handle)
+
+    def synchronize(self):
+        _check(lib().wdl_ctx_synchronize(self.handle), "wdl_ctx_synchronize")
+
+    def close(self):
+        if self.handle:
+            lib().wdl_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class HipGaussian2DKernel:
+    """The KernelType surface of kernels.h (Gaussian2DKernel / LogSignArrayGaussian2DKernel).
+
+    Constructor argument order follows kernels.h:632 (gamma, W, H).
+    """
+
+    def __init__(self, ctx: Context, gamma: float, W: int, H: int, flags: int = 0):
+        h = _vp()
+        _check(lib().wdl_gauss2d_create(ctx.handle, W, H, float(gamma), flags, ctypes.byref(h)),
+               "wdl_gauss2d_create")
+        self.handle, self.ctx, self.gamma, self.W, self.H = h, ctx, float(gamma), W, H
+        self.N = W * H
+
+    def kernel1d(self):
+        out = np.zeros(max(self.W, self.H))
+        _check(lib().wdl_gauss2d_table(self.handle, _dp(out)), "wdl_gauss2d_table")
+        return out
+
+    def convolve(self, u, nvectors: int | None = None):
+        u = _f64(u)
+        nv = u.size // self.N if nvectors is None else nvectors
+        out = np.zeros_like(u)
+        _check(lib().wdl_gauss2d_convolve(self.handle, _dp(u), _dp(out), nv), "wdl_gauss2d_convolve")
+        return out
+
+    convolveAdjoint = convolve
+
+    def log_convolve(self, log_u, nvectors: int | None = None):
+        u = _f64(log_u)
+        nv = u.size // self.N if nvectors is None else nvectors
+        out = np.zeros_like(u)
+        _check(lib().wdl_gauss2d_log_convolve(self.handle, _dp(u), _dp(out), nv),
+               "wdl_gauss2d_log_convolve")
+        return out
+
+    log_convolveAdjoint = log_convolve
+
+    def log_convolve_signArray(self, log_u, signs, nvectors: int | None = None):
+        """Returns (out, signs_out); the input sign array is not modified."""
+        u = _f64(log_u)
+        nv = u.size // self.N if nvectors is None else nvectors
+        s = np.array(signs, dtype=np.uint8, copy=True)
+        out = np.zeros_like(u)
+        _check(lib().wdl_gauss2d_log_convolve_signarray(self.handle, _dp(u),
+                                                        s.ctypes.data_as(_u8), _dp(out), nv),
+               "wdl_gauss2d_log_convolve_signarray")
+        return out, s
+
+    log_convolve_signArrayAdjoint = log_convolve_signArray
+
+    def convolve_device(self, d_in: int, d_out: int, nvectors: int):
+        _check(lib().wdl_gauss2d_convolve_device(self.handle, d_in, d_out, nvectors),
+               "wdl_gauss2d_convolve_device")
+
+    def close(self):
+        if self.handle:
+            lib().wdl_gauss2d_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class SinkhornSolver:
+    """P compute_gradient_both solves at once (wdl_solver), plus evaluate_both."""
+
+    def __init__(self, ctx: Context, W, H, K, P, L, gamma, loss=LOSS_QUADRATIC, flags=0):
+        h = _vp()
+        _check(lib().wdl_solver_create(ctx.handle, W, H, K, P, L, float(gamma), loss, flags,
+                                       ctypes.byref(h)), "wdl_solver_create")
+        self.handle, self.ctx = h, ctx
+        self.W, self.H, self.K, self.P, self.L = W, H, K, P, L
+        self.N = W * H
+
+    @staticmethod
+    def footprint(W, H, K, P, L, flags=0) -> int:
+        return int(lib().wdl_solver_footprint(W, H, K, P, L, flags))
+
+    def set_observed(self, observed):
+        o = _f64(observed)
+        assert o.size == self.P * self.N
+        _check(lib().wdl_solver_set_observed(self.handle, _dp(o)), "wdl_solver_set_observed")
+
+    def set_observed_device(self, d_obs: int):
+        _check(lib().wdl_solver_set_observed_device(self.handle, d_obs),
+               "wdl_solver_set_observed_device")
+
+    def set_problem(self, atoms, lambdas):
+        a, l = _f64(atoms), _f64(lambdas)
+        assert a.size == self.K * self.N and l.size == self.P * self.K
+        _check(lib().wdl_solver_set_problem(self.handle, _dp(a), _dp(l)), "wdl_solver_set_problem")
+
+    def set_problem_device(self, d_atoms: int, d_lambdas: int):
+        _check(lib().wdl_solver_set_problem_device(self.handle, d_atoms, d_lambdas),
+               "wdl_solver_set_problem_device")
+
+    def set_warm_state(self, b_storage):
+        b = _f64(b_storage)
+        _check(lib().wdl_solver_set_warm_state(self.handle, _dp(b)), "wdl_solver_set_warm_state")
+
+    def get_warm_state(self):
+        out = np.zeros((self.P, self.K, self.N))
+        _check(lib().wdl_solver_get_warm_state(self.handle, _dp(out)), "wdl_solver_get_warm_state")
+        return out
+
+    def warm_commit(self):
+        _check(lib().wdl_solver_warm_commit(self.handle), "wdl_solver_warm_commit")
+
+    def run(self, mode: int = RUN_BATCHED):
+        _check(lib().wdl_solver_run(self.handle, mode), "wdl_solver_run")
+
+    def run_stage(self, stage: int, step: int):
+        _check(lib().wdl_solver_run_stage(self.handle, stage, step), "wdl_solver_run_stage")
+
+    def results(self, barycenters: bool = True):
+        loss = np.zeros(self.P)
+        gl = np.zeros((self.P, self.K))
+        gd = np.zeros((self.K, self.N))
+        q = np.zeros((self.P, self.N)) if barycenters else None
+        _check(lib().wdl_solver_get_results(self.handle, _dp(loss), _dp(gl), _dp(gd), _dp(q)),
+               "wdl_solver_get_results")
+        return dict(loss=loss, grad_lambda=gl, grad_dic_sum=gd, bary=q)
+
+    def grad_dic(self):
+        out = np.zeros((self.P, self.K, self.N))
+        _check(lib().wdl_solver_get_grad_dic(self.handle, _dp(out)), "wdl_solver_get_grad_dic")
+        return out
+
+    def set_scale(self, scale_dictionary: float, exp_weight: bool = True):
+        _check(lib().wdl_solver_set_scale(self.handle, float(scale_dictionary), int(exp_weight)),
+               "wdl_solver_set_scale")
+
+    def evaluate_both(self, x):
+        """The lbfgs_evaluate_t callback: returns (f, g)."""
+        x = _f64(x)
+        g = np.zeros_like(x)
+        f = lib().wdl_evaluate_both(self.handle, _dp(x), _dp(g), x.size, 0.0)
+        if np.isnan(f):
+            err = lib().wdl_last_error().decode()
+            if err:
+                raise WdlError("wdl_evaluate_both: " + err)
+        return f, g
+
+    def close(self):
+        if self.handle:
+            lib().wdl_solver_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,147 @@
+// Standard-domain separable Gaussian convolution: the drop-in for
+// Gaussian2DKernel::convolve (kernels.h:669-699) and its FP32 device form.
+#include <cmath>
+#include <cstring>
+
+#include "wdl_internal.h"
+
+namespace wdl {
+
+// One workgroup per image (grid-stride over images).  T is FP32 (device form)
+// or FP64 (host drop-in: values are rounded to FP32 for the MFMA passes and
+// widened back on store).
+template <class T>
+__global__ __launch_bounds__(kThreads, 2) void k_conv(const T* __restrict__ in,
+                                                      T* __restrict__ out, int nvec, ConvGeom G,
+                                                      const float* __restrict__ gtab) {
+  extern __shared__ __align__(16) float smem[];
+  float* tabs = smem;
+  float* buf = smem + 4 * kTabLen;
+  load_tables(tabs, gtab);
+  const size_t N = (size_t)G.W * G.H;
+  for (int v = blockIdx.x; v < nvec; v += gridDim.x) {
+    const T* src = in + (size_t)v * N;
+    for_each_px(G, [&](int li, int gi, bool ok) { buf[li] = ok ? (float)src[gi] : 0.f; });
+    conv_lds(buf, tabs, G);
+    T* dst = out + (size_t)v * N;
+    for_each_px(G, [&](int li, int gi, bool ok) {
+      if (ok) dst[gi] = (T)buf[li];
+    });
+    __syncthreads();
+  }
+}
+
+template <class T>
+static int launch_conv(const wdl_gauss2d* k, const T* in, T* out, int nvec, hipStream_t s) {
+  if (nvec <= 0) return WDL_OK;
+  const ConvGeom G = k->geom(!(k->flags & WDL_FLAG_DENSE));
+  const int lds = conv_lds_bytes(G);
+  static bool attr_set = false;
+  if (!attr_set) {
+    WDL_HIP(hipFuncSetAttribute((const void*)k_conv<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                conv_lds_bytes(make_geom(kMaxSide, kMaxSide, 0))));
+    attr_set = true;
+  }
+  const int grid = nvec;
+  hipLaunchKernelGGL(k_conv<T>, dim3(grid), dim3(kThreads), lds, s, in, out, nvec, G,
+                     k->tab.as<float>());
+  WDL_HIP(hipGetLastError());
+  return WDL_OK;
+}
+
+int launch_conv_f32(const wdl_gauss2d* k, const float* in, float* out, int nvec, hipStream_t s) {
+  return launch_conv<float>(k, in, out, nvec, s);
+}
+
+int launch_conv_f64io(const wdl_gauss2d* k, const double* in, double* out, int nvec,
+                      hipStream_t s) {
+  return launch_conv<double>(k, in, out, nvec, s);
+}
+
+}  // namespace wdl
+
+wdl::ConvGeom wdl_gauss2d::geom(bool band) const {
+  return wdl::make_geom(W, H, band ? D : (1 << 20));
+}
+
+extern "C" {
+
+int wdl_gauss2d_create(wdl_ctx* ctx, int W, int H, double gamma, int flags, wdl_gauss2d** out) {
+  WDL_CHECK(ctx && out, WDL_ERR_ARG, "wdl_gauss2d_create: NULL argument");
+  WDL_CHECK(W > 0 && H > 0, WDL_ERR_ARG, "wdl_gauss2d_create: W and H must be positive");
+  WDL_CHECK(gamma > 0 && std::isfinite(gamma), WDL_ERR_ARG,
+            "wdl_gauss2d_create: gamma must be positive and finite");
+  WDL_CHECK(W <= wdl::kMaxSide && H <= wdl::kMaxSide, WDL_ERR_UNSUPPORTED,
+            "wdl_gauss2d_create: sides above " + std::to_string(wdl::kMaxSide) +
+                " are not supported by the LDS-resident convolution");
+  WDL_HIP(hipSetDevice(ctx->device));
+  wdl_gauss2d* k = new wdl_gauss2d();
+  k->ctx = ctx;
+  k->W = W;
+  k->H = H;
+  k->gamma = gamma;
+  k->flags = flags;
+  std::vector<float> tab;
+  k->D = wdl::build_table(W, H, gamma, tab);
+  int st = k->tab.ensure(tab.size() * sizeof(float));
+  if (st != WDL_OK) {
+    delete k;
+    return st;
+  }
+  hipError_t e = hipMemcpy(k->tab.p, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    k->tab.release();
+    delete k;
+    wdl::set_error(std::string("hipMemcpy table: ") + hipGetErrorString(e));
+    return WDL_ERR_HIP;
+  }
+  *out = k;
+  return WDL_OK;
+}
+
+int wdl_gauss2d_destroy(wdl_gauss2d* k) {
+  if (!k) return WDL_OK;
+  (void)hipSetDevice(k->ctx->device);
+  (void)hipStreamSynchronize(k->ctx->stream);
+  k->tab.release();
+  k->io_in.release();
+  k->io_out.release();
+  k->io_sign.release();
+  delete k;
+  return WDL_OK;
+}
+
+int wdl_gauss2d_table(const wdl_gauss2d* k, double* out) {
+  WDL_CHECK(k && out, WDL_ERR_ARG, "wdl_gauss2d_table: NULL argument");
+  const int n = k->W > k->H ? k->W : k->H;
+  for (int d = 0; d < n; ++d) out[d] = std::max(1e-250, std::exp(-(double)d * d / k->gamma));
+  return WDL_OK;
+}
+
+int wdl_gauss2d_convolve(wdl_gauss2d* k, const double* u, double* out, int nvectors) {
+  WDL_CHECK(k && u && out, WDL_ERR_ARG, "wdl_gauss2d_convolve: NULL argument");
+  WDL_CHECK(nvectors >= 0, WDL_ERR_ARG, "wdl_gauss2d_convolve: nvectors < 0");
+  if (nvectors == 0) return WDL_OK;
+  WDL_HIP(hipSetDevice(k->ctx->device));
+  const size_t bytes = (size_t)k->W * k->H * nvectors * sizeof(double);
+  int st = k->io_in.ensure(bytes);
+  if (st) return st;
+  if ((st = k->io_out.ensure(bytes))) return st;
+  hipStream_t s = k->ctx->stream;
+  WDL_HIP(hipMemcpyAsync(k->io_in.p, u, bytes, hipMemcpyHostToDevice, s));
+  if ((st = wdl::launch_conv_f64io(k, k->io_in.as<double>(), k->io_out.as<double>(), nvectors, s)))
+    return st;
+  WDL_HIP(hipMemcpyAsync(out, k->io_out.p, bytes, hipMemcpyDeviceToHost, s));
+  WDL_HIP(hipStreamSynchronize(s));
+  return WDL_OK;
+}
+
+int wdl_gauss2d_convolve_device(wdl_gauss2d* k, const float* d_u, float* d_out, int nvectors) {
+  WDL_CHECK(k && d_u && d_out, WDL_ERR_ARG, "wdl_gauss2d_convolve_device: NULL argument");
+  WDL_CHECK(nvectors >= 0, WDL_ERR_ARG, "wdl_gauss2d_convolve_device: nvectors < 0");
+  WDL_CHECK(d_u != d_out, WDL_ERR_ARG, "wdl_gauss2d_convolve_device: in and out alias");
+  WDL_HIP(hipSetDevice(k->ctx->device));
+  return wdl::launch_conv_f32(k, d_u, d_out, nvectors, k->ctx->stream);
+}
+
+}  // extern "C"

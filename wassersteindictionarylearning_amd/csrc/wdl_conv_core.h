@@ -1,0 +1,159 @@
+// Device-side building blocks for the separable Gaussian convolution on gfx950.
+//
+// One workgroup (256 threads = 4 wave64) owns one H x W image at a time; the
+// image lives in LDS for both 1-D passes, so a whole image-convolution costs
+// one HBM read and one HBM write of the image (plus whatever the fused
+// prologue / epilogue touches).  Each 1-D pass is a GEMM
+//     OUT[r][j] = sum_k IN[r][k] * Kg(k, j),   Kg(k, j) = table[|k - j|]
+// run on v_mfma_f32_16x16x4_f32, and its result is written back to LDS
+// transposed, so pass 2 is the same routine applied to the transposed
+// intermediate (R = K_H . U . K_W with K symmetric, kernels.h:669-699).
+//
+// The Gaussian factor is never materialised: the B operand is read from a
+// 1-D table of exp(-d^2/gamma) kept in LDS in four copies shifted by 0..3
+// floats, so every lane fetches its four consecutive-k factors with one
+// aligned 16-byte LDS read.  With `band` on, 16-wide contraction chunks whose
+// factors are all exactly 0.0f in FP32 (|k - j| beyond the FP32 underflow of
+// exp(-d^2/gamma)) are skipped; they would add exact zeros.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace wdl {
+
+constexpr int kThreads = 256;              // 4 wave64 per workgroup
+constexpr int kWaves = kThreads / 64;
+constexpr int kMaxSide = 128;              // largest padded side held in LDS
+constexpr int kTabOff = kMaxSide + 16;     // table index of d = 0
+constexpr int kTabLen = 2 * kMaxSide + 48; // multiple of 4
+constexpr int kMaxUnits = 4;               // 16x64 output units per wave per pass
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+struct ConvGeom {
+  int W, H;     // image (x fastest)
+  int Wp, Hp;   // padded to a multiple of 16
+  int S1;       // LDS row stride of the row-major image (floats)
+  int S2;       // LDS row stride of the transposed intermediate (floats)
+  int D;        // band half-width: factors with |d| > D are exactly 0
+};
+
+// Bytes of dynamic LDS for one workgroup.
+__host__ __device__ inline int conv_lds_bytes(const ConvGeom& g) {
+  const int img = (g.Hp * g.S1 > g.Wp * g.S2) ? g.Hp * g.S1 : g.Wp * g.S2;
+  return (4 * kTabLen + img) * (int)sizeof(float);
+}
+
+// copy s of the table holds tab[x + s]; all four copies are 16-byte aligned.
+__device__ inline void load_tables(float* tabs, const float* __restrict__ gtab) {
+  for (int e = threadIdx.x; e < 4 * kTabLen; e += kThreads) {
+    const int s = e / kTabLen, x = e - s * kTabLen;
+    tabs[e] = (x + s < kTabLen) ? gtab[x + s] : 0.f;
+  }
+}
+
+__device__ inline floatx4 mfma4(float a, float b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// One 1-D pass over an LDS matrix IN (nrows x ncols, row stride sin):
+// OUT = IN . Kg (ncols x ncols).  Output unit u of this wave covers output
+// columns [16*cb, 16*cb+16) and rows [64*rg, 64*rg+64); acc[uu][rb] holds the
+// 16x16 block of rows 64*rg+16*rb.. in the MFMA 16x16 layout (lane l, reg r:
+// row 4*(l/16)+r, col l%16).
+__device__ inline void rows_gemm(const float* __restrict__ in, int sin, int nrows, int ncols,
+                                 const float* __restrict__ tabs, int D,
+                                 floatx4 (&acc)[kMaxUnits][4]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int c = lane & 15, g = lane >> 4;
+  const int ncb = ncols >> 4;
+  const int nunits = ncb * ((nrows + 63) >> 6);
+#pragma unroll
+  for (int uu = 0; uu < kMaxUnits; ++uu) {
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) acc[uu][rb] = floatx4{0.f, 0.f, 0.f, 0.f};
+    const int u = wave + uu * kWaves;
+    if (u < nunits) {
+      const int cb = u % ncb, rg = u / ncb;
+      const int c0 = cb << 4, r0 = rg << 6;
+      const int nrb = min(4, (nrows - r0) >> 4);
+      const int lo = c0 - D;
+      const int kc_lo = lo > 0 ? (lo >> 4) : 0;
+      const int kc_hi = min(ncb - 1, (c0 + 15 + D) >> 4);
+      const float* arow = in + (r0 + c) * sin + 4 * g;
+      for (int kc = kc_lo; kc <= kc_hi; ++kc) {
+        const int kk = kc << 4;
+        // B[k][j] = Kg(kk + 4g + t, c0 + c) = tab[kTabOff + kk + 4g + t - c0 - c]
+        const int a = kTabOff + kk + 4 * g - c0 - c;
+        const floatx4 bv = *reinterpret_cast<const floatx4*>(tabs + (a & 3) * kTabLen + (a & ~3));
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) {
+          if (rb < nrb) {
+            // A[i][k] = IN[r0 + 16rb + c][kk + 4g + t]
+            const floatx4 av = *reinterpret_cast<const floatx4*>(arow + rb * 16 * sin + kk);
+            acc[uu][rb] = mfma4(av.x, bv.x, acc[uu][rb]);
+            acc[uu][rb] = mfma4(av.y, bv.y, acc[uu][rb]);
+            acc[uu][rb] = mfma4(av.z, bv.z, acc[uu][rb]);
+            acc[uu][rb] = mfma4(av.w, bv.w, acc[uu][rb]);
+          }
+        }
+      }
+    }
+  }
+}
+
+// Writes OUT (nrows x ncols, held in acc by rows_gemm) to dst transposed:
+// dst[j * sdst + r] = OUT[r][j].
+__device__ inline void store_transposed(float* dst, int sdst, int nrows, int ncols,
+                                        const floatx4 (&acc)[kMaxUnits][4]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int c = lane & 15, g = lane >> 4;
+  const int ncb = ncols >> 4;
+  const int nunits = ncb * ((nrows + 63) >> 6);
+#pragma unroll
+  for (int uu = 0; uu < kMaxUnits; ++uu) {
+    const int u = wave + uu * kWaves;
+    if (u < nunits) {
+      const int cb = u % ncb, rg = u / ncb;
+      const int c0 = cb << 4, r0 = rg << 6;
+      const int nrb = min(4, (nrows - r0) >> 4);
+#pragma unroll
+      for (int rb = 0; rb < 4; ++rb)
+        if (rb < nrb)
+          *reinterpret_cast<floatx4*>(dst + (c0 + c) * sdst + r0 + 16 * rb + 4 * g) = acc[uu][rb];
+    }
+  }
+}
+
+// buf holds U (Hp x Wp, stride S1, zero outside H x W) on entry and
+// R = K_H . U . K_W (stride S1; values outside H x W are don't-care) on exit.
+// All threads of the workgroup must call it; it synchronises internally.
+__device__ inline void conv_lds(float* buf, const float* tabs, const ConvGeom& G) {
+  floatx4 acc[kMaxUnits][4];
+  __syncthreads();
+  rows_gemm(buf, G.S1, G.Hp, G.Wp, tabs, G.D, acc);   // T = U . K_W
+  __syncthreads();
+  store_transposed(buf, G.S2, G.Hp, G.Wp, acc);        // buf = T^T (Wp x Hp)
+  __syncthreads();
+  rows_gemm(buf, G.S2, G.Wp, G.Hp, tabs, G.D, acc);   // R^T = T^T . K_H
+  __syncthreads();
+  store_transposed(buf, G.S1, G.Wp, G.Hp, acc);        // buf = R (Hp x Wp)
+  __syncthreads();
+}
+
+// Visits every padded position of the LDS image; one wave per row, so global
+// accesses of a wave are 64 consecutive floats.  f(lds_index, pixel, valid).
+template <class F>
+__device__ inline void for_each_px(const ConvGeom& G, F&& f) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int row = wave; row < G.Hp; row += kWaves)
+    for (int col = lane; col < G.Wp; col += 64)
+      f(row * G.S1 + col, row * G.W + col, row < G.H && col < G.W);
+}
+
+// safe_log (loss.h:3-5) in FP32: log(max(1e-250, x)); 1e-250 is below the
+// FP32 range, so every x <= 0 (and NaN, as std::max returns its first
+// argument) maps to log(1e-250).
+__device__ inline float safe_logf(float x) { return x > 0.f ? logf(x) : -575.64627324851142f; }
+
+}  // namespace wdl

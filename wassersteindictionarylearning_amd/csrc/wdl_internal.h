@@ -1,0 +1,79 @@
+// Host-side internals shared by the HIP translation units of libwdl.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/wdl_c.h"
+#include "wdl_conv_core.h"
+
+namespace wdl {
+
+// Thread-local message behind wdl_last_error().
+void set_error(const std::string& msg);
+
+#define WDL_HIP(expr)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess) {                                                             \
+      ::wdl::set_error(std::string(#expr) + ": " + hipGetErrorString(e_));              \
+      return WDL_ERR_HIP;                                                               \
+    }                                                                                   \
+  } while (0)
+
+#define WDL_CHECK(cond, code, msg)   \
+  do {                               \
+    if (!(cond)) {                   \
+      ::wdl::set_error(msg);         \
+      return (code);                 \
+    }                                \
+  } while (0)
+
+// Device buffer owned by the library (freed in the owner's destructor).
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t nbytes);   // grows (never shrinks); returns status
+  void release();
+  template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// FP32 table for the LDS copies: tab[kTabOff + d] = (float)max(1e-250, exp(-d^2/gamma))
+// for |d| < max(W,H), 0 elsewhere (kernels.h:636-640).  Returns D, the largest
+// |d| whose FP32 factor is non-zero.
+int build_table(int W, int H, double gamma, std::vector<float>& tab);
+
+ConvGeom make_geom(int W, int H, int D);
+
+}  // namespace wdl
+
+struct wdl_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int num_cus = 0;
+};
+
+struct wdl_gauss2d {
+  wdl_ctx* ctx = nullptr;
+  int W = 0, H = 0;
+  double gamma = 0;
+  int D = 0;              // FP32 band half-width
+  int flags = 0;
+  wdl::DevBuf tab;        // kTabLen floats
+  wdl::DevBuf io_in, io_out;   // staging for the host-pointer entry points
+  wdl::DevBuf io_sign;
+  wdl::ConvGeom geom(bool band) const;
+};
+
+namespace wdl {
+
+// Launchers (stream-ordered, no synchronisation).
+int launch_conv_f32(const wdl_gauss2d* k, const float* in, float* out, int nvec, hipStream_t s);
+int launch_conv_f64io(const wdl_gauss2d* k, const double* in, double* out, int nvec,
+                      hipStream_t s);
+
+}  // namespace wdl

@@ -1,0 +1,780 @@
+// Batched Sinkhorn barycenter + unrolled gradient, standard domain
+// (compute_gradient_both, inverseWasserstein.h:544-674), for P histograms at
+// once, and the evaluate_both callback around it (:986-1145).
+//
+// Device layout (FP32 unless noted; pixel index fastest):
+//   A   [K][N]        atoms (normalised)            lam [P][K]   obs [P][N]
+//   Bh  [L][P][K][N]  Bh[s-1] = b^s   (s = 1..L)    history b    (:553-558)
+//   Ch  [L][P][K][N]  Ch[s-1] = c^{s-1} = K b^{s-1}  history conv_b
+//   Fh  [L][P][K][N]  Fh[s-1] = phi^s = K a^s        history phi
+//   V0/V1, R0/R1 [P][K][N]  ping-pong v (atom chain) and r (weight chain)
+//   GD  [P][K][N]  resultDic per histogram;  GL [P][K] (FP64) resultLa
+//   Q, G0 [P][N]   barycenter q^L and the loss gradient at q^L
+//
+// Each Sinkhorn stage is one launch over (histogram, atom) workgroups; a
+// workgroup keeps its image in LDS through two convolutions with the
+// elementwise steps fused around them.  Cross-atom reductions (the geometric
+// mean, n = sum_i v_i, g = sum_i r_i) are recomputed by each of the K
+// workgroups of a histogram from the previous stage's output; the K
+// workgroups of one histogram share an XCD so these re-reads hit its L2.
+#include <cmath>
+#include <cstring>
+#include <limits>
+
+#include "wdl_internal.h"
+
+namespace wdl {
+
+constexpr int kLossBlocks = 32;
+
+struct SolveArgs {
+  ConvGeom G;
+  int K, P, L, p0, np, loss;
+  long long N;      // pixels per image
+  long long stage;  // P*K*N floats between consecutive history entries
+  const float* tab;
+  const float* A;
+  const float* lam;
+  const float* obs;
+  float* Bh;
+  float* Ch;
+  float* Fh;
+  float* V0;
+  float* V1;
+  float* R0;
+  float* R1;
+  float* GD;
+  double* GL;
+  float* Q;
+  float* G0;
+  double* lossPart;
+  const float* bInit;
+  float* bSave;
+};
+
+// Workgroup -> (histogram, atom).  With 8+ histograms the K workgroups of a
+// histogram get block ids congruent mod 8, i.e. the same XCD.
+__device__ inline bool map_block(const SolveArgs& a, int& p, int& i) {
+  const int b = blockIdx.x;
+  int pl;
+  if (a.np >= 8) {
+    const int xcd = b & 7, slot = b >> 3;
+    pl = (slot / a.K) * 8 + xcd;
+    i = slot % a.K;
+  } else {
+    pl = b / a.K;
+    i = b % a.K;
+  }
+  if (pl >= a.np) return false;
+  p = a.p0 + pl;
+  return true;
+}
+
+static int stage_grid(int np, int K) { return np >= 8 ? 8 * K * ((np + 7) / 8) : np * K; }
+
+__device__ inline float loss_grad(int loss, float q, float o) {
+  if (loss == WDL_LOSS_KL) return safe_logf(q / o);          // loss.h:83-89
+  if (loss == WDL_LOSS_TV) return (q - o > 0.f) ? 0.5f : -0.5f;  // loss.h:58-64
+  return 2.f * (q - o);                                       // loss.h:34-40
+}
+
+__device__ inline double loss_term(int loss, float q, float o) {
+  const double dq = q, dob = o;
+  if (loss == WDL_LOSS_KL) {                                  // loss.h:74-81
+    const double r = dq / dob;
+    return dq * std::log(r > 1e-250 ? r : 1e-250) - dq + dob;
+  }
+  if (loss == WDL_LOSS_TV) return 0.5 * std::fabs(dq - dob);  // loss.h:49-56
+  return (dq - dob) * (dq - dob);                             // loss.h:25-32
+}
+
+// Sum over the workgroup, result valid in thread 0.
+__device__ inline double block_sum(double v) {
+  __shared__ double red[kWaves];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < kWaves; ++w) t += red[w];
+  __syncthreads();
+  return t;
+}
+
+// Forward iteration s (1..L): b^{s-1} (geometric-mean step of iteration s-1,
+// :577-592), c^{s-1} = K b^{s-1} (:568), a = p / c (:569-574), phi^s = K a (:575).
+__global__ __launch_bounds__(kThreads, 2) void k_forward(SolveArgs a, int s) {
+  int p, i;
+  if (!map_block(a, p, i)) return;
+  extern __shared__ __align__(16) float smem[];
+  float* tabs = smem;
+  float* buf = smem + 4 * kTabLen;
+  load_tables(tabs, a.tab);
+  const ConvGeom G = a.G;
+  const long long N = a.N;
+  const long long hbase = (long long)p * a.K * N;
+  const long long img = hbase + (long long)i * N;
+  if (s == 1) {
+    const float* bi = a.bInit ? a.bInit + img : nullptr;
+    for_each_px(G, [&](int li, int gi, bool ok) { buf[li] = ok ? (bi ? bi[gi] : 1.f) : 0.f; });
+  } else {
+    const float* Fp = a.Fh + (long long)(s - 2) * a.stage + hbase;  // phi^{s-1}
+    float* Bd = a.Bh + (long long)(s - 2) * a.stage + img;          // b^{s-1}
+    const float* lam = a.lam + (long long)p * a.K;
+    const int K = a.K;
+    for_each_px(G, [&](int li, int gi, bool ok) {
+      float b = 0.f;
+      if (ok) {
+        float lq = 0.f;
+        for (int j = 0; j < K; ++j) lq += lam[j] * safe_logf(Fp[(long long)j * N + gi]);
+        b = expf(lq) / Fp[(long long)i * N + gi];
+        Bd[gi] = b;
+      }
+      buf[li] = b;
+    });
+  }
+  conv_lds(buf, tabs, G);
+  {
+    float* Cd = a.Ch + (long long)(s - 1) * a.stage + img;
+    const float* Ai = a.A + (long long)i * N;
+    for_each_px(G, [&](int li, int gi, bool ok) {
+      float u = 0.f;
+      if (ok) {
+        const float c = buf[li];
+        Cd[gi] = c;
+        u = Ai[gi] / c;
+      }
+      buf[li] = u;
+    });
+  }
+  conv_lds(buf, tabs, G);
+  float* Fd = a.Fh + (long long)(s - 1) * a.stage + img;
+  for_each_px(G, [&](int li, int gi, bool ok) {
+    if (ok) Fd[gi] = buf[li];
+  });
+}
+
+// After iteration L: q = exp(sum_i lam_i safe_log phi_i^L), b^L = q / phi^L,
+// the loss and its gradient at q (:577-597, :1089).
+__global__ __launch_bounds__(kThreads) void k_final(SolveArgs a) {
+  const int pl = blockIdx.y;
+  const int p = a.p0 + pl;
+  const long long N = a.N;
+  const int K = a.K;
+  const long long hbase = (long long)p * K * N;
+  const float* F = a.Fh + (long long)(a.L - 1) * a.stage + hbase;
+  float* B = a.Bh + (long long)(a.L - 1) * a.stage + hbase;
+  const float* lam = a.lam + (long long)p * K;
+  double acc = 0;
+  for (long long x = (long long)blockIdx.x * kThreads + threadIdx.x; x < N;
+       x += (long long)gridDim.x * kThreads) {
+    float lq = 0.f;
+    for (int j = 0; j < K; ++j) lq += lam[j] * safe_logf(F[j * N + x]);
+    const float q = expf(lq);
+    for (int j = 0; j < K; ++j) {
+      const float b = q / F[j * N + x];
+      B[j * N + x] = b;
+      if (a.bSave) a.bSave[hbase + j * N + x] = b;
+    }
+    const float o = a.obs[(long long)p * N + x];
+    a.Q[(long long)p * N + x] = q;
+    a.G0[(long long)p * N + x] = loss_grad(a.loss, q, o);
+    acc += loss_term(a.loss, q, o);
+  }
+  const double t = block_sum(acc);
+  if (threadIdx.x == 0) a.lossPart[(long long)p * kLossBlocks + blockIdx.x] = t;
+}
+
+// Backward step s (L..1).  blockIdx.y = 0: dictionary gradient (:600-629);
+// blockIdx.y = 1: weight gradient (:635-673).
+__global__ __launch_bounds__(kThreads, 2) void k_backward(SolveArgs a, int s) {
+  int p, i;
+  if (!map_block(a, p, i)) return;
+  extern __shared__ __align__(16) float smem[];
+  float* tabs = smem;
+  float* buf = smem + 4 * kTabLen;
+  load_tables(tabs, a.tab);
+  const ConvGeom G = a.G;
+  const long long N = a.N;
+  const int K = a.K;
+  const long long hbase = (long long)p * K * N;
+  const long long img = hbase + (long long)i * N;
+  const float lpi = a.lam[(long long)p * K + i];
+  const float* Ai = a.A + (long long)i * N;
+  const float* Cs = a.Ch + (long long)(s - 1) * a.stage + img;  // c^{s-1}
+  const bool last = (s == a.L);
+  if (blockIdx.y == 0) {
+    const float* Vp = (s & 1) ? a.V0 : a.V1;
+    float* Vc = (s & 1) ? a.V1 : a.V0;
+    const float* Bs = a.Bh + (long long)(s - 1) * a.stage + img;  // b^s
+    const float* G0 = a.G0 + (long long)p * N;
+    for_each_px(G, [&](int li, int gi, bool ok) {
+      float t = 0.f;
+      if (ok) {
+        float n, v;
+        if (last) {
+          n = G0[gi];
+          v = 0.f;
+        } else {
+          n = 0.f;
+          for (int j = 0; j < K; ++j) n += Vp[hbase + (long long)j * N + gi];
+          v = Vp[img + gi];
+        }
+        t = (lpi * n - v) * Bs[gi];
+      }
+      buf[li] = t;
+    });
+    conv_lds(buf, tabs, G);
+    float* GDp = a.GD + img;
+    for_each_px(G, [&](int li, int gi, bool ok) {
+      float t = 0.f;
+      if (ok) {
+        const float c = buf[li];
+        const float cb = Cs[gi];
+        const float gd = c / cb;
+        GDp[gi] = last ? gd : GDp[gi] + gd;
+        t = -Ai[gi] * c / (cb * cb);
+      }
+      buf[li] = t;
+    });
+    if (s == 1) return;
+    conv_lds(buf, tabs, G);
+    const float* Fm = a.Fh + (long long)(s - 2) * a.stage + img;  // phi^{s-1}
+    for_each_px(G, [&](int li, int gi, bool ok) {
+      if (ok) Vc[img + gi] = buf[li] / Fm[gi];
+    });
+  } else {
+    const float* Rp = (s & 1) ? a.R0 : a.R1;
+    float* Rc = (s & 1) ? a.R1 : a.R0;
+    const float* Fs = a.Fh + (long long)(s - 1) * a.stage + img;  // phi^s
+    const float* G0 = a.G0 + (long long)p * N;
+    const float* Qp = a.Q + (long long)p * N;
+    double dot = 0;
+    for_each_px(G, [&](int li, int gi, bool ok) {
+      float u = 0.f;
+      if (ok) {
+        float g, r;
+        if (last) {
+          g = G0[gi] * Qp[gi];
+          r = 0.f;
+        } else {
+          g = 0.f;
+          for (int j = 0; j < K; ++j) g += Rp[hbase + (long long)j * N + gi];
+          r = Rp[img + gi];
+        }
+        const float f = Fs[gi];
+        dot += (double)logf(f) * (double)g;  // plain log (:649)
+        u = (lpi * g - r) / f;
+      }
+      buf[li] = u;
+    });
+    const double t = block_sum(dot);
+    if (threadIdx.x == 0) {
+      double* gl = a.GL + (long long)p * K + i;
+      *gl = last ? t : *gl + t;
+    }
+    if (s == 1) return;
+    conv_lds(buf, tabs, G);
+    for_each_px(G, [&](int li, int gi, bool ok) {
+      float u = 0.f;
+      if (ok) {
+        const float cb = Cs[gi];
+        u = buf[li] * (-Ai[gi] / (cb * cb));
+      }
+      buf[li] = u;
+    });
+    conv_lds(buf, tabs, G);
+    const float* Bm = a.Bh + (long long)(s - 2) * a.stage + img;  // b^{s-1}
+    for_each_px(G, [&](int li, int gi, bool ok) {
+      if (ok) Rc[img + gi] = buf[li] * Bm[gi];
+    });
+  }
+}
+
+// GDsum[i][x] = sum_p GD[p][i][x] in FP64 (the per-histogram projection of
+// :1103-1108 is linear, so it is applied to the sum on the host).
+__global__ void k_reduce_gd(const float* __restrict__ GD, double* __restrict__ out, int P,
+                            long long KN) {
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < KN;
+       e += (long long)gridDim.x * blockDim.x) {
+    double acc = 0;
+    for (int p = 0; p < P; ++p) acc += GD[(long long)p * KN + e];
+    out[e] = acc;
+  }
+}
+
+__global__ void k_fill(float* __restrict__ dst, float v, long long n) {
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (long long)gridDim.x * blockDim.x)
+    dst[e] = v;
+}
+
+}  // namespace wdl
+
+using namespace wdl;
+
+struct wdl_solver {
+  wdl_ctx* ctx = nullptr;
+  int W = 0, H = 0, K = 0, P = 0, L = 0, loss = 0, flags = 0;
+  double gamma = 0;
+  long long N = 0;
+  int D = 0;
+  DevBuf tab, A, lam, obs, Bh, Ch, Fh, V0, V1, R0, R1, GD, GL, Q, G0, lossPart, bStore, bTemp,
+      GDsum;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool observed = false, problem = false, ran = false;
+  double scaleDic = 1.0;
+  int expWeight = 1;
+  std::vector<double> hostAtoms, hostLam;  // normalised FP64 copies
+
+  SolveArgs args(int p0, int np) const {
+    SolveArgs a;
+    a.G = make_geom(W, H, (flags & WDL_FLAG_DENSE) ? (1 << 20) : D);
+    a.K = K;
+    a.P = P;
+    a.L = L;
+    a.p0 = p0;
+    a.np = np;
+    a.loss = loss;
+    a.N = N;
+    a.stage = (long long)P * K * N;
+    a.tab = tab.as<float>();
+    a.A = A.as<float>();
+    a.lam = lam.as<float>();
+    a.obs = obs.as<float>();
+    a.Bh = Bh.as<float>();
+    a.Ch = Ch.as<float>();
+    a.Fh = Fh.as<float>();
+    a.V0 = V0.as<float>();
+    a.V1 = V1.as<float>();
+    a.R0 = R0.as<float>();
+    a.R1 = R1.as<float>();
+    a.GD = GD.as<float>();
+    a.GL = GL.as<double>();
+    a.Q = Q.as<float>();
+    a.G0 = G0.as<float>();
+    a.lossPart = lossPart.as<double>();
+    const bool warm = flags & WDL_FLAG_WARM_RESTART;
+    a.bInit = warm ? bStore.as<float>() : nullptr;
+    a.bSave = warm ? bTemp.as<float>() : nullptr;
+    return a;
+  }
+
+  int enqueue(int p0, int np, bool reduce) {
+    hipStream_t st = ctx->stream;
+    const SolveArgs a = args(p0, np);
+    const int lds = conv_lds_bytes(a.G);
+    const dim3 blk(kThreads);
+    const int gs = stage_grid(np, K);
+    for (int s = 1; s <= L; ++s)
+      hipLaunchKernelGGL(k_forward, dim3(gs), blk, lds, st, a, s);
+    const int fb = (int)std::min<long long>(kLossBlocks, (N + kThreads - 1) / kThreads);
+    // every lossPart slot of the histograms run is written (unused blocks write 0)
+    hipLaunchKernelGGL(k_final, dim3(kLossBlocks, np), blk, 0, st, a);
+    (void)fb;
+    for (int s = L; s >= 1; --s)
+      hipLaunchKernelGGL(k_backward, dim3(gs, 2), blk, lds, st, a, s);
+    if (reduce) {
+      const long long KN = (long long)K * N;
+      const int g = (int)std::min<long long>(1024, (KN + 255) / 256);
+      hipLaunchKernelGGL(k_reduce_gd, dim3(g), dim3(256), 0, st, GD.as<float>(),
+                         GDsum.as<double>(), P, KN);
+    }
+    WDL_HIP(hipGetLastError());
+    return WDL_OK;
+  }
+};
+
+static int set_lds_attr() {
+  static bool done = false;
+  if (done) return WDL_OK;
+  const int bytes = conv_lds_bytes(make_geom(kMaxSide, kMaxSide, 0));
+  WDL_HIP(hipFuncSetAttribute((const void*)k_forward, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              bytes));
+  WDL_HIP(hipFuncSetAttribute((const void*)k_backward, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              bytes));
+  done = true;
+  return WDL_OK;
+}
+
+static long long footprint(int W, int H, int K, int P, int L, int flags) {
+  const long long N = (long long)W * H, PKN = (long long)P * K * N;
+  long long f = 3LL * L * PKN * 4;           // Bh, Ch, Fh
+  f += 5 * PKN * 4;                          // V0, V1, R0, R1, GD
+  f += (long long)K * N * (4 + 8);           // A, GDsum
+  f += (long long)P * N * 4 * 3;             // obs, Q, G0
+  f += (long long)P * K * (4 + 8) + (long long)P * kLossBlocks * 8;
+  if (flags & WDL_FLAG_WARM_RESTART) f += 2 * PKN * 4;
+  f += kTabLen * 4;
+  return f;
+}
+
+extern "C" {
+
+long long wdl_solver_footprint(int W, int H, int K, int P, int L, int flags) {
+  if (W <= 0 || H <= 0 || K <= 0 || P <= 0 || L <= 0) return -1;
+  return footprint(W, H, K, P, L, flags);
+}
+
+int wdl_solver_create(wdl_ctx* ctx, int W, int H, int K, int P, int L, double gamma, int loss,
+                      int flags, wdl_solver** out) {
+  WDL_CHECK(ctx && out, WDL_ERR_ARG, "wdl_solver_create: NULL argument");
+  WDL_CHECK(W > 0 && H > 0 && K > 0 && P > 0 && L > 0, WDL_ERR_ARG,
+            "wdl_solver_create: W, H, K, P, L must be positive");
+  WDL_CHECK(gamma > 0 && std::isfinite(gamma), WDL_ERR_ARG,
+            "wdl_solver_create: gamma must be positive and finite");
+  WDL_CHECK(W <= kMaxSide && H <= kMaxSide, WDL_ERR_UNSUPPORTED,
+            "wdl_solver_create: sides above " + std::to_string(kMaxSide) + " not supported");
+  WDL_CHECK(loss == WDL_LOSS_QUADRATIC || loss == WDL_LOSS_KL || loss == WDL_LOSS_TV,
+            WDL_ERR_UNSUPPORTED, "wdl_solver_create: loss must be TV (1), quadratic (2) or KL (4)");
+  WDL_CHECK(!(flags & WDL_FLAG_LOG_DOMAIN), WDL_ERR_UNSUPPORTED,
+            "wdl_solver_create: log domain solver not available in this build");
+  WDL_HIP(hipSetDevice(ctx->device));
+  int st = set_lds_attr();
+  if (st) return st;
+  wdl_solver* s = new wdl_solver();
+  s->ctx = ctx;
+  s->W = W;
+  s->H = H;
+  s->K = K;
+  s->P = P;
+  s->L = L;
+  s->loss = loss;
+  s->flags = flags;
+  s->gamma = gamma;
+  s->N = (long long)W * H;
+  std::vector<float> tab;
+  s->D = build_table(W, H, gamma, tab);
+  const long long N = s->N, PKN = (long long)P * K * N;
+  const size_t f4 = sizeof(float);
+  struct {
+    DevBuf* b;
+    size_t n;
+  } allocs[] = {
+      {&s->tab, tab.size() * f4},
+      {&s->A, (size_t)K * N * f4},
+      {&s->lam, (size_t)P * K * f4},
+      {&s->obs, (size_t)P * N * f4},
+      {&s->Bh, (size_t)L * PKN * f4},
+      {&s->Ch, (size_t)L * PKN * f4},
+      {&s->Fh, (size_t)L * PKN * f4},
+      {&s->V0, (size_t)PKN * f4},
+      {&s->V1, (size_t)PKN * f4},
+      {&s->R0, (size_t)PKN * f4},
+      {&s->R1, (size_t)PKN * f4},
+      {&s->GD, (size_t)PKN * f4},
+      {&s->GL, (size_t)P * K * sizeof(double)},
+      {&s->Q, (size_t)P * N * f4},
+      {&s->G0, (size_t)P * N * f4},
+      {&s->lossPart, (size_t)P * kLossBlocks * sizeof(double)},
+      {&s->GDsum, (size_t)K * N * sizeof(double)},
+      {&s->bStore, (flags & WDL_FLAG_WARM_RESTART) ? (size_t)PKN * f4 : 0},
+      {&s->bTemp, (flags & WDL_FLAG_WARM_RESTART) ? (size_t)PKN * f4 : 0},
+  };
+  for (auto& al : allocs) {
+    if ((st = al.b->ensure(al.n))) {
+      wdl_solver_destroy(s);
+      return st;
+    }
+  }
+  hipError_t e = hipMemcpy(s->tab.p, tab.data(), tab.size() * f4, hipMemcpyHostToDevice);
+  if (e == hipSuccess && (flags & WDL_FLAG_WARM_RESTART)) {
+    // b_storage starts at 1 (inverseWasserstein.h:380-384).
+    hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, ctx->stream, s->bStore.as<float>(), 1.f,
+                       PKN);
+    e = hipStreamSynchronize(ctx->stream);
+  }
+  if (e != hipSuccess) {
+    wdl_solver_destroy(s);
+    set_error(std::string("wdl_solver_create: ") + hipGetErrorString(e));
+    return WDL_ERR_HIP;
+  }
+  *out = s;
+  return WDL_OK;
+}
+
+int wdl_solver_destroy(wdl_solver* s) {
+  if (!s) return WDL_OK;
+  (void)hipSetDevice(s->ctx->device);
+  (void)hipStreamSynchronize(s->ctx->stream);
+  if (s->exec) (void)hipGraphExecDestroy(s->exec);
+  if (s->graph) (void)hipGraphDestroy(s->graph);
+  DevBuf* bufs[] = {&s->tab, &s->A,  &s->lam, &s->obs, &s->Bh, &s->Ch,       &s->Fh,
+                    &s->V0,  &s->V1, &s->R0,  &s->R1,  &s->GD, &s->GL,       &s->Q,
+                    &s->G0,  &s->lossPart,    &s->bStore,      &s->bTemp,    &s->GDsum};
+  for (DevBuf* b : bufs) b->release();
+  delete s;
+  return WDL_OK;
+}
+
+int wdl_solver_set_observed(wdl_solver* s, const double* observed) {
+  WDL_CHECK(s && observed, WDL_ERR_ARG, "wdl_solver_set_observed: NULL argument");
+  std::vector<float> h((size_t)s->P * s->N);
+  for (size_t e = 0; e < h.size(); ++e) h[e] = (float)observed[e];
+  WDL_HIP(hipSetDevice(s->ctx->device));
+  WDL_HIP(hipMemcpyAsync(s->obs.p, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice,
+                         s->ctx->stream));
+  WDL_HIP(hipStreamSynchronize(s->ctx->stream));
+  s->observed = true;
+  return WDL_OK;
+}
+
+int wdl_solver_set_observed_device(wdl_solver* s, const float* d_observed) {
+  WDL_CHECK(s && d_observed, WDL_ERR_ARG, "wdl_solver_set_observed_device: NULL argument");
+  WDL_HIP(hipSetDevice(s->ctx->device));
+  WDL_HIP(hipMemcpyAsync(s->obs.p, d_observed, (size_t)s->P * s->N * sizeof(float),
+                         hipMemcpyDeviceToDevice, s->ctx->stream));
+  s->observed = true;
+  return WDL_OK;
+}
+
+int wdl_solver_set_problem(wdl_solver* s, const double* atoms, const double* lambdas) {
+  WDL_CHECK(s && atoms && lambdas, WDL_ERR_ARG, "wdl_solver_set_problem: NULL argument");
+  const int K = s->K, P = s->P;
+  const long long N = s->N;
+  // Problem::normalize_values (inverseWasserstein.h:53-74), per solve.
+  s->hostLam.assign(lambdas, lambdas + (size_t)P * K);
+  for (int p = 0; p < P; ++p) {
+    double t = 0;
+    for (int i = 0; i < K; ++i) t += s->hostLam[(size_t)p * K + i];
+    for (int i = 0; i < K; ++i) s->hostLam[(size_t)p * K + i] /= t;
+  }
+  s->hostAtoms.assign(atoms, atoms + (size_t)K * N);
+  for (int i = 0; i < K; ++i) {
+    double t = 0;
+    for (long long x = 0; x < N; ++x) t += s->hostAtoms[(size_t)i * N + x];
+    for (long long x = 0; x < N; ++x) s->hostAtoms[(size_t)i * N + x] /= t;
+  }
+  std::vector<float> ha(s->hostAtoms.begin(), s->hostAtoms.end());
+  std::vector<float> hl(s->hostLam.begin(), s->hostLam.end());
+  WDL_HIP(hipSetDevice(s->ctx->device));
+  WDL_HIP(hipMemcpyAsync(s->A.p, ha.data(), ha.size() * sizeof(float), hipMemcpyHostToDevice,
+                         s->ctx->stream));
+  WDL_HIP(hipMemcpyAsync(s->lam.p, hl.data(), hl.size() * sizeof(float), hipMemcpyHostToDevice,
+                         s->ctx->stream));
+  WDL_HIP(hipStreamSynchronize(s->ctx->stream));
+  s->problem = true;
+  return WDL_OK;
+}
+
+int wdl_solver_set_problem_device(wdl_solver* s, const float* d_atoms, const float* d_lambdas) {
+  WDL_CHECK(s && d_atoms && d_lambdas, WDL_ERR_ARG, "wdl_solver_set_problem_device: NULL argument");
+  WDL_HIP(hipSetDevice(s->ctx->device));
+  WDL_HIP(hipMemcpyAsync(s->A.p, d_atoms, (size_t)s->K * s->N * sizeof(float),
+                         hipMemcpyDeviceToDevice, s->ctx->stream));
+  WDL_HIP(hipMemcpyAsync(s->lam.p, d_lambdas, (size_t)s->P * s->K * sizeof(float),
+                         hipMemcpyDeviceToDevice, s->ctx->stream));
+  s->problem = true;
+  return WDL_OK;
+}
+
+int wdl_solver_set_warm_state(wdl_solver* s, const double* b_storage) {
+  WDL_CHECK(s && b_storage, WDL_ERR_ARG, "wdl_solver_set_warm_state: NULL argument");
+  WDL_CHECK(s->flags & WDL_FLAG_WARM_RESTART, WDL_ERR_STATE,
+            "wdl_solver_set_warm_state: solver created without WDL_FLAG_WARM_RESTART");
+  const size_t n = (size_t)s->P * s->K * s->N;
+  std::vector<float> h(n);
+  for (size_t e = 0; e < n; ++e) h[e] = (float)b_storage[e];
+  WDL_HIP(hipSetDevice(s->ctx->device));
+  WDL_HIP(hipMemcpyAsync(s->bStore.p, h.data(), n * sizeof(float), hipMemcpyHostToDevice,
+                         s->ctx->stream));
+  WDL_HIP(hipStreamSynchronize(s->ctx->stream));
+  return WDL_OK;
+}
+
+int wdl_solver_get_warm_state(wdl_solver* s, double* b_temp) {
+  WDL_CHECK(s && b_temp, WDL_ERR_ARG, "wdl_solver_get_warm_state: NULL argument");
+  WDL_CHECK(s->flags & WDL_FLAG_WARM_RESTART, WDL_ERR_STATE,
+            "wdl_solver_get_warm_state: solver created without WDL_FLAG_WARM_RESTART");
+  const size_t n = (size_t)s->P * s->K * s->N;
+  std::vector<float> h(n);
+  WDL_HIP(hipSetDevice(s->ctx->device));
+  WDL_HIP(hipMemcpyAsync(h.data(), s->bTemp.p, n * sizeof(float), hipMemcpyDeviceToHost,
+                         s->ctx->stream));
+  WDL_HIP(hipStreamSynchronize(s->ctx->stream));
+  for (size_t e = 0; e < n; ++e) b_temp[e] = h[e];
+  return WDL_OK;
+}
+
+int wdl_solver_warm_commit(wdl_solver* s) {
+  WDL_CHECK(s, WDL_ERR_ARG, "wdl_solver_warm_commit: NULL argument");
+  WDL_CHECK(s->flags & WDL_FLAG_WARM_RESTART, WDL_ERR_STATE,
+            "wdl_solver_warm_commit: solver created without WDL_FLAG_WARM_RESTART");
+  WDL_HIP(hipSetDevice(s->ctx->device));
+  WDL_HIP(hipMemcpyAsync(s->bStore.p, s->bTemp.p, (size_t)s->P * s->K * s->N * sizeof(float),
+                         hipMemcpyDeviceToDevice, s->ctx->stream));
+  return WDL_OK;
+}
+
+int wdl_solver_run(wdl_solver* s, int mode) {
+  WDL_CHECK(s, WDL_ERR_ARG, "wdl_solver_run: NULL argument");
+  WDL_CHECK(s->observed && s->problem, WDL_ERR_STATE,
+            "wdl_solver_run: set_observed and set_problem must be called first");
+  WDL_CHECK(mode == WDL_RUN_BATCHED || mode == WDL_RUN_PLAIN, WDL_ERR_ARG,
+            "wdl_solver_run: unknown mode");
+  WDL_HIP(hipSetDevice(s->ctx->device));
+  int st;
+  if (mode == WDL_RUN_PLAIN) {
+    for (int p = 0; p < s->P; ++p)
+      if ((st = s->enqueue(p, 1, p == s->P - 1))) return st;
+  } else if (s->flags & WDL_FLAG_NO_GRAPH) {
+    if ((st = s->enqueue(0, s->P, true))) return st;
+  } else {
+    if (!s->exec) {
+      WDL_HIP(hipStreamBeginCapture(s->ctx->stream, hipStreamCaptureModeThreadLocal));
+      st = s->enqueue(0, s->P, true);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(s->ctx->stream, &g);
+      if (st) return st;
+      if (e != hipSuccess) {
+        set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        return WDL_ERR_HIP;
+      }
+      s->graph = g;
+      WDL_HIP(hipGraphInstantiate(&s->exec, s->graph, nullptr, nullptr, 0));
+    }
+    WDL_HIP(hipGraphLaunch(s->exec, s->ctx->stream));
+  }
+  s->ran = true;
+  return WDL_OK;
+}
+
+int wdl_solver_run_stage(wdl_solver* s, int stage, int step) {
+  WDL_CHECK(s, WDL_ERR_ARG, "wdl_solver_run_stage: NULL argument");
+  WDL_CHECK(s->ran, WDL_ERR_STATE, "wdl_solver_run_stage: run the solver once first");
+  WDL_CHECK(step >= 1 && step <= s->L, WDL_ERR_ARG, "wdl_solver_run_stage: step out of 1..L");
+  WDL_CHECK(stage == WDL_STAGE_FORWARD || stage == WDL_STAGE_BACKWARD, WDL_ERR_ARG,
+            "wdl_solver_run_stage: unknown stage");
+  WDL_HIP(hipSetDevice(s->ctx->device));
+  const SolveArgs a = s->args(0, s->P);
+  const int lds = conv_lds_bytes(a.G);
+  const int gs = stage_grid(s->P, s->K);
+  if (stage == WDL_STAGE_FORWARD)
+    hipLaunchKernelGGL(k_forward, dim3(gs), dim3(kThreads), lds, s->ctx->stream, a, step);
+  else
+    hipLaunchKernelGGL(k_backward, dim3(gs, 2), dim3(kThreads), lds, s->ctx->stream, a, step);
+  WDL_HIP(hipGetLastError());
+  return WDL_OK;
+}
+
+int wdl_solver_get_results(wdl_solver* s, double* loss, double* grad_lambda, double* grad_dic_sum,
+                           double* barycenters) {
+  WDL_CHECK(s, WDL_ERR_ARG, "wdl_solver_get_results: NULL argument");
+  WDL_CHECK(s->ran, WDL_ERR_STATE, "wdl_solver_get_results: nothing has run");
+  WDL_HIP(hipSetDevice(s->ctx->device));
+  hipStream_t st = s->ctx->stream;
+  const int P = s->P, K = s->K;
+  const long long N = s->N;
+  std::vector<double> lp;
+  std::vector<float> q;
+  if (loss) {
+    lp.resize((size_t)P * kLossBlocks);
+    WDL_HIP(hipMemcpyAsync(lp.data(), s->lossPart.p, lp.size() * 8, hipMemcpyDeviceToHost, st));
+  }
+  if (grad_lambda)
+    WDL_HIP(hipMemcpyAsync(grad_lambda, s->GL.p, (size_t)P * K * 8, hipMemcpyDeviceToHost, st));
+  if (grad_dic_sum)
+    WDL_HIP(hipMemcpyAsync(grad_dic_sum, s->GDsum.p, (size_t)K * N * 8, hipMemcpyDeviceToHost, st));
+  if (barycenters) {
+    q.resize((size_t)P * N);
+    WDL_HIP(hipMemcpyAsync(q.data(), s->Q.p, q.size() * 4, hipMemcpyDeviceToHost, st));
+  }
+  WDL_HIP(hipStreamSynchronize(st));
+  if (loss)
+    for (int p = 0; p < P; ++p) {
+      double t = 0;
+      for (int b = 0; b < kLossBlocks; ++b) t += lp[(size_t)p * kLossBlocks + b];
+      loss[p] = t;
+    }
+  if (barycenters)
+    for (size_t e = 0; e < q.size(); ++e) barycenters[e] = q[e];
+  return WDL_OK;
+}
+
+int wdl_solver_get_grad_dic(wdl_solver* s, double* grad_dic) {
+  WDL_CHECK(s && grad_dic, WDL_ERR_ARG, "wdl_solver_get_grad_dic: NULL argument");
+  WDL_CHECK(s->ran, WDL_ERR_STATE, "wdl_solver_get_grad_dic: nothing has run");
+  const size_t n = (size_t)s->P * s->K * s->N;
+  std::vector<float> h(n);
+  WDL_HIP(hipSetDevice(s->ctx->device));
+  WDL_HIP(hipMemcpyAsync(h.data(), s->GD.p, n * 4, hipMemcpyDeviceToHost, s->ctx->stream));
+  WDL_HIP(hipStreamSynchronize(s->ctx->stream));
+  for (size_t e = 0; e < n; ++e) grad_dic[e] = h[e];
+  return WDL_OK;
+}
+
+int wdl_gradient_both(wdl_solver* s, const double* atoms, const double* lambdas, double* loss,
+                      double* grad_lambda, double* grad_dic_sum, double* barycenters) {
+  int st = wdl_solver_set_problem(s, atoms, lambdas);
+  if (st) return st;
+  if ((st = wdl_solver_run(s, WDL_RUN_BATCHED))) return st;
+  return wdl_solver_get_results(s, loss, grad_lambda, grad_dic_sum, barycenters);
+}
+
+int wdl_solver_set_scale(wdl_solver* s, double scale_dictionary, int exp_weight) {
+  WDL_CHECK(s, WDL_ERR_ARG, "wdl_solver_set_scale: NULL argument");
+  WDL_CHECK(scale_dictionary != 0 && std::isfinite(scale_dictionary), WDL_ERR_ARG,
+            "wdl_solver_set_scale: scale_dictionary must be finite and non-zero");
+  s->scaleDic = scale_dictionary;
+  s->expWeight = exp_weight ? 1 : 0;
+  return WDL_OK;
+}
+
+double wdl_evaluate_both(void* instance, const double* x, double* g, int n, double step) {
+  (void)step;
+  const double nan = std::numeric_limits<double>::quiet_NaN();
+  wdl_solver* s = static_cast<wdl_solver*>(instance);
+  if (!s || !x || !g) {
+    set_error("wdl_evaluate_both: NULL argument");
+    return nan;
+  }
+  const int P = s->P, K = s->K;
+  const long long N = s->N;
+  if ((long long)n != (long long)P * K + (long long)K * N) {
+    set_error("wdl_evaluate_both: n != P*K + K*N");
+    return nan;
+  }
+  const double scale = s->scaleDic;
+  // Back to the original domain and normalised (inverseWasserstein.h:998-1031).
+  std::vector<double> v2(x, x + n);
+  for (int e = 0; e < P * K; ++e) v2[e] = s->expWeight ? std::exp(x[e] / scale) : x[e] / scale;
+  for (long long e = 0; e < (long long)K * N; ++e) v2[P * K + e] = std::exp(x[P * K + e]);
+  for (int p = 0; p < P; ++p) {
+    double t = 0;
+    for (int i = 0; i < K; ++i) t += v2[p * K + i];
+    for (int i = 0; i < K; ++i) v2[p * K + i] /= t;
+  }
+  for (int i = 0; i < K; ++i) {
+    double t = 0;
+    for (long long j = 0; j < N; ++j) t += v2[P * K + i * N + j];
+    for (long long j = 0; j < N; ++j) v2[P * K + i * N + j] /= t;
+  }
+  std::vector<double> lossv(P), gl((size_t)P * K), gd((size_t)K * N);
+  if (wdl_gradient_both(s, &v2[P * K], &v2[0], lossv.data(), gl.data(), gd.data(), nullptr))
+    return nan;
+  double f = 0;
+  for (int p = 0; p < P; ++p) f += lossv[p];  // :1089-1091
+  // Simplex-tangent projections (:1095-1108); the atom projection is linear in
+  // the per-histogram gradients, so it is applied to their sum (:1110-1112).
+  for (int p = 0; p < P; ++p) {
+    double d = 0;
+    for (int i = 0; i < K; ++i) d += v2[p * K + i] * gl[(size_t)p * K + i];
+    for (int i = 0; i < K; ++i) g[p * K + i] = gl[(size_t)p * K + i] - d;
+  }
+  for (int i = 0; i < K; ++i) {
+    double d = 0;
+    for (long long j = 0; j < N; ++j) d += v2[P * K + i * N + j] * gd[(size_t)i * N + j];
+    for (long long j = 0; j < N; ++j) g[P * K + i * N + j] = gd[(size_t)i * N + j] - d;
+  }
+  // Chain rule of the exp / scaling reparametrisation (:1125-1136).
+  if (s->expWeight) {
+    for (int e = 0; e < n; ++e) g[e] *= v2[e];
+  } else {
+    for (long long e = P * K; e < n; ++e) g[e] *= v2[e];
+  }
+  for (int e = 0; e < P * K; ++e) g[e] /= scale;
+  return f;
+}
+
+}  // extern "C"

@@ -1,0 +1,79 @@
+"""Synthetic inputs for the benchmark workloads (histograms, atoms, weights).
+
+Blob histograms: each is a sum of isotropic Gaussian blobs (centres uniform
+in the grid, sigma and amplitude uniform in stated ranges) plus a floor,
+normalised to sum 1, on a pixel-unit grid.  Curve histograms: polylines of
+random segments rasterised with a Gaussian profile.  Atoms and weights are
+uniform(0,1), normalised -- the initialisation regress_both receives
+(main_dictionary_learning.cpp:142-159 with random atoms).  All draws come
+from one numpy PCG64 stream per seed: histograms, then atoms, then weights.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def blob_histograms(rng, n, count, nblobs=(1, 3), sigma=(3.0, 8.0), floor=1e-6):
+    y, x = np.mgrid[0:n, 0:n].astype(np.float64)
+    out = np.empty((count, n * n))
+    for c in range(count):
+        h = np.zeros((n, n))
+        for _ in range(int(rng.integers(nblobs[0], nblobs[1] + 1))):
+            cx, cy = rng.uniform(0, n, 2)
+            s = rng.uniform(*sigma)
+            a = rng.uniform(0.5, 1.0)
+            h += a * np.exp(-((x - cx) ** 2 + (y - cy) ** 2) / (2 * s * s))
+        h += floor
+        out[c] = (h / h.sum()).ravel()
+    return out
+
+
+def curve_histograms(rng, n, count, segments=(3, 6), width=1.5, floor=1e-8):
+    y, x = np.mgrid[0:n, 0:n].astype(np.float64)
+    out = np.empty((count, n * n))
+    for c in range(count):
+        nseg = int(rng.integers(segments[0], segments[1] + 1))
+        pts = rng.uniform(0, n, (nseg + 1, 2))
+        h = np.zeros((n, n))
+        for a, b in zip(pts[:-1], pts[1:]):
+            d = b - a
+            t = np.clip(((x - a[0]) * d[0] + (y - a[1]) * d[1]) / max(float(d @ d), 1e-12), 0, 1)
+            dist2 = (x - a[0] - t * d[0]) ** 2 + (y - a[1] - t * d[1]) ** 2
+            h = np.maximum(h, np.exp(-dist2 / (2 * width * width)))
+        h += floor
+        out[c] = (h / h.sum()).ravel()
+    return out
+
+
+def init_atoms_weights(rng, K, P, N):
+    atoms = rng.uniform(0, 1, (K, N))
+    atoms /= atoms.sum(axis=1, keepdims=True)
+    w = rng.uniform(0, 1, (P, K))
+    w /= w.sum(axis=1, keepdims=True)
+    return atoms, w
+
+
+# Workloads (sizes of the benchmark configurations).
+CONFIGS = {
+    "plain_small": dict(n=64, P=16, K=4, gamma=2.0, L=25, loss=2, nblobs=(1, 3), sigma=(3, 8),
+                        scale_factor=100.0),
+    "batched_std": dict(n=128, P=64, K=5, gamma=1.0, L=100, loss=2, nblobs=(1, 4),
+                        sigma=(4, 16), scale_factor=100.0),
+    "log_small_eps": dict(n=256, P=32, K=6, gamma=0.5, L=200, loss=4, curves=True,
+                          scale_factor=100.0),
+}
+
+
+def make_problem(name: str, seed: int | None = None, P: int | None = None):
+    c = dict(CONFIGS[name])
+    if P is not None:
+        c["P"] = P
+    rng = np.random.default_rng(20260000 + list(CONFIGS).index(name) if seed is None else seed)
+    n, Pn, K = c["n"], c["P"], c["K"]
+    if c.get("curves"):
+        obs = curve_histograms(rng, n, Pn)
+    else:
+        obs = blob_histograms(rng, n, Pn, c["nblobs"], c["sigma"])
+    atoms, w = init_atoms_weights(rng, K, Pn, n * n)
+    c.update(obs=obs, atoms=atoms, weights=w)
+    return c
