@@ -179,9 +179,24 @@ def main():
     e1.record(stream)
     e1.synchronize()
     fwd_ms = e0.elapsed_ms(e1) / reps
-    kern = wdl.HipGaussian2DKernel(ctx, g, n, n)
+    kern = wdl.HipGaussian2DKernel(ctx, g, n, n, wdl.FLAG_DENSE if args.dense else 0)
     D = 1 << 20 if args.dense else band_halfwidth(kern.kernel1d())
     f_exec, f_dense = conv_flops(n, D)
+    # The bare convolution kernel (no fused elementwise work) on one backward
+    # stage's worth of images, for the MFMA efficiency of the conv alone.
+    nimg = 4 * P * K
+    dbuf_in = _hip.DeviceArray((nimg, n, n))
+    dbuf_out = _hip.DeviceArray((nimg, n, n))
+    dbuf_in.upload(np.random.default_rng(0).uniform(0, 1, (nimg, n, n)))
+    kern.convolve_device(dbuf_in.ptr, dbuf_out.ptr, nimg)
+    e0.record(stream)
+    for _ in range(reps):
+        kern.convolve_device(dbuf_in.ptr, dbuf_out.ptr, nimg)
+    e1.record(stream)
+    e1.synchronize()
+    conv_ms = e0.elapsed_ms(e1) / reps
+    dbuf_in.free()
+    dbuf_out.free()
     convs_bwd = 4 * P * K
     achieved = convs_bwd * f_exec / (bwd_ms * 1e-3) / 1e12
     # algorithmic HBM bytes of one backward launch (FP32): per (p,i) atom chain
@@ -200,6 +215,8 @@ def main():
         "hbm_algorithmic_bytes_per_launch": bytes_bwd,
         "hbm_achieved_gbs": round(bytes_bwd / (bwd_ms * 1e-3) / 1e9, 1),
         "band_halfwidth": D,
+        "conv_only": {"images": nimg, "launch_ms": round(conv_ms, 4),
+                      "executed_tflops": round(nimg * f_exec / (conv_ms * 1e-3) / 1e12, 2)},
     }
     log("backward launch %.4f ms -> %.1f TFLOP/s executed" % (bwd_ms, achieved))
     whole_flops = P * (6 * L - 3) * K * f_dense
@@ -225,8 +242,9 @@ def main():
         cb, err = cpu_baseline(c, res)
         line["cpu_baseline"] = cb
         line["err_vs_fp64"] = err
-    s.close()
-    ctx.close()
+    for e in ev + [e0, e1]:
+        e.destroy()
+    ctx.close()  # closes the solver and the kernel object first
     print(json.dumps(line), flush=True)
 
 

@@ -10,6 +10,8 @@ from __future__ import annotations
 
 import ctypes
 import os
+import sys
+import weakref
 
 import numpy as np
 
@@ -104,6 +106,7 @@ class Context:
         _check(lib().wdl_ctx_create(device, ctypes.byref(h)), "wdl_ctx_create")
         self.handle = h
         self.device = device
+        self._children = weakref.WeakSet()
 
     @property
     def stream(self) -> int:
@@ -113,15 +116,19 @@ class Context:
         _check(lib().wdl_ctx_synchronize(self.handle), "wdl_ctx_synchronize")
 
     def close(self):
+        """Destroys the objects created on this context, then the context."""
         if self.handle:
+            for child in list(self._children):
+                child.close()
             lib().wdl_ctx_destroy(self.handle)
             self.handle = None
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        if not sys.is_finalizing():
+            try:
+                self.close()
+            except Exception:
+                pass
 
 
 class HipGaussian2DKernel:
@@ -136,6 +143,7 @@ class HipGaussian2DKernel:
                "wdl_gauss2d_create")
         self.handle, self.ctx, self.gamma, self.W, self.H = h, ctx, float(gamma), W, H
         self.N = W * H
+        ctx._children.add(self)
 
     def kernel1d(self):
         out = np.zeros(max(self.W, self.H))
@@ -184,10 +192,11 @@ class HipGaussian2DKernel:
             self.handle = None
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        if not sys.is_finalizing():
+            try:
+                self.close()
+            except Exception:
+                pass
 
 
 class SinkhornSolver:
@@ -200,6 +209,7 @@ class SinkhornSolver:
         self.handle, self.ctx = h, ctx
         self.W, self.H, self.K, self.P, self.L = W, H, K, P, L
         self.N = W * H
+        ctx._children.add(self)
 
     @staticmethod
     def footprint(W, H, K, P, L, flags=0) -> int:
@@ -276,7 +286,8 @@ class SinkhornSolver:
             self.handle = None
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        if not sys.is_finalizing():
+            try:
+                self.close()
+            except Exception:
+                pass

@@ -6,6 +6,7 @@ Loaded by soname after libwdl.so, so it is the same runtime instance.
 from __future__ import annotations
 
 import ctypes
+import sys
 
 import numpy as np
 
@@ -74,10 +75,11 @@ class DeviceArray:
             self.ptr = None
 
     def __del__(self):
-        try:
-            self.free()
-        except Exception:
-            pass
+        if not sys.is_finalizing():
+            try:
+                self.free()
+            except Exception:
+                pass
 
 
 class Event:
@@ -97,11 +99,17 @@ class Event:
         _ok(rt().hipEventElapsedTime(ctypes.byref(ms), self.h, end.h), "hipEventElapsedTime")
         return float(ms.value)
 
-    def __del__(self):
-        try:
+    def destroy(self):
+        if self.h:
             rt().hipEventDestroy(self.h)
-        except Exception:
-            pass
+            self.h = None
+
+    def __del__(self):
+        if not sys.is_finalizing():
+            try:
+                self.destroy()
+            except Exception:
+                pass
 
 
 def synchronize():

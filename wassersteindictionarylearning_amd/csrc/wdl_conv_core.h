@@ -151,9 +151,75 @@ __device__ inline void for_each_px(const ConvGeom& G, F&& f) {
       f(row * G.S1 + col, row * G.W + col, row < G.H && col < G.W);
 }
 
+// ---- vectorised elementwise passes -------------------------------------------
+// Elementwise steps touch VW consecutive pixels per call (VW = 4 when W % 4 == 0,
+// so a group never straddles a row, else 1).  Each thread first issues the
+// global loads of kUnroll groups, then applies them: the loads of one thread
+// are in flight together instead of one dependent chain at a time.
+template <int VW> struct Vec;
+template <> struct Vec<1> { typedef float T; };
+template <> struct Vec<4> { typedef floatx4 T; };
+
+template <int VW>
+__device__ inline typename Vec<VW>::T vld(const float* p) {
+  return *reinterpret_cast<const typename Vec<VW>::T*>(p);
+}
+template <int VW>
+__device__ inline void vst(float* p, typename Vec<VW>::T v) {
+  *reinterpret_cast<typename Vec<VW>::T*>(p) = v;
+}
+template <int VW>
+__device__ inline typename Vec<VW>::T vsplat(float x) {
+  return typename Vec<VW>::T(x);
+}
+
+template <class F> __device__ inline float vmap(float x, F f) { return f(x); }
+template <class F> __device__ inline floatx4 vmap(floatx4 x, F f) {
+  return floatx4{f(x.x), f(x.y), f(x.z), f(x.w)};
+}
+__device__ inline double vdot_d(float a, float b) { return (double)a * (double)b; }
+__device__ inline double vdot_d(floatx4 a, floatx4 b) {
+  return (double)a.x * b.x + (double)a.y * b.y + (double)a.z * b.z + (double)a.w * b.w;
+}
+
+constexpr int kUnroll = 4;
+
+// load(gi) -> any value type; apply(li, gi, valid, value).  li / gi are the
+// LDS and pixel indices of the group's first pixel.
+template <int VW, class Load, class Apply>
+__device__ inline void px_pass(const ConvGeom& G, Load&& load, Apply&& apply) {
+  const int q = G.Wp / VW;
+  const int total = G.Hp * q;
+  typedef decltype(load(0)) V;
+  for (int base = threadIdx.x; base < total; base += kThreads * kUnroll) {
+    V v[kUnroll];
+    int li[kUnroll], gi[kUnroll];
+    bool ok[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int e = base + u * kThreads;
+      const int row = e / q, c = (e - row * q) * VW;
+      li[u] = row * G.S1 + c;
+      gi[u] = row * G.W + c;
+      ok[u] = e < total && row < G.H && c < G.W;
+      if (ok[u]) v[u] = load(gi[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u)
+      if (base + u * kThreads < total) apply(li[u], gi[u], ok[u], v[u]);
+  }
+}
+
 // safe_log (loss.h:3-5) in FP32: log(max(1e-250, x)); 1e-250 is below the
 // FP32 range, so every x <= 0 (and NaN, as std::max returns its first
 // argument) maps to log(1e-250).
 __device__ inline float safe_logf(float x) { return x > 0.f ? logf(x) : -575.64627324851142f; }
+__device__ inline floatx4 safe_logf(floatx4 x) {
+  return floatx4{safe_logf(x.x), safe_logf(x.y), safe_logf(x.z), safe_logf(x.w)};
+}
+__device__ inline float vexpf(float x) { return expf(x); }
+__device__ inline floatx4 vexpf(floatx4 x) { return floatx4{expf(x.x), expf(x.y), expf(x.z), expf(x.w)}; }
+__device__ inline float vlogf(float x) { return logf(x); }
+__device__ inline floatx4 vlogf(floatx4 x) { return floatx4{logf(x.x), logf(x.y), logf(x.z), logf(x.w)}; }
 
 }  // namespace wdl

@@ -103,7 +103,9 @@ __device__ inline double block_sum(double v) {
 
 // Forward iteration s (1..L): b^{s-1} (geometric-mean step of iteration s-1,
 // :577-592), c^{s-1} = K b^{s-1} (:568), a = p / c (:569-574), phi^s = K a (:575).
+template <int VW>
 __global__ __launch_bounds__(kThreads, 2) void k_forward(SolveArgs a, int s) {
+  typedef typename Vec<VW>::T V;
   int p, i;
   if (!map_block(a, p, i)) return;
   extern __shared__ __align__(16) float smem[];
@@ -112,46 +114,64 @@ __global__ __launch_bounds__(kThreads, 2) void k_forward(SolveArgs a, int s) {
   load_tables(tabs, a.tab);
   const ConvGeom G = a.G;
   const long long N = a.N;
-  const long long hbase = (long long)p * a.K * N;
+  const int K = a.K;
+  const long long hbase = (long long)p * K * N;
   const long long img = hbase + (long long)i * N;
   if (s == 1) {
     const float* bi = a.bInit ? a.bInit + img : nullptr;
-    for_each_px(G, [&](int li, int gi, bool ok) { buf[li] = ok ? (bi ? bi[gi] : 1.f) : 0.f; });
+    px_pass<VW>(
+        G, [&](int gi) { return bi ? vld<VW>(bi + gi) : vsplat<VW>(1.f); },
+        [&](int li, int gi, bool ok, V b) { vst<VW>(buf + li, ok ? b : vsplat<VW>(0.f)); });
   } else {
     const float* Fp = a.Fh + (long long)(s - 2) * a.stage + hbase;  // phi^{s-1}
     float* Bd = a.Bh + (long long)(s - 2) * a.stage + img;          // b^{s-1}
-    const float* lam = a.lam + (long long)p * a.K;
-    const int K = a.K;
-    for_each_px(G, [&](int li, int gi, bool ok) {
-      float b = 0.f;
-      if (ok) {
-        float lq = 0.f;
-        for (int j = 0; j < K; ++j) lq += lam[j] * safe_logf(Fp[(long long)j * N + gi]);
-        b = expf(lq) / Fp[(long long)i * N + gi];
-        Bd[gi] = b;
-      }
-      buf[li] = b;
-    });
+    const float* lam = a.lam + (long long)p * K;
+    struct LdF { V lq, fi; };
+    px_pass<VW>(
+        G,
+        [&](int gi) {
+          LdF r;
+          r.lq = vsplat<VW>(0.f);
+          r.fi = vsplat<VW>(1.f);
+          for (int j = 0; j < K; ++j) {
+            const V f = vld<VW>(Fp + (long long)j * N + gi);
+            r.lq += lam[j] * safe_logf(f);
+            if (j == i) r.fi = f;
+          }
+          return r;
+        },
+        [&](int li, int gi, bool ok, const LdF& r) {
+          V b = vsplat<VW>(0.f);
+          if (ok) {
+            b = vexpf(r.lq) / r.fi;
+            vst<VW>(Bd + gi, b);
+          }
+          vst<VW>(buf + li, b);
+        });
   }
   conv_lds(buf, tabs, G);
   {
     float* Cd = a.Ch + (long long)(s - 1) * a.stage + img;
     const float* Ai = a.A + (long long)i * N;
-    for_each_px(G, [&](int li, int gi, bool ok) {
-      float u = 0.f;
-      if (ok) {
-        const float c = buf[li];
-        Cd[gi] = c;
-        u = Ai[gi] / c;
-      }
-      buf[li] = u;
-    });
+    px_pass<VW>(
+        G, [&](int gi) { return vld<VW>(Ai + gi); },
+        [&](int li, int gi, bool ok, V av) {
+          V u = vsplat<VW>(0.f);
+          if (ok) {
+            const V c = vld<VW>(buf + li);
+            vst<VW>(Cd + gi, c);
+            u = av / c;
+          }
+          vst<VW>(buf + li, u);
+        });
   }
   conv_lds(buf, tabs, G);
   float* Fd = a.Fh + (long long)(s - 1) * a.stage + img;
-  for_each_px(G, [&](int li, int gi, bool ok) {
-    if (ok) Fd[gi] = buf[li];
-  });
+  px_pass<VW>(
+      G, [&](int gi) { return 0; },
+      [&](int li, int gi, bool ok, int) {
+        if (ok) vst<VW>(Fd + gi, vld<VW>(buf + li));
+      });
 }
 
 // After iteration L: q = exp(sum_i lam_i safe_log phi_i^L), b^L = q / phi^L,
@@ -187,7 +207,9 @@ __global__ __launch_bounds__(kThreads) void k_final(SolveArgs a) {
 
 // Backward step s (L..1).  blockIdx.y = 0: dictionary gradient (:600-629);
 // blockIdx.y = 1: weight gradient (:635-673).
+template <int VW>
 __global__ __launch_bounds__(kThreads, 2) void k_backward(SolveArgs a, int s) {
+  typedef typename Vec<VW>::T V;
   int p, i;
   if (!map_block(a, p, i)) return;
   extern __shared__ __align__(16) float smem[];
@@ -203,46 +225,63 @@ __global__ __launch_bounds__(kThreads, 2) void k_backward(SolveArgs a, int s) {
   const float* Ai = a.A + (long long)i * N;
   const float* Cs = a.Ch + (long long)(s - 1) * a.stage + img;  // c^{s-1}
   const bool last = (s == a.L);
+  struct Two { V x, y; };
+  struct Three { V x, y, z; };
   if (blockIdx.y == 0) {
     const float* Vp = (s & 1) ? a.V0 : a.V1;
     float* Vc = (s & 1) ? a.V1 : a.V0;
     const float* Bs = a.Bh + (long long)(s - 1) * a.stage + img;  // b^s
     const float* G0 = a.G0 + (long long)p * N;
-    for_each_px(G, [&](int li, int gi, bool ok) {
-      float t = 0.f;
-      if (ok) {
-        float n, v;
-        if (last) {
-          n = G0[gi];
-          v = 0.f;
-        } else {
-          n = 0.f;
-          for (int j = 0; j < K; ++j) n += Vp[hbase + (long long)j * N + gi];
-          v = Vp[img + gi];
-        }
-        t = (lpi * n - v) * Bs[gi];
-      }
-      buf[li] = t;
-    });
+    px_pass<VW>(
+        G,
+        [&](int gi) {
+          Three r;  // n, v_i, b^s
+          r.z = vld<VW>(Bs + gi);
+          if (last) {
+            r.x = vld<VW>(G0 + gi);
+            r.y = vsplat<VW>(0.f);
+          } else {
+            r.x = vsplat<VW>(0.f);
+            r.y = vsplat<VW>(0.f);
+            for (int j = 0; j < K; ++j) {
+              const V v = vld<VW>(Vp + hbase + (long long)j * N + gi);
+              r.x += v;
+              if (j == i) r.y = v;
+            }
+          }
+          return r;
+        },
+        [&](int li, int gi, bool ok, const Three& r) {
+          vst<VW>(buf + li, ok ? (lpi * r.x - r.y) * r.z : vsplat<VW>(0.f));
+        });
     conv_lds(buf, tabs, G);
     float* GDp = a.GD + img;
-    for_each_px(G, [&](int li, int gi, bool ok) {
-      float t = 0.f;
-      if (ok) {
-        const float c = buf[li];
-        const float cb = Cs[gi];
-        const float gd = c / cb;
-        GDp[gi] = last ? gd : GDp[gi] + gd;
-        t = -Ai[gi] * c / (cb * cb);
-      }
-      buf[li] = t;
-    });
+    px_pass<VW>(
+        G,
+        [&](int gi) {
+          Three r;  // conv_b, resultDic so far, atom
+          r.x = vld<VW>(Cs + gi);
+          r.y = last ? vsplat<VW>(0.f) : vld<VW>(GDp + gi);
+          r.z = vld<VW>(Ai + gi);
+          return r;
+        },
+        [&](int li, int gi, bool ok, const Three& r) {
+          V t = vsplat<VW>(0.f);
+          if (ok) {
+            const V c = vld<VW>(buf + li);
+            vst<VW>(GDp + gi, r.y + c / r.x);
+            t = -r.z * c / (r.x * r.x);
+          }
+          vst<VW>(buf + li, t);
+        });
     if (s == 1) return;
     conv_lds(buf, tabs, G);
     const float* Fm = a.Fh + (long long)(s - 2) * a.stage + img;  // phi^{s-1}
-    for_each_px(G, [&](int li, int gi, bool ok) {
-      if (ok) Vc[img + gi] = buf[li] / Fm[gi];
-    });
+    px_pass<VW>(
+        G, [&](int gi) { return vld<VW>(Fm + gi); },
+        [&](int li, int gi, bool ok, V f) {
+          if (ok) vst<VW>(Vc + img + gi, vld<VW>(buf + li) / f);
+        });
   } else {
     const float* Rp = (s & 1) ? a.R0 : a.R1;
     float* Rc = (s & 1) ? a.R1 : a.R0;
@@ -250,24 +289,33 @@ __global__ __launch_bounds__(kThreads, 2) void k_backward(SolveArgs a, int s) {
     const float* G0 = a.G0 + (long long)p * N;
     const float* Qp = a.Q + (long long)p * N;
     double dot = 0;
-    for_each_px(G, [&](int li, int gi, bool ok) {
-      float u = 0.f;
-      if (ok) {
-        float g, r;
-        if (last) {
-          g = G0[gi] * Qp[gi];
-          r = 0.f;
-        } else {
-          g = 0.f;
-          for (int j = 0; j < K; ++j) g += Rp[hbase + (long long)j * N + gi];
-          r = Rp[img + gi];
-        }
-        const float f = Fs[gi];
-        dot += (double)logf(f) * (double)g;  // plain log (:649)
-        u = (lpi * g - r) / f;
-      }
-      buf[li] = u;
-    });
+    px_pass<VW>(
+        G,
+        [&](int gi) {
+          Three r;  // g, r_i, phi^s
+          r.z = vld<VW>(Fs + gi);
+          if (last) {
+            r.x = vld<VW>(G0 + gi) * vld<VW>(Qp + gi);
+            r.y = vsplat<VW>(0.f);
+          } else {
+            r.x = vsplat<VW>(0.f);
+            r.y = vsplat<VW>(0.f);
+            for (int j = 0; j < K; ++j) {
+              const V v = vld<VW>(Rp + hbase + (long long)j * N + gi);
+              r.x += v;
+              if (j == i) r.y = v;
+            }
+          }
+          return r;
+        },
+        [&](int li, int gi, bool ok, const Three& r) {
+          V u = vsplat<VW>(0.f);
+          if (ok) {
+            dot += vdot_d(vlogf(r.z), r.x);  // plain log (:649)
+            u = (lpi * r.x - r.y) / r.z;
+          }
+          vst<VW>(buf + li, u);
+        });
     const double t = block_sum(dot);
     if (threadIdx.x == 0) {
       double* gl = a.GL + (long long)p * K + i;
@@ -275,19 +323,24 @@ __global__ __launch_bounds__(kThreads, 2) void k_backward(SolveArgs a, int s) {
     }
     if (s == 1) return;
     conv_lds(buf, tabs, G);
-    for_each_px(G, [&](int li, int gi, bool ok) {
-      float u = 0.f;
-      if (ok) {
-        const float cb = Cs[gi];
-        u = buf[li] * (-Ai[gi] / (cb * cb));
-      }
-      buf[li] = u;
-    });
+    px_pass<VW>(
+        G,
+        [&](int gi) {
+          Two r;
+          r.x = vld<VW>(Cs + gi);
+          r.y = vld<VW>(Ai + gi);
+          return r;
+        },
+        [&](int li, int gi, bool ok, const Two& r) {
+          vst<VW>(buf + li, ok ? vld<VW>(buf + li) * (-r.y / (r.x * r.x)) : vsplat<VW>(0.f));
+        });
     conv_lds(buf, tabs, G);
     const float* Bm = a.Bh + (long long)(s - 2) * a.stage + img;  // b^{s-1}
-    for_each_px(G, [&](int li, int gi, bool ok) {
-      if (ok) Rc[img + gi] = buf[li] * Bm[gi];
-    });
+    px_pass<VW>(
+        G, [&](int gi) { return vld<VW>(Bm + gi); },
+        [&](int li, int gi, bool ok, V b) {
+          if (ok) vst<VW>(Rc + img + gi, vld<VW>(buf + li) * b);
+        });
   }
 }
 
@@ -307,6 +360,21 @@ __global__ void k_fill(float* __restrict__ dst, float v, long long n) {
   for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
        e += (long long)gridDim.x * blockDim.x)
     dst[e] = v;
+}
+
+// Vector width 4 when rows hold whole float4 groups.
+static void launch_forward(const SolveArgs& a, int grid, int lds, hipStream_t st, int s) {
+  if (a.G.W % 4 == 0)
+    hipLaunchKernelGGL(k_forward<4>, dim3(grid), dim3(kThreads), lds, st, a, s);
+  else
+    hipLaunchKernelGGL(k_forward<1>, dim3(grid), dim3(kThreads), lds, st, a, s);
+}
+
+static void launch_backward(const SolveArgs& a, int grid, int lds, hipStream_t st, int s) {
+  if (a.G.W % 4 == 0)
+    hipLaunchKernelGGL(k_backward<4>, dim3(grid, 2), dim3(kThreads), lds, st, a, s);
+  else
+    hipLaunchKernelGGL(k_backward<1>, dim3(grid, 2), dim3(kThreads), lds, st, a, s);
 }
 
 }  // namespace wdl
@@ -368,13 +436,13 @@ struct wdl_solver {
     const dim3 blk(kThreads);
     const int gs = stage_grid(np, K);
     for (int s = 1; s <= L; ++s)
-      hipLaunchKernelGGL(k_forward, dim3(gs), blk, lds, st, a, s);
+      launch_forward(a, gs, lds, st, s);
     const int fb = (int)std::min<long long>(kLossBlocks, (N + kThreads - 1) / kThreads);
     // every lossPart slot of the histograms run is written (unused blocks write 0)
     hipLaunchKernelGGL(k_final, dim3(kLossBlocks, np), blk, 0, st, a);
     (void)fb;
     for (int s = L; s >= 1; --s)
-      hipLaunchKernelGGL(k_backward, dim3(gs, 2), blk, lds, st, a, s);
+      launch_backward(a, gs, lds, st, s);
     if (reduce) {
       const long long KN = (long long)K * N;
       const int g = (int)std::min<long long>(1024, (KN + 255) / 256);
@@ -390,10 +458,10 @@ static int set_lds_attr() {
   static bool done = false;
   if (done) return WDL_OK;
   const int bytes = conv_lds_bytes(make_geom(kMaxSide, kMaxSide, 0));
-  WDL_HIP(hipFuncSetAttribute((const void*)k_forward, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              bytes));
-  WDL_HIP(hipFuncSetAttribute((const void*)k_backward, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              bytes));
+  const void* fns[] = {(const void*)k_forward<1>, (const void*)k_forward<4>,
+                       (const void*)k_backward<1>, (const void*)k_backward<4>};
+  for (const void* f : fns)
+    WDL_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
   done = true;
   return WDL_OK;
 }
@@ -651,9 +719,9 @@ int wdl_solver_run_stage(wdl_solver* s, int stage, int step) {
   const int lds = conv_lds_bytes(a.G);
   const int gs = stage_grid(s->P, s->K);
   if (stage == WDL_STAGE_FORWARD)
-    hipLaunchKernelGGL(k_forward, dim3(gs), dim3(kThreads), lds, s->ctx->stream, a, step);
+    launch_forward(a, gs, lds, s->ctx->stream, step);
   else
-    hipLaunchKernelGGL(k_backward, dim3(gs, 2), dim3(kThreads), lds, s->ctx->stream, a, step);
+    launch_backward(a, gs, lds, s->ctx->stream, step);
   WDL_HIP(hipGetLastError());
   return WDL_OK;
 }
