@@ -110,6 +110,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-form timing")
     ap.add_argument("--dense", action="store_true", help="do not skip all-zero factor chunks")
+    ap.add_argument("--streams", type=int, default=0,
+                    help="histogram groups on separate streams (0: the library default)")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="also run one 256 MiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -126,6 +130,8 @@ def main():
     s = wdl.SinkhornSolver(ctx, n, n, K, P, L, g, loss, flags)
     s.set_observed(c["obs"])
     s.set_problem(c["atoms"], c["weights"])
+    if args.streams:
+        s.set_streams(args.streams)
     stream = ctx.stream
 
     for _ in range(args.warmup):
@@ -242,6 +248,17 @@ def main():
         cb, err = cpu_baseline(c, res)
         line["cpu_baseline"] = cb
         line["err_vs_fp64"] = err
+    if args.copy_calib:
+        # A plain copy of known size: FETCH_SIZE / WRITE_SIZE of this copy kernel
+        # scale the counters read for the solver kernels to bytes.
+        nb = 256 << 20
+        src = _hip.DeviceArray((nb // 4,))
+        dst = _hip.DeviceArray((nb // 4,))
+        _hip.rt().hipMemcpy(dst.ptr, src.ptr, nb, _hip.D2D)
+        _hip.synchronize()
+        line["copy_calibration_bytes"] = nb
+        src.free()
+        dst.free()
     for e in ev + [e0, e1]:
         e.destroy()
     ctx.close()  # closes the solver and the kernel object first

@@ -101,6 +101,10 @@ int wdl_solver_set_problem_device(wdl_solver* s, const float* d_atoms, const flo
 int wdl_solver_set_warm_state(wdl_solver* s, const double* b_storage);
 int wdl_solver_get_warm_state(wdl_solver* s, double* b_temp);
 int wdl_solver_warm_commit(wdl_solver* s);
+/* Batched runs split the P histograms into nstreams groups, each advancing
+ * on its own stream (default min(4, P/16), at least 1).  No counterpart in
+ * the reference; a tuning knob of the batched launch. */
+int wdl_solver_set_streams(wdl_solver* s, int nstreams);
 /* Enqueue the forward L iterations and both unrolled backwards for all P
  * solves (compute_gradient_both, inverseWasserstein.h:544-674, or its log
  * form :678-814).  Asynchronous on the context stream. */

@@ -90,15 +90,18 @@ def test_gradient_both_golden(ctx, name, d):
         assert rel(r["b_temp"][0], d["b_last"]) < TOL
 
 
-def test_plain_equals_batched_and_graph_replays(ctx):
-    # Same kernels, one solve per launch sequence vs all at once: identical bits.
+@pytest.mark.parametrize("nstreams", [1, 3])
+def test_plain_equals_batched_and_graph_replays(ctx, nstreams):
+    # Same kernels, one solve per launch sequence vs all at once (on one or
+    # several streams): identical bits.
     W = H = 32
     K, P, L, g = 3, 10, 12, 1.0
-    print("plain vs batched P=%d" % P, flush=True)
+    print("plain vs batched P=%d streams=%d" % (P, nstreams), flush=True)
     rng = np.random.default_rng(3)
     obs = synthetic.blob_histograms(rng, W, P, (1, 3), (2, 6))
     atoms, lam = synthetic.init_atoms_weights(rng, K, P, W * H)
     s = wdl.SinkhornSolver(ctx, W, H, K, P, L, g)
+    s.set_streams(nstreams)
     s.set_observed(obs)
     s.set_problem(atoms, lam)
     s.run(wdl.RUN_PLAIN)
@@ -114,6 +117,7 @@ def test_plain_equals_batched_and_graph_replays(ctx):
         for k in ("grad_lambda", "grad_dic_sum", "bary", "loss"):
             np.testing.assert_array_equal(o[k], plain[k], err_msg=k)
     ne = wdl.SinkhornSolver(ctx, W, H, K, P, L, g, flags=wdl.FLAG_NO_GRAPH)
+    ne.set_streams(nstreams)
     ne.set_observed(obs)
     ne.set_problem(atoms, lam)
     ne.run()

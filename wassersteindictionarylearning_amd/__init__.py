@@ -58,6 +58,7 @@ _SIGS = {
     "wdl_solver_warm_commit": (_i, [_vp]),
     "wdl_solver_run": (_i, [_vp, _i]),
     "wdl_solver_run_stage": (_i, [_vp, _i, _i]),
+    "wdl_solver_set_streams": (_i, [_vp, _i]),
     "wdl_solver_get_results": (_i, [_vp, _d, _d, _d, _d]),
     "wdl_solver_get_grad_dic": (_i, [_vp, _d]),
     "wdl_gradient_both": (_i, [_vp, _d, _d, _d, _d, _d, _d]),
@@ -247,6 +248,9 @@ class SinkhornSolver:
 
     def run(self, mode: int = RUN_BATCHED):
         _check(lib().wdl_solver_run(self.handle, mode), "wdl_solver_run")
+
+    def set_streams(self, nstreams: int):
+        _check(lib().wdl_solver_set_streams(self.handle, nstreams), "wdl_solver_set_streams")
 
     def run_stage(self, stage: int, step: int):
         _check(lib().wdl_solver_run_stage(self.handle, stage, step), "wdl_solver_run_stage")

@@ -11,7 +11,7 @@ namespace wdl {
 // or FP64 (host drop-in: values are rounded to FP32 for the MFMA passes and
 // widened back on store).
 template <class T>
-__global__ __launch_bounds__(kThreads, 2) void k_conv(const T* __restrict__ in,
+__global__ __launch_bounds__(kThreads, 4) void k_conv(const T* __restrict__ in,
                                                       T* __restrict__ out, int nvec, ConvGeom G,
                                                       const float* __restrict__ gtab) {
   extern __shared__ __align__(16) float smem[];
@@ -31,6 +31,31 @@ __global__ __launch_bounds__(kThreads, 2) void k_conv(const T* __restrict__ in,
   }
 }
 
+// FP32 form with float4 loads and stores (rows of whole float4 groups).
+__global__ __launch_bounds__(kThreads, 4) void k_conv4(const float* __restrict__ in,
+                                                       float* __restrict__ out, int nvec,
+                                                       ConvGeom G, const float* __restrict__ gtab) {
+  extern __shared__ __align__(16) float smem[];
+  float* tabs = smem;
+  float* buf = smem + 4 * kTabLen;
+  load_tables(tabs, gtab);
+  const size_t N = (size_t)G.W * G.H;
+  for (int v = blockIdx.x; v < nvec; v += gridDim.x) {
+    const float* src = in + (size_t)v * N;
+    px_pass<4>(
+        G, [&](int gi) { return vld<4>(src + gi); },
+        [&](int li, int gi, bool ok, floatx4 x) { vst<4>(buf + li, ok ? x : vsplat<4>(0.f)); });
+    conv_lds(buf, tabs, G);
+    float* dst = out + (size_t)v * N;
+    px_pass<4>(
+        G, [&](int gi) { return 0; },
+        [&](int li, int gi, bool ok, int) {
+          if (ok) vst<4>(dst + gi, vld<4>(buf + li));
+        });
+    __syncthreads();
+  }
+}
+
 template <class T>
 static int launch_conv(const wdl_gauss2d* k, const T* in, T* out, int nvec, hipStream_t s) {
   if (nvec <= 0) return WDL_OK;
@@ -38,13 +63,20 @@ static int launch_conv(const wdl_gauss2d* k, const T* in, T* out, int nvec, hipS
   const int lds = conv_lds_bytes(G);
   static bool attr_set = false;
   if (!attr_set) {
+    const int bytes = conv_lds_bytes(make_geom(kMaxSide, kMaxSide, 0));
     WDL_HIP(hipFuncSetAttribute((const void*)k_conv<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                conv_lds_bytes(make_geom(kMaxSide, kMaxSide, 0))));
+                                bytes));
+    WDL_HIP(hipFuncSetAttribute((const void*)k_conv4, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                bytes));
     attr_set = true;
   }
   const int grid = nvec;
-  hipLaunchKernelGGL(k_conv<T>, dim3(grid), dim3(kThreads), lds, s, in, out, nvec, G,
-                     k->tab.as<float>());
+  if (sizeof(T) == sizeof(float) && G.W % 4 == 0)
+    hipLaunchKernelGGL(k_conv4, dim3(grid), dim3(kThreads), lds, s, (const float*)in, (float*)out,
+                       nvec, G, k->tab.as<float>());
+  else
+    hipLaunchKernelGGL(k_conv<T>, dim3(grid), dim3(kThreads), lds, s, in, out, nvec, G,
+                       k->tab.as<float>());
   WDL_HIP(hipGetLastError());
   return WDL_OK;
 }

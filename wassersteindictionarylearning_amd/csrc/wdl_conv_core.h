@@ -1,6 +1,6 @@
 // Device-side building blocks for the separable Gaussian convolution on gfx950.
 //
-// One workgroup (256 threads = 4 wave64) owns one H x W image at a time; the
+// One workgroup (512 threads = 8 wave64) owns one H x W image at a time; the
 // image lives in LDS for both 1-D passes, so a whole image-convolution costs
 // one HBM read and one HBM write of the image (plus whatever the fused
 // prologue / epilogue touches).  Each 1-D pass is a GEMM
@@ -21,12 +21,12 @@
 
 namespace wdl {
 
-constexpr int kThreads = 256;              // 4 wave64 per workgroup
+constexpr int kThreads = 512;              // 8 wave64 per workgroup
 constexpr int kWaves = kThreads / 64;
 constexpr int kMaxSide = 128;              // largest padded side held in LDS
 constexpr int kTabOff = kMaxSide + 16;     // table index of d = 0
 constexpr int kTabLen = 2 * kMaxSide + 48; // multiple of 4
-constexpr int kMaxUnits = 4;               // 16x64 output units per wave per pass
+constexpr int kMaxUnits = 2;               // 16x64 output units per wave per pass
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
@@ -86,17 +86,17 @@ __device__ inline void rows_gemm(const float* __restrict__ in, int sin, int nrow
         // B[k][j] = Kg(kk + 4g + t, c0 + c) = tab[kTabOff + kk + 4g + t - c0 - c]
         const int a = kTabOff + kk + 4 * g - c0 - c;
         const floatx4 bv = *reinterpret_cast<const floatx4*>(tabs + (a & 3) * kTabLen + (a & ~3));
+        // A[i][k] = IN[r0 + 16rb + c][kk + 4g + t]
+        floatx4 av[4];
 #pragma unroll
-        for (int rb = 0; rb < 4; ++rb) {
-          if (rb < nrb) {
-            // A[i][k] = IN[r0 + 16rb + c][kk + 4g + t]
-            const floatx4 av = *reinterpret_cast<const floatx4*>(arow + rb * 16 * sin + kk);
-            acc[uu][rb] = mfma4(av.x, bv.x, acc[uu][rb]);
-            acc[uu][rb] = mfma4(av.y, bv.y, acc[uu][rb]);
-            acc[uu][rb] = mfma4(av.z, bv.z, acc[uu][rb]);
-            acc[uu][rb] = mfma4(av.w, bv.w, acc[uu][rb]);
-          }
-        }
+        for (int rb = 0; rb < 4; ++rb)
+          if (rb < nrb) av[rb] = *reinterpret_cast<const floatx4*>(arow + rb * 16 * sin + kk);
+        // t outer, row block inner: consecutive MFMAs write different accumulators.
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+          for (int rb = 0; rb < 4; ++rb)
+            if (rb < nrb) acc[uu][rb] = mfma4(av[rb][t], bv[t], acc[uu][rb]);
       }
     }
   }

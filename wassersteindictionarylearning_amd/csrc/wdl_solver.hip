@@ -104,7 +104,7 @@ __device__ inline double block_sum(double v) {
 // Forward iteration s (1..L): b^{s-1} (geometric-mean step of iteration s-1,
 // :577-592), c^{s-1} = K b^{s-1} (:568), a = p / c (:569-574), phi^s = K a (:575).
 template <int VW>
-__global__ __launch_bounds__(kThreads, 2) void k_forward(SolveArgs a, int s) {
+__global__ __launch_bounds__(kThreads, 4) void k_forward(SolveArgs a, int s) {
   typedef typename Vec<VW>::T V;
   int p, i;
   if (!map_block(a, p, i)) return;
@@ -208,7 +208,7 @@ __global__ __launch_bounds__(kThreads) void k_final(SolveArgs a) {
 // Backward step s (L..1).  blockIdx.y = 0: dictionary gradient (:600-629);
 // blockIdx.y = 1: weight gradient (:635-673).
 template <int VW>
-__global__ __launch_bounds__(kThreads, 2) void k_backward(SolveArgs a, int s) {
+__global__ __launch_bounds__(kThreads, 4) void k_backward(SolveArgs a, int s) {
   typedef typename Vec<VW>::T V;
   int p, i;
   if (!map_block(a, p, i)) return;
@@ -429,28 +429,89 @@ struct wdl_solver {
     return a;
   }
 
-  int enqueue(int p0, int np, bool reduce) {
-    hipStream_t st = ctx->stream;
+  // Side streams for the batched form: histogram groups advance on their own
+  // stream, so stages of different groups overlap on the CUs (one group's
+  // elementwise HBM passes run beside another group's MFMA passes).
+  int ngroups = 1;
+  std::vector<hipStream_t> side;
+  std::vector<hipEvent_t> joins;
+  hipEvent_t fork = nullptr;
+
+  int set_groups(int n) {
+    n = std::max(1, std::min(n, P));
+    if (n == ngroups) return WDL_OK;
+    WDL_HIP(hipStreamSynchronize(ctx->stream));
+    release_streams();
+    if (exec) {
+      (void)hipGraphExecDestroy(exec);
+      exec = nullptr;
+    }
+    if (graph) {
+      (void)hipGraphDestroy(graph);
+      graph = nullptr;
+    }
+    ngroups = n;
+    if (n == 1) return WDL_OK;
+    WDL_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+    for (int g = 0; g < n; ++g) {
+      hipStream_t st;
+      hipEvent_t ev;
+      WDL_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+      side.push_back(st);
+      WDL_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+      joins.push_back(ev);
+    }
+    return WDL_OK;
+  }
+
+  void release_streams() {
+    for (hipStream_t st : side) (void)hipStreamDestroy(st);
+    for (hipEvent_t ev : joins) (void)hipEventDestroy(ev);
+    if (fork) (void)hipEventDestroy(fork);
+    side.clear();
+    joins.clear();
+    fork = nullptr;
+  }
+
+  // All P solves; with several groups, fork onto the side streams and join
+  // back on the context stream before the final reduction.
+  int enqueue_batched() {
+    if (ngroups <= 1) return enqueue(0, P, true, ctx->stream);
+    WDL_HIP(hipEventRecord(fork, ctx->stream));
+    int st;
+    for (int g = 0; g < ngroups; ++g) {
+      const int p0 = (int)((long long)P * g / ngroups), p1 = (int)((long long)P * (g + 1) / ngroups);
+      WDL_HIP(hipStreamWaitEvent(side[g], fork, 0));
+      if ((st = enqueue(p0, p1 - p0, false, side[g]))) return st;
+      WDL_HIP(hipEventRecord(joins[g], side[g]));
+      WDL_HIP(hipStreamWaitEvent(ctx->stream, joins[g], 0));
+    }
+    return reduce_gd(ctx->stream);
+  }
+
+  int reduce_gd(hipStream_t st) {
+    const long long KN = (long long)K * N;
+    const int g = (int)std::min<long long>(1024, (KN + 255) / 256);
+    hipLaunchKernelGGL(k_reduce_gd, dim3(g), dim3(256), 0, st, GD.as<float>(), GDsum.as<double>(),
+                       P, KN);
+    WDL_HIP(hipGetLastError());
+    return WDL_OK;
+  }
+
+  int enqueue(int p0, int np, bool reduce, hipStream_t st) {
     const SolveArgs a = args(p0, np);
     const int lds = conv_lds_bytes(a.G);
     const dim3 blk(kThreads);
     const int gs = stage_grid(np, K);
     for (int s = 1; s <= L; ++s)
       launch_forward(a, gs, lds, st, s);
-    const int fb = (int)std::min<long long>(kLossBlocks, (N + kThreads - 1) / kThreads);
-    // every lossPart slot of the histograms run is written (unused blocks write 0)
+    // kLossBlocks blocks per histogram: every lossPart slot is written
+    // (blocks past the last pixel write 0).
     hipLaunchKernelGGL(k_final, dim3(kLossBlocks, np), blk, 0, st, a);
-    (void)fb;
     for (int s = L; s >= 1; --s)
       launch_backward(a, gs, lds, st, s);
-    if (reduce) {
-      const long long KN = (long long)K * N;
-      const int g = (int)std::min<long long>(1024, (KN + 255) / 256);
-      hipLaunchKernelGGL(k_reduce_gd, dim3(g), dim3(256), 0, st, GD.as<float>(),
-                         GDsum.as<double>(), P, KN);
-    }
     WDL_HIP(hipGetLastError());
-    return WDL_OK;
+    return reduce ? reduce_gd(st) : WDL_OK;
   }
 };
 
@@ -558,8 +619,22 @@ int wdl_solver_create(wdl_ctx* ctx, int W, int H, int K, int P, int L, double ga
     set_error(std::string("wdl_solver_create: ") + hipGetErrorString(e));
     return WDL_ERR_HIP;
   }
+  // Default: one stream per 16 histograms, at most 4 (the hardware queues a
+  // process gets by default).
+  if ((st = s->set_groups(std::min(4, std::max(1, P / 16))))) {
+    wdl_solver_destroy(s);
+    return st;
+  }
   *out = s;
   return WDL_OK;
+}
+
+int wdl_solver_set_streams(wdl_solver* s, int nstreams) {
+  WDL_CHECK(s, WDL_ERR_ARG, "wdl_solver_set_streams: NULL argument");
+  WDL_CHECK(nstreams >= 1 && nstreams <= 16, WDL_ERR_ARG,
+            "wdl_solver_set_streams: nstreams must be in 1..16");
+  WDL_HIP(hipSetDevice(s->ctx->device));
+  return s->set_groups(nstreams);
 }
 
 int wdl_solver_destroy(wdl_solver* s) {
@@ -568,6 +643,7 @@ int wdl_solver_destroy(wdl_solver* s) {
   (void)hipStreamSynchronize(s->ctx->stream);
   if (s->exec) (void)hipGraphExecDestroy(s->exec);
   if (s->graph) (void)hipGraphDestroy(s->graph);
+  s->release_streams();
   DevBuf* bufs[] = {&s->tab, &s->A,  &s->lam, &s->obs, &s->Bh, &s->Ch,       &s->Fh,
                     &s->V0,  &s->V1, &s->R0,  &s->R1,  &s->GD, &s->GL,       &s->Q,
                     &s->G0,  &s->lossPart,    &s->bStore,      &s->bTemp,    &s->GDsum};
@@ -685,13 +761,13 @@ int wdl_solver_run(wdl_solver* s, int mode) {
   int st;
   if (mode == WDL_RUN_PLAIN) {
     for (int p = 0; p < s->P; ++p)
-      if ((st = s->enqueue(p, 1, p == s->P - 1))) return st;
+      if ((st = s->enqueue(p, 1, p == s->P - 1, s->ctx->stream))) return st;
   } else if (s->flags & WDL_FLAG_NO_GRAPH) {
-    if ((st = s->enqueue(0, s->P, true))) return st;
+    if ((st = s->enqueue_batched())) return st;
   } else {
     if (!s->exec) {
       WDL_HIP(hipStreamBeginCapture(s->ctx->stream, hipStreamCaptureModeThreadLocal));
-      st = s->enqueue(0, s->P, true);
+      st = s->enqueue_batched();
       hipGraph_t g = nullptr;
       hipError_t e = hipStreamEndCapture(s->ctx->stream, &g);
       if (st) return st;
@@ -716,7 +792,7 @@ int wdl_solver_run_stage(wdl_solver* s, int stage, int step) {
             "wdl_solver_run_stage: unknown stage");
   WDL_HIP(hipSetDevice(s->ctx->device));
   const SolveArgs a = s->args(0, s->P);
-  const int lds = conv_lds_bytes(a.G);
+  const int lds = conv_lds_bytes(a.G);  // all P solves, on the context stream
   const int gs = stage_grid(s->P, s->K);
   if (stage == WDL_STAGE_FORWARD)
     launch_forward(a, gs, lds, s->ctx->stream, step);
