@@ -139,6 +139,8 @@ int wdl_gauss2d_destroy(wdl_gauss2d* k) {
   k->io_in.release();
   k->io_out.release();
   k->io_sign.release();
+  k->io_tmp.release();
+  k->io_bits.release();
   delete k;
   return WDL_OK;
 }

@@ -65,7 +65,9 @@ struct wdl_gauss2d {
   int flags = 0;
   wdl::DevBuf tab;        // kTabLen floats
   wdl::DevBuf io_in, io_out;   // staging for the host-pointer entry points
-  wdl::DevBuf io_sign;
+  wdl::DevBuf io_tmp;          // log domain: the transposed pass-1 result (FP64)
+  wdl::DevBuf io_sign;         // log domain: one byte per value, passes 1 and 2
+  wdl::DevBuf io_bits;         // log domain: the caller's packed sign array
   wdl::ConvGeom geom(bool band) const;
 };
 

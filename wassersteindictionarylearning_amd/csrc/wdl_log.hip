@@ -1,20 +1,166 @@
-// Log-domain separable Gaussian convolution (LogSignArrayGaussian2DKernel,
-// kernels.h:1034-1202).
+// Log-domain separable Gaussian convolution: the drop-in for
+// LogSignArrayGaussian2DKernel::log_convolve / log_convolve_signArray
+// (kernels.h:1073-1162), in FP64 on the GPU.
+//
+// Each 1-D pass computes, for every output j of a line x,
+//     m_j = max(-1e99, max_k(-(j-k)^2/gamma + x_k))
+//     s_j = sum_k sign_k * exp(-(j-k)^2/gamma + x_k - m_j)
+//     out_j = log|s_j| + m_j,   sign_out_j = (s_j >= 0)
+// with k ascending, as the reference does; one workgroup per line, the line
+// staged in LDS, one thread per output.  Pass 1 runs along x and writes its
+// result transposed (as the reference's tmp[i + j*H]); pass 2 runs along y on
+// that and writes back in image order.  Output signs go to a byte per value
+// and are packed LSB-first into the caller's sign array (signArray.h:8-17)
+// afterwards, so no two threads write one byte.
+#include <cmath>
+
 #include "wdl_internal.h"
+
+namespace wdl {
+
+constexpr int kLineThreads = 256;
+
+// in: nvec images of `rows` lines of length `len` (line-major).  out: the
+// transposed layout (len x rows per image).  Input signs either from packed
+// bits (bit index = v*rows*len + r*len + k) or from bytes; none means positive.
+template <bool SIGNED, bool BITS_IN>
+__global__ __launch_bounds__(kLineThreads) void k_lse_lines(
+    const double* __restrict__ in, const unsigned char* __restrict__ sgn_in, double* __restrict__ out,
+    unsigned char* __restrict__ sgn_out, int rows, int len, int nvec, double gamma) {
+  extern __shared__ __align__(16) double line[];
+  double* x = line;
+  float* sg = reinterpret_cast<float*>(line + len);
+  const long long L = (long long)rows * len;
+  for (long long blk = blockIdx.x; blk < (long long)nvec * rows; blk += gridDim.x) {
+    const int v = (int)(blk / rows), r = (int)(blk - (long long)v * rows);
+    const long long base = (long long)v * L + (long long)r * len;
+    for (int k = threadIdx.x; k < len; k += kLineThreads) {
+      x[k] = in[base + k];
+      if (SIGNED) {
+        bool pos;
+        if (BITS_IN) {
+          const long long bit = base + k;
+          pos = (sgn_in[bit >> 3] >> (bit & 7)) & 1;
+        } else {
+          pos = sgn_in[base + k] != 0;
+        }
+        sg[k] = pos ? 1.f : -1.f;  // getSign (signArray.h:19-22)
+      }
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < len; j += kLineThreads) {
+      double m = -1e99;
+      for (int k = 0; k < len; ++k) {
+        const double t = -(double)((j - k) * (j - k)) / gamma + x[k];
+        if (m < t) m = t;  // std::max(m, t)
+      }
+      double s = 0;
+      for (int k = 0; k < len; ++k) {
+        const double e = exp(-(double)((j - k) * (j - k)) / gamma + x[k] - m);
+        s += SIGNED ? (double)sg[k] * e : e;
+      }
+      const long long o = (long long)v * L + (long long)j * rows + r;  // transposed
+      out[o] = log(SIGNED ? fabs(s) : s) + m;
+      if (SIGNED) sgn_out[o] = s >= 0 ? 1 : 0;
+    }
+    __syncthreads();
+  }
+}
+
+// Packs one byte per value into bits (LSB first); bits past n keep their value.
+__global__ void k_pack_signs(const unsigned char* __restrict__ bytes, unsigned char* __restrict__ bits,
+                             long long n) {
+  const long long nb = (n + 7) >> 3;
+  for (long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x; b < nb;
+       b += (long long)gridDim.x * blockDim.x) {
+    unsigned char byte = bits[b];
+    for (int t = 0; t < 8; ++t) {
+      const long long i = b * 8 + t;
+      if (i < n) byte = bytes[i] ? (byte | (1u << t)) : (byte & ~(1u << t));
+    }
+    bits[b] = byte;
+  }
+}
+
+static int lse_launch_bytes(int len) { return len * (int)(sizeof(double) + sizeof(float)); }
+
+// Two passes over nvec images; signs == nullptr for the unsigned form.
+static int log_convolve_dev(wdl_gauss2d* k, const double* d_in, double* d_out, unsigned char* d_bits,
+                            int nvec, hipStream_t st) {
+  const long long n = (long long)k->W * k->H * nvec;
+  int rc;
+  if ((rc = k->io_tmp.ensure(n * sizeof(double)))) return rc;
+  const bool sgn = d_bits != nullptr;
+  if (sgn && (rc = k->io_sign.ensure(2 * n))) return rc;
+  unsigned char* b1 = k->io_sign.as<unsigned char>();
+  unsigned char* b2 = b1 + n;
+  const int W = k->W, H = k->H;
+  const int g1 = (int)std::min<long long>((long long)nvec * H, 65535);
+  const int g2 = (int)std::min<long long>((long long)nvec * W, 65535);
+  double* tmp = k->io_tmp.as<double>();
+  if (sgn) {
+    hipLaunchKernelGGL((k_lse_lines<true, true>), dim3(g1), dim3(kLineThreads), lse_launch_bytes(W),
+                       st, d_in, d_bits, tmp, b1, H, W, nvec, k->gamma);
+    hipLaunchKernelGGL((k_lse_lines<true, false>), dim3(g2), dim3(kLineThreads),
+                       lse_launch_bytes(H), st, tmp, b1, d_out, b2, W, H, nvec, k->gamma);
+    const int gp = (int)std::min<long long>(((n + 7) / 8 + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_pack_signs, dim3(gp), dim3(256), 0, st, b2, d_bits, n);
+  } else {
+    hipLaunchKernelGGL((k_lse_lines<false, false>), dim3(g1), dim3(kLineThreads),
+                       lse_launch_bytes(W), st, d_in, nullptr, tmp, nullptr, H, W, nvec, k->gamma);
+    hipLaunchKernelGGL((k_lse_lines<false, false>), dim3(g2), dim3(kLineThreads),
+                       lse_launch_bytes(H), st, tmp, nullptr, d_out, nullptr, W, H, nvec, k->gamma);
+  }
+  WDL_HIP(hipGetLastError());
+  return WDL_OK;
+}
+
+}  // namespace wdl
 
 extern "C" {
 
 int wdl_gauss2d_log_convolve(wdl_gauss2d* k, const double* log_u, double* out, int nvectors) {
-  (void)k; (void)log_u; (void)out; (void)nvectors;
-  wdl::set_error("wdl_gauss2d_log_convolve: not available in this build");
-  return WDL_ERR_UNSUPPORTED;
+  WDL_CHECK(k && log_u && out, WDL_ERR_ARG, "wdl_gauss2d_log_convolve: NULL argument");
+  WDL_CHECK(nvectors >= 0, WDL_ERR_ARG, "wdl_gauss2d_log_convolve: nvectors < 0");
+  if (nvectors == 0) return WDL_OK;
+  WDL_HIP(hipSetDevice(k->ctx->device));
+  const size_t bytes = (size_t)k->W * k->H * nvectors * sizeof(double);
+  int st;
+  if ((st = k->io_in.ensure(bytes)) || (st = k->io_out.ensure(bytes))) return st;
+  hipStream_t s = k->ctx->stream;
+  WDL_HIP(hipMemcpyAsync(k->io_in.p, log_u, bytes, hipMemcpyHostToDevice, s));
+  if ((st = wdl::log_convolve_dev(k, k->io_in.as<double>(), k->io_out.as<double>(), nullptr,
+                                  nvectors, s)))
+    return st;
+  WDL_HIP(hipMemcpyAsync(out, k->io_out.p, bytes, hipMemcpyDeviceToHost, s));
+  WDL_HIP(hipStreamSynchronize(s));
+  return WDL_OK;
 }
 
 int wdl_gauss2d_log_convolve_signarray(wdl_gauss2d* k, const double* log_u, unsigned char* signs,
                                        double* out, int nvectors) {
-  (void)k; (void)log_u; (void)signs; (void)out; (void)nvectors;
-  wdl::set_error("wdl_gauss2d_log_convolve_signarray: not available in this build");
-  return WDL_ERR_UNSUPPORTED;
+  WDL_CHECK(k && log_u && signs && out, WDL_ERR_ARG,
+            "wdl_gauss2d_log_convolve_signarray: NULL argument");
+  WDL_CHECK(nvectors >= 0, WDL_ERR_ARG, "wdl_gauss2d_log_convolve_signarray: nvectors < 0");
+  if (nvectors == 0) return WDL_OK;
+  WDL_HIP(hipSetDevice(k->ctx->device));
+  const long long n = (long long)k->W * k->H * nvectors;
+  const size_t bytes = (size_t)n * sizeof(double);
+  const size_t nbits = (size_t)(n + 7) / 8;
+  int st;
+  if ((st = k->io_in.ensure(bytes)) || (st = k->io_out.ensure(bytes)) ||
+      (st = k->io_bits.ensure(nbits)))
+    return st;
+  hipStream_t s = k->ctx->stream;
+  WDL_HIP(hipMemcpyAsync(k->io_in.p, log_u, bytes, hipMemcpyHostToDevice, s));
+  WDL_HIP(hipMemcpyAsync(k->io_bits.p, signs, nbits, hipMemcpyHostToDevice, s));
+  if ((st = wdl::log_convolve_dev(k, k->io_in.as<double>(), k->io_out.as<double>(),
+                                  k->io_bits.as<unsigned char>(), nvectors, s)))
+    return st;
+  WDL_HIP(hipMemcpyAsync(out, k->io_out.p, bytes, hipMemcpyDeviceToHost, s));
+  WDL_HIP(hipMemcpyAsync(signs, k->io_bits.p, nbits, hipMemcpyDeviceToHost, s));
+  WDL_HIP(hipStreamSynchronize(s));
+  return WDL_OK;
 }
 
 }  // extern "C"
