@@ -118,7 +118,7 @@ def main():
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(the P solves would shard by histogram with one K*N all-reduce per "
-              "evaluation; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+              "evaluation). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     c = synthetic.make_problem("batched_std")

@@ -1,0 +1,95 @@
+"""Parity of the batched log-domain solver (compute_gradient_both_log_signArray
+for P histograms) with the FP64 oracle and the reference's golden vectors.
+
+Log values are stored in FP32, so their absolute precision is ~6e-8 of their
+magnitude; the tolerance is the hot path's relative L2 <= 1e-5 on the weight
+gradient, the dictionary gradient and the barycenters."""
+import numpy as np
+import pytest
+
+import wassersteindictionarylearning_amd as wdl
+from oracle import oracle as O
+from tests._util import fixtures, rel
+from wassersteindictionarylearning_amd import synthetic
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = wdl.Context(0)
+    yield c
+    c.close()
+
+
+def solve(ctx, atoms, lam, obs, W, H, g, L, loss, mode=wdl.RUN_BATCHED, flags=0):
+    P, K = lam.shape
+    s = wdl.SinkhornSolver(ctx, W, H, K, P, L, g, loss, wdl.FLAG_LOG_DOMAIN | flags)
+    s.set_observed(obs)
+    s.set_problem(atoms, lam)
+    s.run(mode)
+    r = s.results()
+    r["grad_dic"] = s.grad_dic()
+    r["events"] = s.log_range_events()
+    s.close()
+    return r
+
+
+@pytest.mark.parametrize("W,H,K,P,L,g,loss,curves", [
+    (12, 12, 3, 2, 5, 0.5, 4, False), (16, 12, 2, 3, 6, 1.0, 2, False),
+    (32, 32, 3, 4, 20, 0.5, 4, True), (64, 64, 4, 2, 30, 0.5, 4, True)])
+def test_gradient_both_log_matches_oracle(ctx, W, H, K, P, L, g, loss, curves):
+    print("log gradient W=%d H=%d K=%d P=%d L=%d gamma=%g" % (W, H, K, P, L, g), flush=True)
+    rng = np.random.default_rng(W * L + P)
+    if curves and W == H:
+        obs = synthetic.curve_histograms(rng, W, P)
+    else:
+        obs = rng.uniform(0.2, 1, (P, W * H))
+        obs /= obs.sum(1, keepdims=True)
+    atoms, lam = synthetic.init_atoms_weights(rng, K, P, W * H)
+    r = solve(ctx, atoms, lam, obs, W, H, g, L, loss)
+    assert r["events"] == 0
+    for p in range(P):
+        o = O.gradient_both_log(atoms, lam[p], obs[p], W, H, g, L, loss)
+        assert rel(r["bary"][p], o["bary"]) < TOL, p
+        assert rel(r["grad_lambda"][p], o["grad_lambda"]) < TOL, p
+        assert rel(r["grad_dic"][p], o["grad_dic"]) < TOL, p
+        assert abs(r["loss"][p] - O.loss_value(loss, o["bary"], obs[p])) < TOL * abs(r["loss"][p])
+
+
+@pytest.mark.parametrize("name,d", fixtures("glog"))
+def test_gradient_both_log_golden(ctx, name, d):
+    W, H, L, g, loss = int(d["W"]), int(d["H"]), int(d["L"]), float(d["gamma"]), int(d["loss"])
+    print("log gradient golden", name, flush=True)
+    r = solve(ctx, d["atoms"], d["lam"][None], d["obs"][None], W, H, g, L, loss)
+    assert rel(r["grad_lambda"][0], d["grad_lambda"]) < TOL
+    assert rel(r["grad_dic"][0], d["grad_dic"]) < TOL
+    assert rel(r["bary"][0], d["bary"]) < TOL
+
+
+def test_log_plain_equals_batched(ctx):
+    W = H = 24
+    K, P, L, g = 3, 6, 8, 0.5
+    print("log plain vs batched", flush=True)
+    rng = np.random.default_rng(8)
+    obs = synthetic.curve_histograms(rng, W, P)
+    atoms, lam = synthetic.init_atoms_weights(rng, K, P, W * H)
+    a = solve(ctx, atoms, lam, obs, W, H, g, L, 4, wdl.RUN_PLAIN)
+    b = solve(ctx, atoms, lam, obs, W, H, g, L, 4, wdl.RUN_BATCHED)
+    for k in ("grad_lambda", "grad_dic_sum", "bary", "loss"):
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+
+
+@pytest.mark.parametrize("name,d", [f for f in fixtures("eval") if f[1]["log_domain"]])
+def test_evaluate_both_log_golden(ctx, name, d):
+    W, H, K, P, L = (int(d[k]) for k in ("W", "H", "K", "P", "L"))
+    print("log evaluate_both golden", name, flush=True)
+    s = wdl.SinkhornSolver(ctx, W, H, K, P, L, float(d["gamma"]), int(d["loss"]),
+                           wdl.FLAG_LOG_DOMAIN)
+    s.set_observed(d["obs"])
+    s.set_scale(float(d["scale"]), bool(d["exp_weight"]))
+    f, g = s.evaluate_both(d["x"])
+    s.close()
+    assert abs(f - float(d["f"])) <= TOL * abs(float(d["f"]))
+    assert rel(g, d["g"]) < TOL

@@ -61,6 +61,7 @@ _SIGS = {
     "wdl_solver_set_streams": (_i, [_vp, _i]),
     "wdl_solver_get_results": (_i, [_vp, _d, _d, _d, _d]),
     "wdl_solver_get_grad_dic": (_i, [_vp, _d]),
+    "wdl_solver_log_range_events": (_i, [_vp, ctypes.POINTER(ctypes.c_longlong)]),
     "wdl_gradient_both": (_i, [_vp, _d, _d, _d, _d, _d, _d]),
     "wdl_solver_set_scale": (_i, [_vp, ctypes.c_double, _i]),
     "wdl_evaluate_both": (ctypes.c_double, [_vp, _d, _d, _i, ctypes.c_double]),
@@ -263,6 +264,12 @@ class SinkhornSolver:
         _check(lib().wdl_solver_get_results(self.handle, _dp(loss), _dp(gl), _dp(gd), _dp(q)),
                "wdl_solver_get_results")
         return dict(loss=loss, grad_lambda=gl, grad_dic_sum=gd, bary=q)
+
+    def log_range_events(self) -> int:
+        c = ctypes.c_longlong()
+        _check(lib().wdl_solver_log_range_events(self.handle, ctypes.byref(c)),
+               "wdl_solver_log_range_events")
+        return int(c.value)
 
     def grad_dic(self):
         out = np.zeros((self.P, self.K, self.N))

@@ -15,8 +15,8 @@ LIB_DIR = os.path.join(PKG, "_lib")
 LIB = os.path.join(LIB_DIR, "libwdl.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-# -packed-fp32-ops: no v_pk_{add,mul,fma}_f32 are generated (see DESIGN.md,
-# "numerics"); hipcc's host pass warns that it does not know the feature.
+# -packed-fp32-ops: hipcc generates no packed FP32 add/mul/fma instructions;
+# hipcc's host pass warns that it does not know the feature.
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

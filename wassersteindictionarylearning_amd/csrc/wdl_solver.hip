@@ -22,6 +22,7 @@
 #include <limits>
 
 #include "wdl_internal.h"
+#include "wdl_logconv.h"
 
 namespace wdl {
 
@@ -50,6 +51,8 @@ struct SolveArgs {
   double* lossPart;
   const float* bInit;
   float* bSave;
+  const float* logA;  // K x N log atoms (log-domain solver)
+  unsigned* events;   // lines whose spread exceeded kLogRange (log-domain solver)
 };
 
 // Workgroup -> (histogram, atom).  With 8+ histograms the K workgroups of a
@@ -356,14 +359,293 @@ __global__ void k_reduce_gd(const float* __restrict__ GD, double* __restrict__ o
   }
 }
 
+__global__ void k_log(const float* __restrict__ x, float* __restrict__ y, long long n) {
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (long long)gridDim.x * blockDim.x)
+    y[e] = logf(x[e]);
+}
+
+__global__ void k_zero_u32(unsigned* p) { *p = 0u; }
+
 __global__ void k_fill(float* __restrict__ dst, float v, long long n) {
   for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
        e += (long long)gridDim.x * blockDim.x)
     dst[e] = v;
 }
 
+// ---- log domain (compute_gradient_both_log_signArray, :678-814) ----------------
+// Bh/Ch/Fh hold log b, log conv_b, log phi in FP64 (a log value needs
+// absolute precision; FP32 would give it ~|x| * 6e-8); V/R hold the linear
+// -v / -r the reference carries (:762, :805) and GD is linear, in FP32.  Each convolution is
+// log_conv_lds (per-line max shifts + MFMA); its input is rebuilt from
+// global data in both of its passes.
+
+__device__ inline double vdot_dd(double a, float b) { return a * (double)b; }
+__device__ inline double vdot_dd(doublex4 a, floatx4 b) {
+  return a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w;
+}
+
+struct LogSmem {
+  float* tabs;
+  float* buf;
+  LogLds L;
+};
+
+__device__ inline LogSmem log_smem(float* smem, const ConvGeom& G) {
+  LogSmem m;
+  m.tabs = smem;
+  m.buf = smem + 4 * kTabLen;
+  const int img = (G.Hp * G.S1 > G.Wp * G.S2) ? G.Hp * G.S1 : G.Wp * G.S2;
+  int* keys = reinterpret_cast<int*>(m.buf + img);
+  m.L.mrow = keys;
+  m.L.mcol = keys + kMaxSide;
+  m.L.rmin = keys + 2 * kMaxSide;
+  return m;
+}
+
+template <int VW>
+__global__ __launch_bounds__(kThreads, 4) void k_forward_log(SolveArgs a, int s) {
+  typedef typename Vec<VW>::T V;
+  int p, i;
+  if (!map_block(a, p, i)) return;
+  extern __shared__ __align__(16) float smem[];
+  const ConvGeom G = a.G;
+  const LogSmem sm = log_smem(smem, G);
+  load_tables(sm.tabs, a.tab);
+  const long long N = a.N;
+  const int K = a.K;
+  const long long hbase = (long long)p * K * N;
+  const long long img = hbase + (long long)i * N;
+  typedef typename VecD<VW>::T VD;
+  struct TwoD { VD x, y; };
+  // conv 1 input: log b^{s-1} (b^{s-1} = lq - phi_i, :711-724)
+  if (s == 1) {
+    const float* bi = a.bInit ? a.bInit + img : nullptr;  // warm start, FP32
+    log_conv_lds<VW>(
+        sm.buf, sm.tabs, sm.L, G, a.events,
+        [&](int gi) { return bi ? vld<VW>(bi + gi) : vsplat<VW>(0.f); },
+        [&](int, int, int, V b, bool) { return SLog<VW>{vd(b), vsplat<VW>(1.f)}; });
+  } else {
+    const double* Fp = (const double*)a.Fh + (long long)(s - 2) * a.stage + hbase;  // log phi^{s-1}
+    double* Bd = (double*)a.Bh + (long long)(s - 2) * a.stage + img;
+    const float* lam = a.lam + (long long)p * K;
+    log_conv_lds<VW>(
+        sm.buf, sm.tabs, sm.L, G, a.events,
+        [&](int gi) {
+          TwoD r;
+          r.x = vd(vsplat<VW>(0.f));
+          r.y = vd(vsplat<VW>(0.f));
+          for (int j = 0; j < K; ++j) {
+            const VD f = vldd<VW>(Fp + (long long)j * N + gi);
+            r.x += (double)lam[j] * f;
+            if (j == i) r.y = f;
+          }
+          return r;
+        },
+        [&](int, int gi, int, const TwoD& r, bool second) {
+          const VD b = r.x - r.y;
+          if (second) vstd<VW>(Bd + gi, b);
+          return SLog<VW>{b, vsplat<VW>(1.f)};
+        });
+  }
+  // conv 2 input: log a = log p - log c (:704-707); store log c
+  double* Cd = (double*)a.Ch + (long long)(s - 1) * a.stage + img;
+  const float* LA = a.logA + (long long)i * N;
+  log_conv_lds<VW>(
+      sm.buf, sm.tabs, sm.L, G, a.events, [&](int gi) { return vld<VW>(LA + gi); },
+      [&](int li, int gi, int row, V la, bool second) {
+        const SLog<VW> c = log_result<VW>(sm.buf, sm.L, G, li, row);
+        if (second) vstd<VW>(Cd + gi, c.x);
+        return SLog<VW>{vd(la) - c.x, vsplat<VW>(1.f)};
+      });
+  double* Fd = (double*)a.Fh + (long long)(s - 1) * a.stage + img;
+  px_pass_r<VW>(
+      G, [&](int) { return 0; },
+      [&](int li, int gi, int row, bool ok, int) {
+        if (ok) vstd<VW>(Fd + gi, log_result<VW>(sm.buf, sm.L, G, li, row).x);
+      });
+}
+
+// After iteration L: log q = sum_i lam_i log phi_i^L, b^L = log q - log phi^L,
+// q = exp(log q) (:711-728), loss and its gradient at q.
+__global__ __launch_bounds__(kThreads) void k_final_log(SolveArgs a) {
+  const int p = a.p0 + blockIdx.y;
+  const long long N = a.N;
+  const int K = a.K;
+  const long long hbase = (long long)p * K * N;
+  const double* F = (const double*)a.Fh + (long long)(a.L - 1) * a.stage + hbase;
+  double* B = (double*)a.Bh + (long long)(a.L - 1) * a.stage + hbase;
+  const float* lam = a.lam + (long long)p * K;
+  double acc = 0;
+  for (long long x = (long long)blockIdx.x * kThreads + threadIdx.x; x < N;
+       x += (long long)gridDim.x * kThreads) {
+    double lq = 0.0;
+    for (int j = 0; j < K; ++j) lq += (double)lam[j] * F[j * N + x];
+    for (int j = 0; j < K; ++j) {
+      const double b = lq - F[j * N + x];
+      B[j * N + x] = b;
+      if (a.bSave) a.bSave[hbase + j * N + x] = (float)b;
+    }
+    const float q = (float)exp(lq);
+    const float o = a.obs[(long long)p * N + x];
+    a.Q[(long long)p * N + x] = q;
+    a.G0[(long long)p * N + x] = loss_grad(a.loss, q, o);
+    acc += loss_term(a.loss, q, o);
+  }
+  const double t = block_sum(acc);
+  if (threadIdx.x == 0) a.lossPart[(long long)p * kLossBlocks + blockIdx.x] = t;
+}
+
+template <int VW>
+__global__ __launch_bounds__(kThreads, 4) void k_backward_log(SolveArgs a, int s) {
+  typedef typename Vec<VW>::T V;
+  int p, i;
+  if (!map_block(a, p, i)) return;
+  extern __shared__ __align__(16) float smem[];
+  const ConvGeom G = a.G;
+  const LogSmem sm = log_smem(smem, G);
+  load_tables(sm.tabs, a.tab);
+  const long long N = a.N;
+  const int K = a.K;
+  const long long hbase = (long long)p * K * N;
+  const long long img = hbase + (long long)i * N;
+  const float lpi = a.lam[(long long)p * K + i];
+  const float* LA = a.logA + (long long)i * N;
+  const double* Cs = (const double*)a.Ch + (long long)(s - 1) * a.stage + img;  // log c^{s-1}
+  const bool last = (s == a.L);
+  typedef typename VecD<VW>::T VD;
+  struct Three { V x, y; VD z; };
+  struct Two { V x, y; };
+  struct TwoL { VD x; V y; };
+  // second conv input: log p + c - 2 log c^{s-1}, with the sign of c (:754-756, :796)
+  auto mid_make = [&](int li, int, int row, const TwoL& r, bool) {
+    const SLog<VW> c = log_result<VW>(sm.buf, sm.L, G, li, row);
+    return SLog<VW>{vd(r.y) + c.x - 2.0 * r.x, c.s};
+  };
+  auto mid_load = [&](int gi) {
+    TwoL r;
+    r.x = vldd<VW>(Cs + gi);
+    r.y = vld<VW>(LA + gi);
+    return r;
+  };
+  if (blockIdx.y == 0) {  // dictionary gradient, :736-770
+    const float* Vp = (s & 1) ? a.V0 : a.V1;
+    float* Vc = (s & 1) ? a.V1 : a.V0;
+    const double* Bs = (const double*)a.Bh + (long long)(s - 1) * a.stage + img;  // log b^s
+    const float* G0 = a.G0 + (long long)p * N;
+    log_conv_lds<VW>(
+        sm.buf, sm.tabs, sm.L, G, a.events,
+        [&](int gi) {
+          Three r;  // n, v_i, log b^s
+          r.z = vldd<VW>(Bs + gi);
+          if (last) {
+            r.x = vld<VW>(G0 + gi);
+            r.y = vsplat<VW>(0.f);
+          } else {
+            r.x = vsplat<VW>(0.f);
+            r.y = vsplat<VW>(0.f);
+            for (int j = 0; j < K; ++j) {
+              const V v = vld<VW>(Vp + hbase + (long long)j * N + gi);
+              r.x -= v;  // n = -sum_i v_i (:764)
+              if (j == i) r.y = v;
+            }
+          }
+          return r;
+        },
+        [&](int, int, int, const Three& r, bool) {
+          const V x = lpi * r.x + r.y;
+          return SLog<VW>{vhlog(vabs(x)) + r.z, vsign(x)};
+        });
+    float* GDp = a.GD + img;
+    px_pass_r<VW>(
+        G,
+        [&](int gi) {
+          TwoL r;
+          r.x = vldd<VW>(Cs + gi);
+          r.y = last ? vsplat<VW>(0.f) : vld<VW>(GDp + gi);
+          return r;
+        },
+        [&](int li, int gi, int row, bool ok, const TwoL& r) {
+          if (ok) {
+            const SLog<VW> c = log_result<VW>(sm.buf, sm.L, G, li, row);
+            vst<VW>(GDp + gi, r.y + c.s * vexpf(vf(c.x - r.x)));  // :752
+          }
+        });
+    if (s == 1) return;
+    log_conv_lds<VW>(sm.buf, sm.tabs, sm.L, G, a.events, mid_load, mid_make);
+    const double* Fm = (const double*)a.Fh + (long long)(s - 2) * a.stage + img;  // log phi^{s-1}
+    px_pass_r<VW>(
+        G, [&](int gi) { return vldd<VW>(Fm + gi); },
+        [&](int li, int gi, int row, bool ok, VD f) {
+          if (ok) {
+            const SLog<VW> v = log_result<VW>(sm.buf, sm.L, G, li, row);
+            vst<VW>(Vc + img + gi, v.s * vexpf(vf(v.x - f)));  // -v (:762)
+          }
+        });
+  } else {  // weight gradient, :774-811
+    const float* Rp = (s & 1) ? a.R0 : a.R1;
+    float* Rc = (s & 1) ? a.R1 : a.R0;
+    const double* Fs = (const double*)a.Fh + (long long)(s - 1) * a.stage + img;  // log phi^s
+    const float* G0 = a.G0 + (long long)p * N;
+    const float* Qp = a.Q + (long long)p * N;
+    double dot = 0;
+    auto wload = [&](int gi) {
+      Three r;  // g, r_i, log phi^s
+      r.z = vldd<VW>(Fs + gi);
+      if (last) {
+        r.x = vld<VW>(G0 + gi) * vld<VW>(Qp + gi);
+        r.y = vsplat<VW>(0.f);
+      } else {
+        r.x = vsplat<VW>(0.f);
+        r.y = vsplat<VW>(0.f);
+        for (int j = 0; j < K; ++j) {
+          const V v = vld<VW>(Rp + hbase + (long long)j * N + gi);
+          r.x -= v;  // g = -sum_i r_i (:808)
+          if (j == i) r.y = v;
+        }
+      }
+      return r;
+    };
+    auto wmake = [&](int, int, int, const Three& r, bool second) {
+      if (second) dot += vdot_dd(r.z, r.x);  // sum phi . g (:786)
+      const V x = lpi * r.x + r.y;
+      return SLog<VW>{vhlog(vabs(x)) - r.z, vsign(x)};
+    };
+    if (s == 1) {
+      px_pass_r<VW>(G, wload, [&](int, int, int, bool ok, const Three& r) {
+        if (ok) dot += vdot_dd(r.z, r.x);
+      });
+    } else {
+      log_conv_lds<VW>(sm.buf, sm.tabs, sm.L, G, a.events, wload, wmake);
+    }
+    const double t = block_sum(dot);
+    if (threadIdx.x == 0) {
+      double* gl = a.GL + (long long)p * K + i;
+      *gl = last ? t : *gl + t;
+    }
+    if (s == 1) return;
+    log_conv_lds<VW>(sm.buf, sm.tabs, sm.L, G, a.events, mid_load, mid_make);
+    const double* Bm = (const double*)a.Bh + (long long)(s - 2) * a.stage + img;  // log b^{s-1}
+    px_pass_r<VW>(
+        G, [&](int gi) { return vldd<VW>(Bm + gi); },
+        [&](int li, int gi, int row, bool ok, VD b) {
+          if (ok) {
+            const SLog<VW> r = log_result<VW>(sm.buf, sm.L, G, li, row);
+            vst<VW>(Rc + img + gi, r.s * vexpf(vf(r.x + b)));  // -r (:805)
+          }
+        });
+  }
+}
+
 // Vector width 4 when rows hold whole float4 groups.
 static void launch_forward(const SolveArgs& a, int grid, int lds, hipStream_t st, int s) {
+  if (a.logA) {
+    if (a.G.W % 4 == 0)
+      hipLaunchKernelGGL(k_forward_log<4>, dim3(grid), dim3(kThreads), lds, st, a, s);
+    else
+      hipLaunchKernelGGL(k_forward_log<1>, dim3(grid), dim3(kThreads), lds, st, a, s);
+    return;
+  }
   if (a.G.W % 4 == 0)
     hipLaunchKernelGGL(k_forward<4>, dim3(grid), dim3(kThreads), lds, st, a, s);
   else
@@ -371,6 +653,13 @@ static void launch_forward(const SolveArgs& a, int grid, int lds, hipStream_t st
 }
 
 static void launch_backward(const SolveArgs& a, int grid, int lds, hipStream_t st, int s) {
+  if (a.logA) {
+    if (a.G.W % 4 == 0)
+      hipLaunchKernelGGL(k_backward_log<4>, dim3(grid, 2), dim3(kThreads), lds, st, a, s);
+    else
+      hipLaunchKernelGGL(k_backward_log<1>, dim3(grid, 2), dim3(kThreads), lds, st, a, s);
+    return;
+  }
   if (a.G.W % 4 == 0)
     hipLaunchKernelGGL(k_backward<4>, dim3(grid, 2), dim3(kThreads), lds, st, a, s);
   else
@@ -388,7 +677,7 @@ struct wdl_solver {
   long long N = 0;
   int D = 0;
   DevBuf tab, A, lam, obs, Bh, Ch, Fh, V0, V1, R0, R1, GD, GL, Q, G0, lossPart, bStore, bTemp,
-      GDsum;
+      GDsum, logA, events;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool observed = false, problem = false, ran = false;
@@ -426,7 +715,14 @@ struct wdl_solver {
     const bool warm = flags & WDL_FLAG_WARM_RESTART;
     a.bInit = warm ? bStore.as<float>() : nullptr;
     a.bSave = warm ? bTemp.as<float>() : nullptr;
+    const bool logd = flags & WDL_FLAG_LOG_DOMAIN;
+    a.logA = logd ? logA.as<float>() : nullptr;
+    a.events = logd ? events.as<unsigned>() : nullptr;
     return a;
+  }
+
+  int lds_bytes(const SolveArgs& a) const {
+    return conv_lds_bytes(a.G) + ((flags & WDL_FLAG_LOG_DOMAIN) ? kLogKeysBytes : 0);
   }
 
   // Side streams for the batched form: histogram groups advance on their own
@@ -476,6 +772,8 @@ struct wdl_solver {
   // All P solves; with several groups, fork onto the side streams and join
   // back on the context stream before the final reduction.
   int enqueue_batched() {
+    if (flags & WDL_FLAG_LOG_DOMAIN)
+      hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, ctx->stream, events.as<unsigned>());
     if (ngroups <= 1) return enqueue(0, P, true, ctx->stream);
     WDL_HIP(hipEventRecord(fork, ctx->stream));
     int st;
@@ -500,14 +798,17 @@ struct wdl_solver {
 
   int enqueue(int p0, int np, bool reduce, hipStream_t st) {
     const SolveArgs a = args(p0, np);
-    const int lds = conv_lds_bytes(a.G);
+    const int lds = lds_bytes(a);
     const dim3 blk(kThreads);
     const int gs = stage_grid(np, K);
     for (int s = 1; s <= L; ++s)
       launch_forward(a, gs, lds, st, s);
     // kLossBlocks blocks per histogram: every lossPart slot is written
     // (blocks past the last pixel write 0).
-    hipLaunchKernelGGL(k_final, dim3(kLossBlocks, np), blk, 0, st, a);
+    if (a.logA)
+      hipLaunchKernelGGL(k_final_log, dim3(kLossBlocks, np), blk, 0, st, a);
+    else
+      hipLaunchKernelGGL(k_final, dim3(kLossBlocks, np), blk, 0, st, a);
     for (int s = L; s >= 1; --s)
       launch_backward(a, gs, lds, st, s);
     WDL_HIP(hipGetLastError());
@@ -523,18 +824,24 @@ static int set_lds_attr() {
                        (const void*)k_backward<1>, (const void*)k_backward<4>};
   for (const void* f : fns)
     WDL_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+  const void* lfns[] = {(const void*)k_forward_log<1>, (const void*)k_forward_log<4>,
+                        (const void*)k_backward_log<1>, (const void*)k_backward_log<4>};
+  for (const void* f : lfns)
+    WDL_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                bytes + kLogKeysBytes));
   done = true;
   return WDL_OK;
 }
 
 static long long footprint(int W, int H, int K, int P, int L, int flags) {
   const long long N = (long long)W * H, PKN = (long long)P * K * N;
-  long long f = 3LL * L * PKN * 4;           // Bh, Ch, Fh
+  long long f = 3LL * L * PKN * ((flags & WDL_FLAG_LOG_DOMAIN) ? 8 : 4);  // Bh, Ch, Fh
   f += 5 * PKN * 4;                          // V0, V1, R0, R1, GD
   f += (long long)K * N * (4 + 8);           // A, GDsum
   f += (long long)P * N * 4 * 3;             // obs, Q, G0
   f += (long long)P * K * (4 + 8) + (long long)P * kLossBlocks * 8;
   if (flags & WDL_FLAG_WARM_RESTART) f += 2 * PKN * 4;
+  if (flags & WDL_FLAG_LOG_DOMAIN) f += (long long)K * N * 4;  // log atoms
   f += kTabLen * 4;
   return f;
 }
@@ -557,8 +864,6 @@ int wdl_solver_create(wdl_ctx* ctx, int W, int H, int K, int P, int L, double ga
             "wdl_solver_create: sides above " + std::to_string(kMaxSide) + " not supported");
   WDL_CHECK(loss == WDL_LOSS_QUADRATIC || loss == WDL_LOSS_KL || loss == WDL_LOSS_TV,
             WDL_ERR_UNSUPPORTED, "wdl_solver_create: loss must be TV (1), quadratic (2) or KL (4)");
-  WDL_CHECK(!(flags & WDL_FLAG_LOG_DOMAIN), WDL_ERR_UNSUPPORTED,
-            "wdl_solver_create: log domain solver not available in this build");
   WDL_HIP(hipSetDevice(ctx->device));
   int st = set_lds_attr();
   if (st) return st;
@@ -577,6 +882,7 @@ int wdl_solver_create(wdl_ctx* ctx, int W, int H, int K, int P, int L, double ga
   s->D = build_table(W, H, gamma, tab);
   const long long N = s->N, PKN = (long long)P * K * N;
   const size_t f4 = sizeof(float);
+  const size_t hsz = (flags & WDL_FLAG_LOG_DOMAIN) ? sizeof(double) : sizeof(float);  // histories
   struct {
     DevBuf* b;
     size_t n;
@@ -585,9 +891,9 @@ int wdl_solver_create(wdl_ctx* ctx, int W, int H, int K, int P, int L, double ga
       {&s->A, (size_t)K * N * f4},
       {&s->lam, (size_t)P * K * f4},
       {&s->obs, (size_t)P * N * f4},
-      {&s->Bh, (size_t)L * PKN * f4},
-      {&s->Ch, (size_t)L * PKN * f4},
-      {&s->Fh, (size_t)L * PKN * f4},
+      {&s->Bh, (size_t)L * PKN * hsz},
+      {&s->Ch, (size_t)L * PKN * hsz},
+      {&s->Fh, (size_t)L * PKN * hsz},
       {&s->V0, (size_t)PKN * f4},
       {&s->V1, (size_t)PKN * f4},
       {&s->R0, (size_t)PKN * f4},
@@ -600,6 +906,8 @@ int wdl_solver_create(wdl_ctx* ctx, int W, int H, int K, int P, int L, double ga
       {&s->GDsum, (size_t)K * N * sizeof(double)},
       {&s->bStore, (flags & WDL_FLAG_WARM_RESTART) ? (size_t)PKN * f4 : 0},
       {&s->bTemp, (flags & WDL_FLAG_WARM_RESTART) ? (size_t)PKN * f4 : 0},
+      {&s->logA, (flags & WDL_FLAG_LOG_DOMAIN) ? (size_t)K * N * f4 : 0},
+      {&s->events, sizeof(unsigned)},
   };
   for (auto& al : allocs) {
     if ((st = al.b->ensure(al.n))) {
@@ -646,7 +954,8 @@ int wdl_solver_destroy(wdl_solver* s) {
   s->release_streams();
   DevBuf* bufs[] = {&s->tab, &s->A,  &s->lam, &s->obs, &s->Bh, &s->Ch,       &s->Fh,
                     &s->V0,  &s->V1, &s->R0,  &s->R1,  &s->GD, &s->GL,       &s->Q,
-                    &s->G0,  &s->lossPart,    &s->bStore,      &s->bTemp,    &s->GDsum};
+                    &s->G0,  &s->lossPart,    &s->bStore,      &s->bTemp,    &s->GDsum,
+                    &s->logA, &s->events};
   for (DevBuf* b : bufs) b->release();
   delete s;
   return WDL_OK;
@@ -693,6 +1002,14 @@ int wdl_solver_set_problem(wdl_solver* s, const double* atoms, const double* lam
   std::vector<float> ha(s->hostAtoms.begin(), s->hostAtoms.end());
   std::vector<float> hl(s->hostLam.begin(), s->hostLam.end());
   WDL_HIP(hipSetDevice(s->ctx->device));
+  if (s->flags & WDL_FLAG_LOG_DOMAIN) {
+    // log(pdfs) of :704, :756, :796, from the FP64 atoms
+    std::vector<float> hla(ha.size());
+    for (size_t e = 0; e < hla.size(); ++e) hla[e] = (float)std::log(s->hostAtoms[e]);
+    WDL_HIP(hipMemcpyAsync(s->logA.p, hla.data(), hla.size() * sizeof(float),
+                           hipMemcpyHostToDevice, s->ctx->stream));
+    WDL_HIP(hipStreamSynchronize(s->ctx->stream));
+  }
   WDL_HIP(hipMemcpyAsync(s->A.p, ha.data(), ha.size() * sizeof(float), hipMemcpyHostToDevice,
                          s->ctx->stream));
   WDL_HIP(hipMemcpyAsync(s->lam.p, hl.data(), hl.size() * sizeof(float), hipMemcpyHostToDevice,
@@ -709,6 +1026,12 @@ int wdl_solver_set_problem_device(wdl_solver* s, const float* d_atoms, const flo
                          hipMemcpyDeviceToDevice, s->ctx->stream));
   WDL_HIP(hipMemcpyAsync(s->lam.p, d_lambdas, (size_t)s->P * s->K * sizeof(float),
                          hipMemcpyDeviceToDevice, s->ctx->stream));
+  if (s->flags & WDL_FLAG_LOG_DOMAIN) {
+    const long long KN = (long long)s->K * s->N;
+    hipLaunchKernelGGL(k_log, dim3((unsigned)std::min<long long>(1024, (KN + 255) / 256)), dim3(256),
+                       0, s->ctx->stream, s->A.as<float>(), s->logA.as<float>(), KN);
+    WDL_HIP(hipGetLastError());
+  }
   s->problem = true;
   return WDL_OK;
 }
@@ -760,6 +1083,8 @@ int wdl_solver_run(wdl_solver* s, int mode) {
   WDL_HIP(hipSetDevice(s->ctx->device));
   int st;
   if (mode == WDL_RUN_PLAIN) {
+    if (s->flags & WDL_FLAG_LOG_DOMAIN)
+      hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s->ctx->stream, s->events.as<unsigned>());
     for (int p = 0; p < s->P; ++p)
       if ((st = s->enqueue(p, 1, p == s->P - 1, s->ctx->stream))) return st;
   } else if (s->flags & WDL_FLAG_NO_GRAPH) {
@@ -792,7 +1117,7 @@ int wdl_solver_run_stage(wdl_solver* s, int stage, int step) {
             "wdl_solver_run_stage: unknown stage");
   WDL_HIP(hipSetDevice(s->ctx->device));
   const SolveArgs a = s->args(0, s->P);
-  const int lds = conv_lds_bytes(a.G);  // all P solves, on the context stream
+  const int lds = s->lds_bytes(a);  // all P solves, on the context stream
   const int gs = stage_grid(s->P, s->K);
   if (stage == WDL_STAGE_FORWARD)
     launch_forward(a, gs, lds, s->ctx->stream, step);
@@ -833,6 +1158,16 @@ int wdl_solver_get_results(wdl_solver* s, double* loss, double* grad_lambda, dou
     }
   if (barycenters)
     for (size_t e = 0; e < q.size(); ++e) barycenters[e] = q[e];
+  return WDL_OK;
+}
+
+int wdl_solver_log_range_events(wdl_solver* s, long long* count) {
+  WDL_CHECK(s && count, WDL_ERR_ARG, "wdl_solver_log_range_events: NULL argument");
+  unsigned h = 0;
+  WDL_HIP(hipSetDevice(s->ctx->device));
+  WDL_HIP(hipMemcpyAsync(&h, s->events.p, sizeof(unsigned), hipMemcpyDeviceToHost, s->ctx->stream));
+  WDL_HIP(hipStreamSynchronize(s->ctx->stream));
+  *count = h;
   return WDL_OK;
 }
 
