@@ -33,8 +33,10 @@ def test_version_and_footprint():
     W, H, K, P, L = 128, 128, 5, 64, 100
     f = wdl.SinkhornSolver.footprint(W, H, K, P, L)
     PKN = P * K * W * H
-    assert f >= 3 * L * PKN * 4          # b, conv_b, phi histories in FP32
-    assert f < 3 * L * PKN * 4 * 1.1
+    hist = 2 * L * PKN * 4 + L * P * W * H * 4   # conv_b, phi (FP32) and q per step
+    assert hist <= f < hist * 1.1
+    logf = wdl.SinkhornSolver.footprint(W, H, K, P, L, wdl.FLAG_LOG_DOMAIN)
+    assert logf >= 3 * L * PKN * 8               # log b, conv_b, phi in FP64
     assert wdl.SinkhornSolver.footprint(W, H, K, P, L, wdl.FLAG_WARM_RESTART) == f + 2 * PKN * 4
     assert wdl.lib().wdl_solver_footprint(0, H, K, P, L, 0) == -1
 

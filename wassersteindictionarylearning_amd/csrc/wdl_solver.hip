@@ -4,7 +4,9 @@
 //
 // Device layout (FP32 unless noted; pixel index fastest):
 //   A   [K][N]        atoms (normalised)            lam [P][K]   obs [P][N]
-//   Bh  [L][P][K][N]  Bh[s-1] = b^s   (s = 1..L)    history b    (:553-558)
+//   Qh  [L][P][N]     Qh[s-1] = q^s   (s = 1..L); the history b^s = q^s / phi^s
+//                     (:587-592) is recomputed from it bit for bit
+//   (log domain: Bh [L][P][K][N] FP64, log b^s)
 //   Ch  [L][P][K][N]  Ch[s-1] = c^{s-1} = K b^{s-1}  history conv_b
 //   Fh  [L][P][K][N]  Fh[s-1] = phi^s = K a^s        history phi
 //   V0/V1, R0/R1 [P][K][N]  ping-pong v (atom chain) and r (weight chain)
@@ -33,11 +35,13 @@ struct SolveArgs {
   int K, P, L, p0, np, loss;
   long long N;      // pixels per image
   long long stage;  // P*K*N floats between consecutive history entries
+  long long qstage; // P*N floats between consecutive Qh entries
   const float* tab;
   const float* A;
   const float* lam;
   const float* obs;
   float* Bh;
+  float* Qh;  // standard domain: q^s per histogram, L x P x N
   float* Ch;
   float* Fh;
   float* V0;
@@ -74,6 +78,28 @@ __device__ inline bool map_block(const SolveArgs& a, int& p, int& i) {
 }
 
 static int stage_grid(int np, int K) { return np >= 8 ? 8 * K * ((np + 7) / 8) : np * K; }
+
+// Backward: two workgroups (dictionary chain, weight chain) per (histogram,
+// atom), on block ids b and b + 8 (np >= 8) -- the same XCD, dispatched
+// together -- so the arrays both chains read come from one L2.
+__device__ inline bool map_block_bwd(const SolveArgs& a, int& p, int& i, int& chain) {
+  const int b = blockIdx.x;
+  int pl;
+  if (a.np >= 8) {
+    const int xcd = b & 7, slot = b >> 3;
+    chain = slot & 1;
+    const int s2 = slot >> 1;
+    pl = (s2 / a.K) * 8 + xcd;
+    i = s2 % a.K;
+  } else {
+    chain = b & 1;
+    pl = (b >> 1) / a.K;
+    i = (b >> 1) % a.K;
+  }
+  if (pl >= a.np) return false;
+  p = a.p0 + pl;
+  return true;
+}
 
 __device__ inline float loss_grad(int loss, float q, float o) {
   if (loss == WDL_LOSS_KL) return safe_logf(q / o);          // loss.h:83-89
@@ -127,7 +153,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_forward(SolveArgs a, int s) {
         [&](int li, int gi, bool ok, V b) { vst<VW>(buf + li, ok ? b : vsplat<VW>(0.f)); });
   } else {
     const float* Fp = a.Fh + (long long)(s - 2) * a.stage + hbase;  // phi^{s-1}
-    float* Bd = a.Bh + (long long)(s - 2) * a.stage + img;          // b^{s-1}
+    float* Qd = i == 0 ? a.Qh + (long long)(s - 2) * a.qstage + (long long)p * N : nullptr;
     const float* lam = a.lam + (long long)p * K;
     struct LdF { V lq, fi; };
     px_pass<VW>(
@@ -146,8 +172,9 @@ __global__ __launch_bounds__(kThreads, 4) void k_forward(SolveArgs a, int s) {
         [&](int li, int gi, bool ok, const LdF& r) {
           V b = vsplat<VW>(0.f);
           if (ok) {
-            b = vexpf(r.lq) / r.fi;
-            vst<VW>(Bd + gi, b);
+            const V q = vexpf(r.lq);
+            if (Qd) vst<VW>(Qd + gi, q);
+            b = q / r.fi;
           }
           vst<VW>(buf + li, b);
         });
@@ -186,7 +213,7 @@ __global__ __launch_bounds__(kThreads) void k_final(SolveArgs a) {
   const int K = a.K;
   const long long hbase = (long long)p * K * N;
   const float* F = a.Fh + (long long)(a.L - 1) * a.stage + hbase;
-  float* B = a.Bh + (long long)(a.L - 1) * a.stage + hbase;
+  float* QL = a.Qh + (long long)(a.L - 1) * a.qstage + (long long)p * N;
   const float* lam = a.lam + (long long)p * K;
   double acc = 0;
   for (long long x = (long long)blockIdx.x * kThreads + threadIdx.x; x < N;
@@ -194,11 +221,9 @@ __global__ __launch_bounds__(kThreads) void k_final(SolveArgs a) {
     float lq = 0.f;
     for (int j = 0; j < K; ++j) lq += lam[j] * safe_logf(F[j * N + x]);
     const float q = expf(lq);
-    for (int j = 0; j < K; ++j) {
-      const float b = q / F[j * N + x];
-      B[j * N + x] = b;
-      if (a.bSave) a.bSave[hbase + j * N + x] = b;
-    }
+    QL[x] = q;
+    if (a.bSave)
+      for (int j = 0; j < K; ++j) a.bSave[hbase + j * N + x] = q / F[j * N + x];
     const float o = a.obs[(long long)p * N + x];
     a.Q[(long long)p * N + x] = q;
     a.G0[(long long)p * N + x] = loss_grad(a.loss, q, o);
@@ -208,13 +233,13 @@ __global__ __launch_bounds__(kThreads) void k_final(SolveArgs a) {
   if (threadIdx.x == 0) a.lossPart[(long long)p * kLossBlocks + blockIdx.x] = t;
 }
 
-// Backward step s (L..1).  blockIdx.y = 0: dictionary gradient (:600-629);
-// blockIdx.y = 1: weight gradient (:635-673).
+// Backward step s (L..1).  chain 0: dictionary gradient (:600-629);
+// chain 1: weight gradient (:635-673).
 template <int VW>
 __global__ __launch_bounds__(kThreads, 4) void k_backward(SolveArgs a, int s) {
   typedef typename Vec<VW>::T V;
-  int p, i;
-  if (!map_block(a, p, i)) return;
+  int p, i, chain;
+  if (!map_block_bwd(a, p, i, chain)) return;
   extern __shared__ __align__(16) float smem[];
   float* tabs = smem;
   float* buf = smem + 4 * kTabLen;
@@ -230,16 +255,17 @@ __global__ __launch_bounds__(kThreads, 4) void k_backward(SolveArgs a, int s) {
   const bool last = (s == a.L);
   struct Two { V x, y; };
   struct Three { V x, y, z; };
-  if (blockIdx.y == 0) {
+  if (chain == 0) {
     const float* Vp = (s & 1) ? a.V0 : a.V1;
     float* Vc = (s & 1) ? a.V1 : a.V0;
-    const float* Bs = a.Bh + (long long)(s - 1) * a.stage + img;  // b^s
+    const float* Qs = a.Qh + (long long)(s - 1) * a.qstage + (long long)p * N;  // q^s
+    const float* Fs = a.Fh + (long long)(s - 1) * a.stage + img;                // phi^s
     const float* G0 = a.G0 + (long long)p * N;
     px_pass<VW>(
         G,
         [&](int gi) {
-          Three r;  // n, v_i, b^s
-          r.z = vld<VW>(Bs + gi);
+          Three r;  // n, v_i, b^s = q^s / phi^s
+          r.z = vld<VW>(Qs + gi) / vld<VW>(Fs + gi);
           if (last) {
             r.x = vld<VW>(G0 + gi);
             r.y = vsplat<VW>(0.f);
@@ -338,9 +364,10 @@ __global__ __launch_bounds__(kThreads, 4) void k_backward(SolveArgs a, int s) {
           vst<VW>(buf + li, ok ? vld<VW>(buf + li) * (-r.y / (r.x * r.x)) : vsplat<VW>(0.f));
         });
     conv_lds(buf, tabs, G);
-    const float* Bm = a.Bh + (long long)(s - 2) * a.stage + img;  // b^{s-1}
+    const float* Qm = a.Qh + (long long)(s - 2) * a.qstage + (long long)p * N;  // q^{s-1}
+    const float* Fm = a.Fh + (long long)(s - 2) * a.stage + img;                // phi^{s-1}
     px_pass<VW>(
-        G, [&](int gi) { return vld<VW>(Bm + gi); },
+        G, [&](int gi) { return vld<VW>(Qm + gi) / vld<VW>(Fm + gi); },  // b^{s-1}
         [&](int li, int gi, bool ok, V b) {
           if (ok) vst<VW>(Rc + img + gi, vld<VW>(buf + li) * b);
         });
@@ -499,8 +526,8 @@ __global__ __launch_bounds__(kThreads) void k_final_log(SolveArgs a) {
 template <int VW>
 __global__ __launch_bounds__(kThreads, 4) void k_backward_log(SolveArgs a, int s) {
   typedef typename Vec<VW>::T V;
-  int p, i;
-  if (!map_block(a, p, i)) return;
+  int p, i, chain;
+  if (!map_block_bwd(a, p, i, chain)) return;
   extern __shared__ __align__(16) float smem[];
   const ConvGeom G = a.G;
   const LogSmem sm = log_smem(smem, G);
@@ -528,7 +555,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_backward_log(SolveArgs a, int s
     r.y = vld<VW>(LA + gi);
     return r;
   };
-  if (blockIdx.y == 0) {  // dictionary gradient, :736-770
+  if (chain == 0) {  // dictionary gradient, :736-770
     const float* Vp = (s & 1) ? a.V0 : a.V1;
     float* Vc = (s & 1) ? a.V1 : a.V0;
     const double* Bs = (const double*)a.Bh + (long long)(s - 1) * a.stage + img;  // log b^s
@@ -655,15 +682,15 @@ static void launch_forward(const SolveArgs& a, int grid, int lds, hipStream_t st
 static void launch_backward(const SolveArgs& a, int grid, int lds, hipStream_t st, int s) {
   if (a.logA) {
     if (a.G.W % 4 == 0)
-      hipLaunchKernelGGL(k_backward_log<4>, dim3(grid, 2), dim3(kThreads), lds, st, a, s);
+      hipLaunchKernelGGL(k_backward_log<4>, dim3(2 * grid), dim3(kThreads), lds, st, a, s);
     else
-      hipLaunchKernelGGL(k_backward_log<1>, dim3(grid, 2), dim3(kThreads), lds, st, a, s);
+      hipLaunchKernelGGL(k_backward_log<1>, dim3(2 * grid), dim3(kThreads), lds, st, a, s);
     return;
   }
   if (a.G.W % 4 == 0)
-    hipLaunchKernelGGL(k_backward<4>, dim3(grid, 2), dim3(kThreads), lds, st, a, s);
+    hipLaunchKernelGGL(k_backward<4>, dim3(2 * grid), dim3(kThreads), lds, st, a, s);
   else
-    hipLaunchKernelGGL(k_backward<1>, dim3(grid, 2), dim3(kThreads), lds, st, a, s);
+    hipLaunchKernelGGL(k_backward<1>, dim3(2 * grid), dim3(kThreads), lds, st, a, s);
 }
 
 }  // namespace wdl
@@ -676,8 +703,8 @@ struct wdl_solver {
   double gamma = 0;
   long long N = 0;
   int D = 0;
-  DevBuf tab, A, lam, obs, Bh, Ch, Fh, V0, V1, R0, R1, GD, GL, Q, G0, lossPart, bStore, bTemp,
-      GDsum, logA, events;
+  DevBuf tab, A, lam, obs, Bh, Qh, Ch, Fh, V0, V1, R0, R1, GD, GL, Q, G0, lossPart, bStore,
+      bTemp, GDsum, logA, events;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool observed = false, problem = false, ran = false;
@@ -701,6 +728,8 @@ struct wdl_solver {
     a.lam = lam.as<float>();
     a.obs = obs.as<float>();
     a.Bh = Bh.as<float>();
+    a.Qh = Qh.as<float>();
+    a.qstage = (long long)P * N;
     a.Ch = Ch.as<float>();
     a.Fh = Fh.as<float>();
     a.V0 = V0.as<float>();
@@ -835,7 +864,8 @@ static int set_lds_attr() {
 
 static long long footprint(int W, int H, int K, int P, int L, int flags) {
   const long long N = (long long)W * H, PKN = (long long)P * K * N;
-  long long f = 3LL * L * PKN * ((flags & WDL_FLAG_LOG_DOMAIN) ? 8 : 4);  // Bh, Ch, Fh
+  long long f = (flags & WDL_FLAG_LOG_DOMAIN) ? 3LL * L * PKN * 8           // Bh, Ch, Fh (FP64)
+                                             : 2LL * L * PKN * 4 + (long long)L * P * N * 4;  // Ch, Fh, Qh
   f += 5 * PKN * 4;                          // V0, V1, R0, R1, GD
   f += (long long)K * N * (4 + 8);           // A, GDsum
   f += (long long)P * N * 4 * 3;             // obs, Q, G0
@@ -891,7 +921,8 @@ int wdl_solver_create(wdl_ctx* ctx, int W, int H, int K, int P, int L, double ga
       {&s->A, (size_t)K * N * f4},
       {&s->lam, (size_t)P * K * f4},
       {&s->obs, (size_t)P * N * f4},
-      {&s->Bh, (size_t)L * PKN * hsz},
+      {&s->Bh, (flags & WDL_FLAG_LOG_DOMAIN) ? (size_t)L * PKN * hsz : 0},
+      {&s->Qh, (flags & WDL_FLAG_LOG_DOMAIN) ? 0 : (size_t)L * P * N * f4},
       {&s->Ch, (size_t)L * PKN * hsz},
       {&s->Fh, (size_t)L * PKN * hsz},
       {&s->V0, (size_t)PKN * f4},
@@ -952,7 +983,7 @@ int wdl_solver_destroy(wdl_solver* s) {
   if (s->exec) (void)hipGraphExecDestroy(s->exec);
   if (s->graph) (void)hipGraphDestroy(s->graph);
   s->release_streams();
-  DevBuf* bufs[] = {&s->tab, &s->A,  &s->lam, &s->obs, &s->Bh, &s->Ch,       &s->Fh,
+  DevBuf* bufs[] = {&s->tab, &s->A,  &s->lam, &s->obs, &s->Bh, &s->Qh, &s->Ch,  &s->Fh,
                     &s->V0,  &s->V1, &s->R0,  &s->R1,  &s->GD, &s->GL,       &s->Q,
                     &s->G0,  &s->lossPart,    &s->bStore,      &s->bTemp,    &s->GDsum,
                     &s->logA, &s->events};
