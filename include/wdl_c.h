@@ -38,6 +38,7 @@ extern "C" {
 #define WDL_FLAG_WARM_RESTART 2 /* solver keeps b_storage / b_temp (inverseWasserstein.h:380-387) */
 #define WDL_FLAG_DENSE 4        /* do not skip all-zero FP32 factor chunks */
 #define WDL_FLAG_NO_GRAPH 8     /* batched runs launch eagerly instead of replaying a hipGraph */
+#define WDL_FLAG_WEIGHTS_ONLY 16 /* only dL/dlambda (:635-673): projection onto a fixed dictionary */
 
 /* Run modes of wdl_solver_run. */
 #define WDL_RUN_BATCHED 0 /* all P solves advance together, one launch per Sinkhorn stage */
@@ -105,6 +106,11 @@ int wdl_solver_warm_commit(wdl_solver* s);
  * on its own stream (default min(4, P/16), at least 1).  No counterpart in
  * the reference; a tuning knob of the batched launch. */
 int wdl_solver_set_streams(wdl_solver* s, int nstreams);
+/* Per-histogram Sinkhorn iterations (1..L each; NULL = L for all): histogram p
+ * runs iterations[p] forward steps and as many backward steps (the reference
+ * has one Niters for all, inverseWasserstein.h:546; uneven counts serve
+ * requests that arrive with their own budgets). */
+int wdl_solver_set_iterations(wdl_solver* s, const int* iterations);
 /* Enqueue the forward L iterations and both unrolled backwards for all P
  * solves (compute_gradient_both, inverseWasserstein.h:544-674, or its log
  * form :678-814).  Asynchronous on the context stream. */

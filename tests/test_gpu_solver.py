@@ -184,3 +184,45 @@ def test_evaluate_both_config_shape(ctx):
     assert abs(f - fo) <= TOL * abs(fo)
     assert rel(gr[:P * K], go[:P * K]) < TOL
     assert rel(gr[P * K:], go[P * K:]) < TOL
+
+
+@pytest.mark.parametrize("flags", [0, wdl.FLAG_LOG_DOMAIN])
+def test_uneven_iterations_and_weights_only(ctx, flags):
+    # Each histogram runs its own number of Sinkhorn iterations (and as many
+    # backward steps); the weights-only mode gives the same weight gradients.
+    W = H = 20
+    K, P, L, g = 3, 7, 10, 1.0
+    # (with very few iterations dL/dlambda is a handful of dot products with
+    # heavy cancellation -- one iteration: 9e-5 in FP32, four in the log
+    # domain: 1.2e-5 -- beyond what FP32 inputs resolve to 1e-5; counts here
+    # start at 5)
+    iters = np.array([5, 7, 10, 6, 5, 10, 8], np.int32)
+    print("uneven iterations flags=%d" % flags, flush=True)
+    rng = np.random.default_rng(21)
+    obs = synthetic.blob_histograms(rng, W, P, (1, 3), (2, 5))
+    atoms, lam = synthetic.init_atoms_weights(rng, K, P, W * H)
+    out = {}
+    for extra in (0, wdl.FLAG_WEIGHTS_ONLY):
+        s = wdl.SinkhornSolver(ctx, W, H, K, P, L, g, 2, flags | extra | wdl.FLAG_WARM_RESTART)
+        s.set_observed(obs)
+        s.set_problem(atoms, lam)
+        s.set_iterations(iters)
+        s.run()
+        out[extra] = s.results()
+        out[extra]["b_temp"] = s.get_warm_state()
+        if not extra:
+            out[extra]["grad_dic"] = s.grad_dic()
+        s.close()
+    full, wo = out[0], out[wdl.FLAG_WEIGHTS_ONLY]
+    np.testing.assert_array_equal(wo["grad_lambda"], full["grad_lambda"])
+    np.testing.assert_array_equal(wo["bary"], full["bary"])
+    assert not wo["grad_dic_sum"].any()
+    solve = O.gradient_both_log if flags else O.gradient_both
+    ones = np.ones((K, W * H))  # b_storage starts at 1 in both domains (:380-384)
+    for p in range(P):
+        o = solve(atoms, lam[p], obs[p], W, H, g, int(iters[p]), 2, b0=ones)
+        assert rel(full["grad_lambda"][p], o["grad_lambda"]) < TOL, p
+        assert rel(full["grad_dic"][p], o["grad_dic"]) < TOL, p
+        assert rel(full["bary"][p], o["bary"]) < TOL, p
+        b_last = o["b_last"]
+        assert rel(full["b_temp"][p], b_last) < TOL, p

@@ -20,7 +20,7 @@ LIB_PATH = os.path.join(_PKG, "_lib", "libwdl.so")
 
 WDL_OK = 0
 LOSS_TV, LOSS_QUADRATIC, LOSS_WASSERSTEIN, LOSS_KL = 1, 2, 3, 4
-FLAG_LOG_DOMAIN, FLAG_WARM_RESTART, FLAG_DENSE, FLAG_NO_GRAPH = 1, 2, 4, 8
+FLAG_LOG_DOMAIN, FLAG_WARM_RESTART, FLAG_DENSE, FLAG_NO_GRAPH, FLAG_WEIGHTS_ONLY = 1, 2, 4, 8, 16
 RUN_BATCHED, RUN_PLAIN = 0, 1
 STAGE_FORWARD, STAGE_BACKWARD = 0, 1
 
@@ -59,6 +59,7 @@ _SIGS = {
     "wdl_solver_run": (_i, [_vp, _i]),
     "wdl_solver_run_stage": (_i, [_vp, _i, _i]),
     "wdl_solver_set_streams": (_i, [_vp, _i]),
+    "wdl_solver_set_iterations": (_i, [_vp, ctypes.POINTER(_i)]),
     "wdl_solver_get_results": (_i, [_vp, _d, _d, _d, _d]),
     "wdl_solver_get_grad_dic": (_i, [_vp, _d]),
     "wdl_solver_log_range_events": (_i, [_vp, ctypes.POINTER(ctypes.c_longlong)]),
@@ -249,6 +250,15 @@ class SinkhornSolver:
 
     def run(self, mode: int = RUN_BATCHED):
         _check(lib().wdl_solver_run(self.handle, mode), "wdl_solver_run")
+
+    def set_iterations(self, iterations=None):
+        if iterations is None:
+            _check(lib().wdl_solver_set_iterations(self.handle, None), "wdl_solver_set_iterations")
+            return
+        it = np.ascontiguousarray(iterations, dtype=np.int32)
+        assert it.size == self.P
+        _check(lib().wdl_solver_set_iterations(self.handle, it.ctypes.data_as(ctypes.POINTER(_i))),
+               "wdl_solver_set_iterations")
 
     def set_streams(self, nstreams: int):
         _check(lib().wdl_solver_set_streams(self.handle, nstreams), "wdl_solver_set_streams")

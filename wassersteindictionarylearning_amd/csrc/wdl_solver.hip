@@ -55,6 +55,8 @@ struct SolveArgs {
   double* lossPart;
   const float* bInit;
   float* bSave;
+  const int* Lp;      // per-histogram Sinkhorn iterations (<= L), or null for L
+  int weightsOnly;    // skip the dictionary-gradient chain
   const float* logA;  // K x N log atoms (log-domain solver)
   unsigned* events;   // lines whose spread exceeded kLogRange (log-domain solver)
 };
@@ -77,6 +79,14 @@ __device__ inline bool map_block(const SolveArgs& a, int& p, int& i) {
   return true;
 }
 
+__device__ inline int steps_of(const SolveArgs& a, int p) { return a.Lp ? a.Lp[p] : a.L; }
+
+// Launch t of the backward (t = L..1) is step t - (L - L_p) of histogram p;
+// histograms with fewer iterations join later.  Returns 0 when p has no step.
+__device__ inline int bwd_step(const SolveArgs& a, int p, int t) {
+  return t - (a.L - steps_of(a, p));
+}
+
 static int stage_grid(int np, int K) { return np >= 8 ? 8 * K * ((np + 7) / 8) : np * K; }
 
 // Backward: two workgroups (dictionary chain, weight chain) per (histogram,
@@ -85,6 +95,10 @@ static int stage_grid(int np, int K) { return np >= 8 ? 8 * K * ((np + 7) / 8) :
 __device__ inline bool map_block_bwd(const SolveArgs& a, int& p, int& i, int& chain) {
   const int b = blockIdx.x;
   int pl;
+  if (a.weightsOnly) {  // one workgroup per (histogram, atom), the weight chain
+    chain = 1;
+    return map_block(a, p, i);
+  }
   if (a.np >= 8) {
     const int xcd = b & 7, slot = b >> 3;
     chain = slot & 1;
@@ -136,7 +150,7 @@ template <int VW>
 __global__ __launch_bounds__(kThreads, 4) void k_forward(SolveArgs a, int s) {
   typedef typename Vec<VW>::T V;
   int p, i;
-  if (!map_block(a, p, i)) return;
+  if (!map_block(a, p, i) || s > steps_of(a, p)) return;
   extern __shared__ __align__(16) float smem[];
   float* tabs = smem;
   float* buf = smem + 4 * kTabLen;
@@ -212,8 +226,9 @@ __global__ __launch_bounds__(kThreads) void k_final(SolveArgs a) {
   const long long N = a.N;
   const int K = a.K;
   const long long hbase = (long long)p * K * N;
-  const float* F = a.Fh + (long long)(a.L - 1) * a.stage + hbase;
-  float* QL = a.Qh + (long long)(a.L - 1) * a.qstage + (long long)p * N;
+  const int Lp = steps_of(a, p);
+  const float* F = a.Fh + (long long)(Lp - 1) * a.stage + hbase;
+  float* QL = a.Qh + (long long)(Lp - 1) * a.qstage + (long long)p * N;
   const float* lam = a.lam + (long long)p * K;
   double acc = 0;
   for (long long x = (long long)blockIdx.x * kThreads + threadIdx.x; x < N;
@@ -236,10 +251,12 @@ __global__ __launch_bounds__(kThreads) void k_final(SolveArgs a) {
 // Backward step s (L..1).  chain 0: dictionary gradient (:600-629);
 // chain 1: weight gradient (:635-673).
 template <int VW>
-__global__ __launch_bounds__(kThreads, 4) void k_backward(SolveArgs a, int s) {
+__global__ __launch_bounds__(kThreads, 4) void k_backward(SolveArgs a, int t) {
   typedef typename Vec<VW>::T V;
   int p, i, chain;
   if (!map_block_bwd(a, p, i, chain)) return;
+  const int s = bwd_step(a, p, t);
+  if (s < 1) return;
   extern __shared__ __align__(16) float smem[];
   float* tabs = smem;
   float* buf = smem + 4 * kTabLen;
@@ -252,7 +269,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_backward(SolveArgs a, int s) {
   const float lpi = a.lam[(long long)p * K + i];
   const float* Ai = a.A + (long long)i * N;
   const float* Cs = a.Ch + (long long)(s - 1) * a.stage + img;  // c^{s-1}
-  const bool last = (s == a.L);
+  const bool last = (s == steps_of(a, p));
   struct Two { V x, y; };
   struct Three { V x, y, z; };
   if (chain == 0) {
@@ -434,7 +451,7 @@ template <int VW>
 __global__ __launch_bounds__(kThreads, 4) void k_forward_log(SolveArgs a, int s) {
   typedef typename Vec<VW>::T V;
   int p, i;
-  if (!map_block(a, p, i)) return;
+  if (!map_block(a, p, i) || s > steps_of(a, p)) return;
   extern __shared__ __align__(16) float smem[];
   const ConvGeom G = a.G;
   const LogSmem sm = log_smem(smem, G);
@@ -500,8 +517,9 @@ __global__ __launch_bounds__(kThreads) void k_final_log(SolveArgs a) {
   const long long N = a.N;
   const int K = a.K;
   const long long hbase = (long long)p * K * N;
-  const double* F = (const double*)a.Fh + (long long)(a.L - 1) * a.stage + hbase;
-  double* B = (double*)a.Bh + (long long)(a.L - 1) * a.stage + hbase;
+  const int Lp = steps_of(a, p);
+  const double* F = (const double*)a.Fh + (long long)(Lp - 1) * a.stage + hbase;
+  double* B = (double*)a.Bh + (long long)(Lp - 1) * a.stage + hbase;
   const float* lam = a.lam + (long long)p * K;
   double acc = 0;
   for (long long x = (long long)blockIdx.x * kThreads + threadIdx.x; x < N;
@@ -524,10 +542,12 @@ __global__ __launch_bounds__(kThreads) void k_final_log(SolveArgs a) {
 }
 
 template <int VW>
-__global__ __launch_bounds__(kThreads, 4) void k_backward_log(SolveArgs a, int s) {
+__global__ __launch_bounds__(kThreads, 4) void k_backward_log(SolveArgs a, int t) {
   typedef typename Vec<VW>::T V;
   int p, i, chain;
   if (!map_block_bwd(a, p, i, chain)) return;
+  const int s = bwd_step(a, p, t);
+  if (s < 1) return;
   extern __shared__ __align__(16) float smem[];
   const ConvGeom G = a.G;
   const LogSmem sm = log_smem(smem, G);
@@ -539,7 +559,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_backward_log(SolveArgs a, int s
   const float lpi = a.lam[(long long)p * K + i];
   const float* LA = a.logA + (long long)i * N;
   const double* Cs = (const double*)a.Ch + (long long)(s - 1) * a.stage + img;  // log c^{s-1}
-  const bool last = (s == a.L);
+  const bool last = (s == steps_of(a, p));
   typedef typename VecD<VW>::T VD;
   struct Three { V x, y; VD z; };
   struct Two { V x, y; };
@@ -679,18 +699,19 @@ static void launch_forward(const SolveArgs& a, int grid, int lds, hipStream_t st
     hipLaunchKernelGGL(k_forward<1>, dim3(grid), dim3(kThreads), lds, st, a, s);
 }
 
-static void launch_backward(const SolveArgs& a, int grid, int lds, hipStream_t st, int s) {
+static void launch_backward(const SolveArgs& a, int grid1, int lds, hipStream_t st, int s) {
+  const int blocks = a.weightsOnly ? grid1 : 2 * grid1;  // one or two chains per (p, i)
   if (a.logA) {
     if (a.G.W % 4 == 0)
-      hipLaunchKernelGGL(k_backward_log<4>, dim3(2 * grid), dim3(kThreads), lds, st, a, s);
+      hipLaunchKernelGGL(k_backward_log<4>, dim3(blocks), dim3(kThreads), lds, st, a, s);
     else
-      hipLaunchKernelGGL(k_backward_log<1>, dim3(2 * grid), dim3(kThreads), lds, st, a, s);
+      hipLaunchKernelGGL(k_backward_log<1>, dim3(blocks), dim3(kThreads), lds, st, a, s);
     return;
   }
   if (a.G.W % 4 == 0)
-    hipLaunchKernelGGL(k_backward<4>, dim3(2 * grid), dim3(kThreads), lds, st, a, s);
+    hipLaunchKernelGGL(k_backward<4>, dim3(blocks), dim3(kThreads), lds, st, a, s);
   else
-    hipLaunchKernelGGL(k_backward<1>, dim3(2 * grid), dim3(kThreads), lds, st, a, s);
+    hipLaunchKernelGGL(k_backward<1>, dim3(blocks), dim3(kThreads), lds, st, a, s);
 }
 
 }  // namespace wdl
@@ -704,7 +725,7 @@ struct wdl_solver {
   long long N = 0;
   int D = 0;
   DevBuf tab, A, lam, obs, Bh, Qh, Ch, Fh, V0, V1, R0, R1, GD, GL, Q, G0, lossPart, bStore,
-      bTemp, GDsum, logA, events;
+      bTemp, GDsum, logA, events, Lp;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool observed = false, problem = false, ran = false;
@@ -744,6 +765,8 @@ struct wdl_solver {
     const bool warm = flags & WDL_FLAG_WARM_RESTART;
     a.bInit = warm ? bStore.as<float>() : nullptr;
     a.bSave = warm ? bTemp.as<float>() : nullptr;
+    a.Lp = Lp.as<int>();
+    a.weightsOnly = (flags & WDL_FLAG_WEIGHTS_ONLY) ? 1 : 0;
     const bool logd = flags & WDL_FLAG_LOG_DOMAIN;
     a.logA = logd ? logA.as<float>() : nullptr;
     a.events = logd ? events.as<unsigned>() : nullptr;
@@ -813,7 +836,7 @@ struct wdl_solver {
       WDL_HIP(hipEventRecord(joins[g], side[g]));
       WDL_HIP(hipStreamWaitEvent(ctx->stream, joins[g], 0));
     }
-    return reduce_gd(ctx->stream);
+    return (flags & WDL_FLAG_WEIGHTS_ONLY) ? WDL_OK : reduce_gd(ctx->stream);
   }
 
   int reduce_gd(hipStream_t st) {
@@ -841,7 +864,7 @@ struct wdl_solver {
     for (int s = L; s >= 1; --s)
       launch_backward(a, gs, lds, st, s);
     WDL_HIP(hipGetLastError());
-    return reduce ? reduce_gd(st) : WDL_OK;
+    return (reduce && !(flags & WDL_FLAG_WEIGHTS_ONLY)) ? reduce_gd(st) : WDL_OK;
   }
 };
 
@@ -939,6 +962,7 @@ int wdl_solver_create(wdl_ctx* ctx, int W, int H, int K, int P, int L, double ga
       {&s->bTemp, (flags & WDL_FLAG_WARM_RESTART) ? (size_t)PKN * f4 : 0},
       {&s->logA, (flags & WDL_FLAG_LOG_DOMAIN) ? (size_t)K * N * f4 : 0},
       {&s->events, sizeof(unsigned)},
+      {&s->Lp, (size_t)P * sizeof(int)},
   };
   for (auto& al : allocs) {
     if ((st = al.b->ensure(al.n))) {
@@ -947,6 +971,10 @@ int wdl_solver_create(wdl_ctx* ctx, int W, int H, int K, int P, int L, double ga
     }
   }
   hipError_t e = hipMemcpy(s->tab.p, tab.data(), tab.size() * f4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    const std::vector<int> lp((size_t)P, L);
+    e = hipMemcpy(s->Lp.p, lp.data(), lp.size() * sizeof(int), hipMemcpyHostToDevice);
+  }
   if (e == hipSuccess && (flags & WDL_FLAG_WARM_RESTART)) {
     // b_storage starts at 1 (inverseWasserstein.h:380-384).
     hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, ctx->stream, s->bStore.as<float>(), 1.f,
@@ -986,7 +1014,7 @@ int wdl_solver_destroy(wdl_solver* s) {
   DevBuf* bufs[] = {&s->tab, &s->A,  &s->lam, &s->obs, &s->Bh, &s->Qh, &s->Ch,  &s->Fh,
                     &s->V0,  &s->V1, &s->R0,  &s->R1,  &s->GD, &s->GL,       &s->Q,
                     &s->G0,  &s->lossPart,    &s->bStore,      &s->bTemp,    &s->GDsum,
-                    &s->logA, &s->events};
+                    &s->logA, &s->events, &s->Lp};
   for (DevBuf* b : bufs) b->release();
   delete s;
   return WDL_OK;
@@ -1174,8 +1202,10 @@ int wdl_solver_get_results(wdl_solver* s, double* loss, double* grad_lambda, dou
   }
   if (grad_lambda)
     WDL_HIP(hipMemcpyAsync(grad_lambda, s->GL.p, (size_t)P * K * 8, hipMemcpyDeviceToHost, st));
-  if (grad_dic_sum)
+  if (grad_dic_sum && !(s->flags & WDL_FLAG_WEIGHTS_ONLY))
     WDL_HIP(hipMemcpyAsync(grad_dic_sum, s->GDsum.p, (size_t)K * N * 8, hipMemcpyDeviceToHost, st));
+  if (grad_dic_sum && (s->flags & WDL_FLAG_WEIGHTS_ONLY))
+    std::memset(grad_dic_sum, 0, (size_t)K * N * sizeof(double));
   if (barycenters) {
     q.resize((size_t)P * N);
     WDL_HIP(hipMemcpyAsync(q.data(), s->Q.p, q.size() * 4, hipMemcpyDeviceToHost, st));
@@ -1202,8 +1232,26 @@ int wdl_solver_log_range_events(wdl_solver* s, long long* count) {
   return WDL_OK;
 }
 
+int wdl_solver_set_iterations(wdl_solver* s, const int* iterations) {
+  WDL_CHECK(s, WDL_ERR_ARG, "wdl_solver_set_iterations: NULL argument");
+  std::vector<int> lp((size_t)s->P, s->L);
+  if (iterations)
+    for (int p = 0; p < s->P; ++p) {
+      WDL_CHECK(iterations[p] >= 1 && iterations[p] <= s->L, WDL_ERR_ARG,
+                "wdl_solver_set_iterations: iterations must be in 1..L");
+      lp[p] = iterations[p];
+    }
+  WDL_HIP(hipSetDevice(s->ctx->device));
+  WDL_HIP(hipMemcpyAsync(s->Lp.p, lp.data(), lp.size() * sizeof(int), hipMemcpyHostToDevice,
+                         s->ctx->stream));
+  WDL_HIP(hipStreamSynchronize(s->ctx->stream));
+  return WDL_OK;
+}
+
 int wdl_solver_get_grad_dic(wdl_solver* s, double* grad_dic) {
   WDL_CHECK(s && grad_dic, WDL_ERR_ARG, "wdl_solver_get_grad_dic: NULL argument");
+  WDL_CHECK(!(s->flags & WDL_FLAG_WEIGHTS_ONLY), WDL_ERR_STATE,
+            "wdl_solver_get_grad_dic: the solver computes weight gradients only");
   WDL_CHECK(s->ran, WDL_ERR_STATE, "wdl_solver_get_grad_dic: nothing has run");
   const size_t n = (size_t)s->P * s->K * s->N;
   std::vector<float> h(n);
