@@ -65,6 +65,85 @@ def conv_flops(n, D):
     return 2 * pass_flops(n, n, D), 4 * n ** 3
 
 
+def projection_leg(ctx, chunk=1024):
+    """Uneven batched projection: a fixed dictionary (K=8 blob atoms, 128x128),
+    4096 histograms, per-request Sinkhorn iterations uniform in 20..100, weights
+    gradient only, b seeded from the warm-restart store.  The requests run in
+    chunks of `chunk` (full unrolled histories for all 4096 would need ~430 GB)."""
+    c = synthetic.make_problem("projection")
+    n, P, K, L, g = c["n"], c["P"], c["K"], c["L"], c["gamma"]
+    rng = np.random.default_rng(7)
+    atoms = synthetic.blob_histograms(rng, n, K, c["nblobs"], c["sigma"])  # the dictionary
+    flags = wdl.FLAG_WEIGHTS_ONLY | wdl.FLAG_WARM_RESTART
+    s = wdl.SinkhornSolver(ctx, n, n, K, chunk, L, g, c["loss"], flags)
+    stream = ctx.stream
+    e0, e1 = _hip.Event(), _hip.Event()
+    total_ms, iters = 0.0, 0
+    for c0 in range(0, P, chunk):
+        sl = slice(c0, c0 + chunk)
+        s.set_observed(c["obs"][sl])
+        s.set_problem(atoms, c["weights"][sl])
+        s.set_iterations(c["iterations"][sl])
+        s.run()  # warm-up of the launch sequence (and the graph) for this chunk
+        ctx.synchronize()
+        e0.record(stream)
+        s.run()
+        e1.record(stream)
+        e1.synchronize()
+        total_ms += e0.elapsed_ms(e1)
+        iters += int(c["iterations"][sl].sum())
+    s.close()
+    e0.destroy()
+    e1.destroy()
+    return {"workload": "projection: 4096 requests, 128x128, K=8 fixed dictionary, gamma=1, "
+                        "L per request uniform 20..100, weights gradient only, warm-restart seed",
+            "requests": P, "chunk": chunk, "sinkhorn_iters": iters,
+            "ms": round(total_ms, 3), "value": round(iters / (total_ms / 1e3), 1),
+            "unit": "Sinkhorn-iters/s"}
+
+
+def log_leg(ctx, steps=2):
+    """Log-domain small-epsilon workload: 256x256 (tiled kernels: the image
+    exceeds one workgroup's LDS), 32 curve histograms, K=6, gamma=0.5, L=200,
+    KL loss; a step is compute_gradient_both_log_signArray for all 32."""
+    c = synthetic.make_problem("log_small_eps")
+    n, P, K, L, g, loss = c["n"], c["P"], c["K"], c["L"], c["gamma"], c["loss"]
+    s = wdl.SinkhornSolver(ctx, n, n, K, P, L, g, loss, wdl.FLAG_LOG_DOMAIN)
+    s.set_observed(c["obs"])
+    s.set_problem(c["atoms"], c["weights"])
+    stream = ctx.stream
+    s.run()  # warm-up and graph capture
+    ctx.synchronize()
+    e0, e1 = _hip.Event(), _hip.Event()
+    e0.record(stream)
+    for _ in range(steps):
+        s.run()
+    e1.record(stream)
+    e1.synchronize()
+    ms = e0.elapsed_ms(e1) / steps
+    stage = {}
+    for name, st in (("forward", wdl.STAGE_FORWARD), ("backward", wdl.STAGE_BACKWARD)):
+        s.run_stage(st, L // 2)
+        e0.record(stream)
+        for _ in range(5):
+            s.run_stage(st, L // 2)
+        e1.record(stream)
+        e1.synchronize()
+        stage[name + "_stage_ms"] = round(e0.elapsed_ms(e1) / 5, 4)
+    events = s.log_range_events()
+    footprint = wdl.SinkhornSolver.footprint(n, n, K, P, L, wdl.FLAG_LOG_DOMAIN)
+    s.close()
+    e0.destroy()
+    e1.destroy()
+    out = {"workload": "log domain, small epsilon: 256x256, 32 curve histograms, K=6, "
+                       "gamma=0.5, L=200, KL loss, all gradients",
+           "steps": steps, "ms_per_step": round(ms, 3),
+           "value": round(P * L / (ms / 1e3), 1), "unit": "Sinkhorn-iters/s",
+           "log_range_events": events, "hbm_footprint_gb": round(footprint / 1e9, 2)}
+    out.update(stage)
+    return out
+
+
 def cpu_baseline(c, gpu, budget_s=20.0):
     """Reference CPU path (oracle/_ref, compiled from the reference sources) on
     a bounded sample of the same workload, and the GPU error against it."""
@@ -112,6 +191,8 @@ def main():
     ap.add_argument("--dense", action="store_true", help="do not skip all-zero factor chunks")
     ap.add_argument("--streams", type=int, default=0,
                     help="histogram groups on separate streams (0: the library default)")
+    ap.add_argument("--no-projection", action="store_true", help="skip the projection leg")
+    ap.add_argument("--no-log", action="store_true", help="skip the 256x256 log-domain leg")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one 256 MiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
@@ -244,6 +325,12 @@ def main():
         "batched_over_plain": (round(value / plain["value"], 2) if plain else None),
         "roofline": roofline,
     }
+    if not args.no_projection:
+        line["projection"] = projection_leg(ctx)
+        log("projection: %s iters/s" % line["projection"]["value"])
+    if not args.no_log:
+        line["log_small_eps"] = log_leg(ctx)
+        log("log domain 256x256: %s iters/s" % line["log_small_eps"]["value"])
     if not args.no_cpu:
         cb, err = cpu_baseline(c, res)
         line["cpu_baseline"] = cb

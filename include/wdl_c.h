@@ -39,6 +39,7 @@ extern "C" {
 #define WDL_FLAG_DENSE 4        /* do not skip all-zero FP32 factor chunks */
 #define WDL_FLAG_NO_GRAPH 8     /* batched runs launch eagerly instead of replaying a hipGraph */
 #define WDL_FLAG_WEIGHTS_ONLY 16 /* only dL/dlambda (:635-673): projection onto a fixed dictionary */
+#define WDL_FLAG_TILED 32       /* 64x64-tile kernels at any size (taken anyway above 128 per side) */
 
 /* Run modes of wdl_solver_run. */
 #define WDL_RUN_BATCHED 0 /* all P solves advance together, one launch per Sinkhorn stage */

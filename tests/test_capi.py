@@ -39,6 +39,11 @@ def test_version_and_footprint():
     assert logf >= 3 * L * PKN * 8               # log b, conv_b, phi in FP64
     assert wdl.SinkhornSolver.footprint(W, H, K, P, L, wdl.FLAG_WARM_RESTART) == f + 2 * PKN * 4
     assert wdl.lib().wdl_solver_footprint(0, H, K, P, L, 0) == -1
+    # tiled sizes add the second-convolution inputs, per-tile dots and log-phi sums
+    ft = wdl.SinkhornSolver.footprint(W, H, K, P, L, wdl.FLAG_TILED)
+    assert ft == f + 2 * PKN * 4 + P * K * 4 * 8 * (1 + L)
+    ftl = wdl.SinkhornSolver.footprint(W, H, K, P, L, wdl.FLAG_TILED | wdl.FLAG_LOG_DOMAIN)
+    assert ftl == logf + 2 * PKN * 9 + P * K * 4 * 8 * (1 + L)
 
 
 def test_null_arguments_are_errors():

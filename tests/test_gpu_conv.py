@@ -71,5 +71,5 @@ def test_convolve_device(ctx):
 def test_errors_are_reported(ctx):
     with pytest.raises(wdl.WdlError):
         wdl.HipGaussian2DKernel(ctx, -1.0, 8, 8)
-    with pytest.raises(wdl.WdlError):
-        wdl.HipGaussian2DKernel(ctx, 1.0, 1000, 8)
+    with pytest.raises(wdl.WdlError):  # tiled size, band half-width 71 > the 32-pixel halo
+        wdl.HipGaussian2DKernel(ctx, 50.0, 200, 8)

@@ -58,6 +58,24 @@ def test_gradient_both_log_matches_oracle(ctx, W, H, K, P, L, g, loss, curves):
         assert abs(r["loss"][p] - O.loss_value(loss, o["bary"], obs[p])) < TOL * abs(r["loss"][p])
 
 
+def test_weight_gradient_common_mode(ctx):
+    # 128x128, K=3, L=30, gamma=1, KL: the weight-gradient dot sums terms ~1e4
+    # times larger than the result, with a large log phi offset common to all
+    # atoms; evaluated plainly its FP32 rounding put dL/dlambda at 2e-5..7e-5
+    # from FP64.  The centred form (exact sum of g) keeps it within 1e-5.
+    W = H = 128
+    K, P, L, g = 3, 9, 30, 1.0
+    print("log weight gradient, common mode", flush=True)
+    rng = np.random.default_rng(100 + W + L)
+    obs = synthetic.blob_histograms(rng, W, P, (1, 3), (2, W / 6))
+    atoms, lam = synthetic.init_atoms_weights(rng, K, P, W * H)
+    r = solve(ctx, atoms, lam, obs, W, H, g, L, 4)
+    for p in (1, 5):
+        o = O.gradient_both_log(atoms, lam[p], obs[p], W, H, g, L, 4)
+        assert rel(r["grad_lambda"][p], o["grad_lambda"]) < TOL, p
+        assert rel(r["grad_dic"][p], o["grad_dic"]) < TOL, p
+
+
 @pytest.mark.parametrize("name,d", fixtures("glog"))
 def test_gradient_both_log_golden(ctx, name, d):
     W, H, L, g, loss = int(d["W"]), int(d["H"]), int(d["L"]), float(d["gamma"]), int(d["loss"])

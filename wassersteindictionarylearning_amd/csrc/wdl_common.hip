@@ -62,6 +62,31 @@ ConvGeom make_geom(int W, int H, int D) {
   return g;
 }
 
+int make_tile_geom(int W, int H, int D, double gamma, bool logdom, TileGeom& t) {
+  // Halo HB >= D: every output of the 64 x 64 tile sees all its non-zero
+  // factors.  In the log domain terms beyond the halo are dropped although
+  // their exact factor is not 0: with HB chosen so that (HB+1)^2 / gamma >= 264,
+  // they weigh below e^-264 times a term inside the checked spread (two
+  // neighbouring windows of at most kLogRange each), i.e. nothing in FP64.
+  int hb = round_up(D > 0 ? D : 1, 16);
+  if (logdom)
+    while ((double)(hb + 1) * (hb + 1) / gamma < 264.0) hb += 16;
+  if (hb > kHaloMax) {
+    set_error("tiled convolution: gamma " + std::to_string(gamma) + " needs a halo of " +
+              std::to_string(hb) + " pixels, above " + std::to_string(kHaloMax));
+    return WDL_ERR_UNSUPPORTED;
+  }
+  t.W = W;
+  t.H = H;
+  t.HB = hb;
+  t.R = kTile + 2 * hb;
+  t.S = round_up(t.R, 64) + 4;  // congruent to 4 mod 64 floats, as make_geom's strides
+  t.D = D;
+  t.nx = (W + kTile - 1) / kTile;
+  t.ny = (H + kTile - 1) / kTile;
+  return WDL_OK;
+}
+
 }  // namespace wdl
 
 extern "C" {

@@ -1,5 +1,6 @@
 // Standard-domain separable Gaussian convolution: the drop-in for
 // Gaussian2DKernel::convolve (kernels.h:669-699) and its FP32 device form.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -56,9 +57,75 @@ __global__ __launch_bounds__(kThreads, 4) void k_conv4(const float* __restrict__
   }
 }
 
+// Images with a side above kMaxSide: one workgroup per 64 x 64 output tile
+// (grid-stride over images x tiles), the haloed region staged in LDS.
+template <class T, int VW>
+__global__ __launch_bounds__(kThreads, 4) void k_conv_tile(const T* __restrict__ in,
+                                                           T* __restrict__ out, int nvec,
+                                                           TileGeom TG,
+                                                           const float* __restrict__ gtab) {
+  typedef typename Vec<VW>::T V;
+  extern __shared__ __align__(16) float smem[];
+  float* tabs = smem;
+  float* buf = smem + 4 * kTabLen;
+  load_tables(tabs, gtab);
+  const size_t N = (size_t)TG.W * TG.H;
+  const int nt = TG.nx * TG.ny;
+  for (long long blk = blockIdx.x; blk < (long long)nvec * nt; blk += gridDim.x) {
+    const int v = (int)(blk / nt), tile = (int)(blk - (long long)v * nt);
+    const int tx = tile % TG.nx, ty = tile / TG.nx;
+    const T* src = in + (size_t)v * N;
+    tile_region_pass<VW>(
+        TG, tx, ty,
+        [&](int gi) {
+          if constexpr (VW == 4) return vld<4>((const float*)src + gi);
+          else return (float)src[gi];
+        },
+        [&](int li, int, bool ok, V x) { vst<VW>(buf + li, ok ? x : vsplat<VW>(0.f)); });
+    tile_conv(buf, tabs, TG);
+    T* dst = out + (size_t)v * N;
+    tile_inner_pass<VW>(
+        TG, tx, ty, [&](int) { return 0; },
+        [&](int li, int gi, bool ok, int) {
+          if (!ok) return;
+          if constexpr (VW == 4) vst<4>((float*)dst + gi, vld<4>(buf + li));
+          else dst[gi] = (T)buf[li];
+        });
+    __syncthreads();
+  }
+}
+
+template <class T>
+static int launch_conv_tiled(const wdl_gauss2d* k, const T* in, T* out, int nvec, hipStream_t s) {
+  static bool attr_set = false;
+  const int lds = tile_lds_bytes(k->tg);
+  if (!attr_set) {
+    TileGeom big = k->tg;
+    big.R = kTile + 2 * kHaloMax;
+    big.S = kMaxSide + 4;
+    const int bytes = tile_lds_bytes(big);
+    WDL_HIP(hipFuncSetAttribute((const void*)k_conv_tile<T, 1>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    WDL_HIP(hipFuncSetAttribute((const void*)k_conv_tile<float, 4>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    attr_set = true;
+  }
+  const long long work = (long long)nvec * k->tg.nx * k->tg.ny;
+  const int grid = (int)std::min<long long>(work, 1 << 20);
+  if (sizeof(T) == sizeof(float) && k->W % 4 == 0)
+    hipLaunchKernelGGL((k_conv_tile<float, 4>), dim3(grid), dim3(kThreads), lds, s,
+                       (const float*)in, (float*)out, nvec, k->tg, k->tab.as<float>());
+  else
+    hipLaunchKernelGGL((k_conv_tile<T, 1>), dim3(grid), dim3(kThreads), lds, s, in, out, nvec,
+                       k->tg, k->tab.as<float>());
+  WDL_HIP(hipGetLastError());
+  return WDL_OK;
+}
+
 template <class T>
 static int launch_conv(const wdl_gauss2d* k, const T* in, T* out, int nvec, hipStream_t s) {
   if (nvec <= 0) return WDL_OK;
+  if (k->tiled) return launch_conv_tiled<T>(k, in, out, nvec, s);
   const ConvGeom G = k->geom(!(k->flags & WDL_FLAG_DENSE));
   const int lds = conv_lds_bytes(G);
   static bool attr_set = false;
@@ -103,9 +170,17 @@ int wdl_gauss2d_create(wdl_ctx* ctx, int W, int H, double gamma, int flags, wdl_
   WDL_CHECK(W > 0 && H > 0, WDL_ERR_ARG, "wdl_gauss2d_create: W and H must be positive");
   WDL_CHECK(gamma > 0 && std::isfinite(gamma), WDL_ERR_ARG,
             "wdl_gauss2d_create: gamma must be positive and finite");
-  WDL_CHECK(W <= wdl::kMaxSide && H <= wdl::kMaxSide, WDL_ERR_UNSUPPORTED,
-            "wdl_gauss2d_create: sides above " + std::to_string(wdl::kMaxSide) +
-                " are not supported by the LDS-resident convolution");
+  const bool tiled = wdl::needs_tiles(W, H);
+  WDL_CHECK(!(tiled && (flags & WDL_FLAG_DENSE)), WDL_ERR_UNSUPPORTED,
+            "wdl_gauss2d_create: WDL_FLAG_DENSE needs sides of at most " +
+                std::to_string(wdl::kMaxSide));
+  std::vector<float> tab;
+  const int D = wdl::build_table(W, H, gamma, tab);
+  wdl::TileGeom tg{};
+  if (tiled) {
+    const int st = wdl::make_tile_geom(W, H, D, gamma, false, tg);
+    if (st) return st;
+  }
   WDL_HIP(hipSetDevice(ctx->device));
   wdl_gauss2d* k = new wdl_gauss2d();
   k->ctx = ctx;
@@ -113,8 +188,9 @@ int wdl_gauss2d_create(wdl_ctx* ctx, int W, int H, double gamma, int flags, wdl_
   k->H = H;
   k->gamma = gamma;
   k->flags = flags;
-  std::vector<float> tab;
-  k->D = wdl::build_table(W, H, gamma, tab);
+  k->D = D;
+  k->tiled = tiled;
+  k->tg = tg;
   int st = k->tab.ensure(tab.size() * sizeof(float));
   if (st != WDL_OK) {
     delete k;

@@ -56,17 +56,20 @@ __device__ inline floatx4 mfma4(float a, float b, floatx4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-// One 1-D pass over an LDS matrix IN (nrows x ncols, row stride sin):
-// OUT = IN . Kg (ncols x ncols).  Output unit u of this wave covers output
-// columns [16*cb, 16*cb+16) and rows [64*rg, 64*rg+64); acc[uu][rb] holds the
-// 16x16 block of rows 64*rg+16*rb.. in the MFMA 16x16 layout (lane l, reg r:
-// row 4*(l/16)+r, col l%16).
-__device__ inline void rows_gemm(const float* __restrict__ in, int sin, int nrows, int ncols,
-                                 const float* __restrict__ tabs, int D,
+// One 1-D pass over an LDS matrix IN (nrows x ncontract, row stride sin):
+// OUT[r][j] = sum_k IN[r][k] * Kg(k, j + off) for j < nout, Kg(k, j) =
+// tab[|k - j|].  Output unit u of this wave covers output columns
+// [16*cb, 16*cb+16) and rows [64*rg, 64*rg+64); acc[uu][rb] holds the 16x16
+// block of rows 64*rg+16*rb.. in the MFMA 16x16 layout (lane l, reg r: row
+// 4*(l/16)+r, col l%16).  Contraction chunks whose factors are all beyond D
+// (exactly 0 in FP32) are skipped.
+__device__ inline void rows_gemm(const float* __restrict__ in, int sin, int nrows, int ncontract,
+                                 int nout, int off, const float* __restrict__ tabs, int D,
                                  floatx4 (&acc)[kMaxUnits][4]) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int c = lane & 15, g = lane >> 4;
-  const int ncb = ncols >> 4;
+  const int ncb = nout >> 4;
+  const int nck = ncontract >> 4;
   const int nunits = ncb * ((nrows + 63) >> 6);
 #pragma unroll
   for (int uu = 0; uu < kMaxUnits; ++uu) {
@@ -75,11 +78,11 @@ __device__ inline void rows_gemm(const float* __restrict__ in, int sin, int nrow
     const int u = wave + uu * kWaves;
     if (u < nunits) {
       const int cb = u % ncb, rg = u / ncb;
-      const int c0 = cb << 4, r0 = rg << 6;
+      const int c0 = (cb << 4) + off, r0 = rg << 6;  // c0: contraction index of output column 0
       const int nrb = min(4, (nrows - r0) >> 4);
       const int lo = c0 - D;
       const int kc_lo = lo > 0 ? (lo >> 4) : 0;
-      const int kc_hi = min(ncb - 1, (c0 + 15 + D) >> 4);
+      const int kc_hi = min(nck - 1, (c0 + 15 + D) >> 4);
       const float* arow = in + (r0 + c) * sin + 4 * g;
       for (int kc = kc_lo; kc <= kc_hi; ++kc) {
         const int kk = kc << 4;
@@ -102,8 +105,15 @@ __device__ inline void rows_gemm(const float* __restrict__ in, int sin, int nrow
   }
 }
 
-// Writes OUT (nrows x ncols, held in acc by rows_gemm) to dst transposed:
-// dst[j * sdst + r] = OUT[r][j].
+// The square pass of the image-resident convolution (nout = ncols, no offset).
+__device__ inline void rows_gemm(const float* __restrict__ in, int sin, int nrows, int ncols,
+                                 const float* __restrict__ tabs, int D,
+                                 floatx4 (&acc)[kMaxUnits][4]) {
+  rows_gemm(in, sin, nrows, ncols, ncols, 0, tabs, D, acc);
+}
+
+// Writes OUT (nrows x ncols output columns, held in acc by rows_gemm) to dst
+// transposed: dst[j * sdst + r] = OUT[r][j].
 __device__ inline void store_transposed(float* dst, int sdst, int nrows, int ncols,
                                         const floatx4 (&acc)[kMaxUnits][4]) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;

@@ -10,6 +10,7 @@
 
 #include "../../include/wdl_c.h"
 #include "wdl_conv_core.h"
+#include "wdl_tile.h"
 
 namespace wdl {
 
@@ -49,6 +50,14 @@ int build_table(int W, int H, double gamma, std::vector<float>& tab);
 
 ConvGeom make_geom(int W, int H, int D);
 
+// Geometry of the tiled convolution for band half-width D; the log domain
+// (logdom) widens the halo until the factors beyond it are below e^-264.
+// Returns WDL_ERR_UNSUPPORTED when the halo would exceed kHaloMax.
+int make_tile_geom(int W, int H, int D, double gamma, bool logdom, TileGeom& t);
+
+// Images with a side above kMaxSide take the tiled kernels.
+inline bool needs_tiles(int W, int H) { return W > kMaxSide || H > kMaxSide; }
+
 }  // namespace wdl
 
 struct wdl_ctx {
@@ -63,6 +72,8 @@ struct wdl_gauss2d {
   double gamma = 0;
   int D = 0;              // FP32 band half-width
   int flags = 0;
+  bool tiled = false;     // a side above kMaxSide: 64 x 64 tiles with a halo
+  wdl::TileGeom tg;
   wdl::DevBuf tab;        // kTabLen floats
   wdl::DevBuf io_in, io_out;   // staging for the host-pointer entry points
   wdl::DevBuf io_tmp;          // log domain: the transposed pass-1 result (FP64)

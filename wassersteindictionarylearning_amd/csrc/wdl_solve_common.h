@@ -1,0 +1,166 @@
+// Device-side pieces shared by the solver's translation units: the argument
+// block of every stage kernel, workgroup -> (histogram, atom) mappings, the
+// losses (loss.h) and a workgroup reduction.
+#pragma once
+
+#include "wdl_internal.h"
+#include "wdl_logconv.h"
+
+namespace wdl {
+
+constexpr int kLossBlocks = 32;
+
+struct SolveArgs {
+  ConvGeom G;
+  int K, P, L, p0, np, loss;
+  long long N;      // pixels per image
+  long long stage;  // P*K*N floats between consecutive history entries
+  long long qstage; // P*N floats between consecutive Qh entries
+  const float* tab;
+  const float* A;
+  const float* lam;
+  const float* obs;
+  float* Bh;
+  float* Qh;  // standard domain: q^s per histogram, L x P x N
+  float* Ch;
+  float* Fh;
+  float* V0;
+  float* V1;
+  float* R0;
+  float* R1;
+  float* GD;
+  double* GL;
+  float* Q;
+  float* G0;
+  double* lossPart;
+  const float* bInit;
+  float* bSave;
+  const int* Lp;      // per-histogram Sinkhorn iterations (<= L), or null for L
+  int weightsOnly;    // skip the dictionary-gradient chain
+  const float* logA;  // K x N log atoms (log-domain solver)
+  unsigned* events;   // lines whose spread exceeded kLogRange (log-domain solver)
+  // Tiled solver (a side above kMaxSide): geometry, the inputs of the second
+  // backward convolution (FP32; log domain: FP64 log magnitudes and sign
+  // bytes), and per-tile partial weight-gradient dots [P][K][tiles].
+  TileGeom T;
+  float* X0;
+  float* X1;
+  unsigned char* XS0;
+  unsigned char* XS1;
+  double* GLt;
+  double* gsumPart;  // [P][kLossBlocks] partial sums of g_L = loss'(q) q (k_final)
+  double* Mt;        // tiled: [L][P][K][tiles] partial sums of log phi (forward)
+};
+
+// Workgroup -> (histogram, atom).  With 8+ histograms the K workgroups of a
+// histogram get block ids congruent mod 8, i.e. the same XCD.
+__device__ inline bool map_block(const SolveArgs& a, int& p, int& i) {
+  const int b = blockIdx.x;
+  int pl;
+  if (a.np >= 8) {
+    const int xcd = b & 7, slot = b >> 3;
+    pl = (slot / a.K) * 8 + xcd;
+    i = slot % a.K;
+  } else {
+    pl = b / a.K;
+    i = b % a.K;
+  }
+  if (pl >= a.np) return false;
+  p = a.p0 + pl;
+  return true;
+}
+
+__device__ inline int steps_of(const SolveArgs& a, int p) { return a.Lp ? a.Lp[p] : a.L; }
+
+// Launch t of the backward (t = L..1) is step t - (L - L_p) of histogram p;
+// histograms with fewer iterations join later.  Returns 0 when p has no step.
+__device__ inline int bwd_step(const SolveArgs& a, int p, int t) {
+  return t - (a.L - steps_of(a, p));
+}
+
+static int stage_grid(int np, int K) { return np >= 8 ? 8 * K * ((np + 7) / 8) : np * K; }
+
+// Backward: two workgroups (dictionary chain, weight chain) per (histogram,
+// atom), on block ids b and b + 8 (np >= 8) -- the same XCD, dispatched
+// together -- so the arrays both chains read come from one L2.
+__device__ inline bool map_block_bwd(const SolveArgs& a, int& p, int& i, int& chain) {
+  const int b = blockIdx.x;
+  int pl;
+  if (a.weightsOnly) {  // one workgroup per (histogram, atom), the weight chain
+    chain = 1;
+    return map_block(a, p, i);
+  }
+  if (a.np >= 8) {
+    const int xcd = b & 7, slot = b >> 3;
+    chain = slot & 1;
+    const int s2 = slot >> 1;
+    pl = (s2 / a.K) * 8 + xcd;
+    i = s2 % a.K;
+  } else {
+    chain = b & 1;
+    pl = (b >> 1) / a.K;
+    i = (b >> 1) % a.K;
+  }
+  if (pl >= a.np) return false;
+  p = a.p0 + pl;
+  return true;
+}
+
+__device__ inline float loss_grad(int loss, float q, float o) {
+  if (loss == WDL_LOSS_KL) return safe_logf(q / o);          // loss.h:83-89
+  if (loss == WDL_LOSS_TV) return (q - o > 0.f) ? 0.5f : -0.5f;  // loss.h:58-64
+  return 2.f * (q - o);                                       // loss.h:34-40
+}
+
+__device__ inline double loss_term(int loss, float q, float o) {
+  const double dq = q, dob = o;
+  if (loss == WDL_LOSS_KL) {                                  // loss.h:74-81
+    const double r = dq / dob;
+    return dq * std::log(r > 1e-250 ? r : 1e-250) - dq + dob;
+  }
+  if (loss == WDL_LOSS_TV) return 0.5 * std::fabs(dq - dob);  // loss.h:49-56
+  return (dq - dob) * (dq - dob);                             // loss.h:25-32
+}
+
+// The weight-gradient dot of backward step s (:649, :786), <log phi_i^s, g_s>,
+// is evaluated as <log phi_i^s - m, g_s> + m E_s with m the image mean of
+// log phi_i^s and E_s = sum_x g_s taken exact: summing the recursion of
+// :655-672 over the pixels gives sum_x r_i^{s-1} = sum_x r_i^s - lam_i sum_x g_s,
+// so with sum_i lam_i = 1, E_{L-1} = -sum_x g_L and E_s = 0 for s <= L-2.  Same
+// value in exact arithmetic; in FP32 the rounding of g no longer enters
+// multiplied by the mean of log phi (a large offset common to all atoms),
+// which otherwise dominates the error of dL/dlambda.  Returns E_s for s < L_p.
+__device__ inline double exact_gsum(const SolveArgs& a, int p, int s) {
+  if (s <= steps_of(a, p) - 2) return 0.0;
+  double t = 0;
+  for (int b = 0; b < kLossBlocks; ++b) t += a.gsumPart[(long long)p * kLossBlocks + b];
+  return -t;
+}
+
+__device__ inline double vsum_d(float x) { return (double)x; }
+__device__ inline double vsum_d(floatx4 x) { return (double)x.x + x.y + x.z + x.w; }
+__device__ inline double vsum_d(double x) { return x; }
+__device__ inline double vsum_d(doublex4 x) { return x.x + x.y + x.z + x.w; }
+
+// Tiled solver launchers (one Sinkhorn stage = several launches), in
+// wdl_solver_tiled.hip.
+int tiled_lds_bytes(const SolveArgs& a);
+int tiled_set_lds_attr();
+int tiled_forward(const SolveArgs& a, hipStream_t st, int s);
+int tiled_backward(const SolveArgs& a, hipStream_t st, int t);
+int tiled_reduce_gl(const SolveArgs& a, hipStream_t st);
+
+// Sum over the workgroup, result valid in thread 0.
+__device__ inline double block_sum(double v) {
+  __shared__ double red[kWaves];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < kWaves; ++w) t += red[w];
+  __syncthreads();
+  return t;
+}
+
+}  // namespace wdl

@@ -23,126 +23,9 @@
 #include <cstring>
 #include <limits>
 
-#include "wdl_internal.h"
-#include "wdl_logconv.h"
+#include "wdl_solve_common.h"
 
 namespace wdl {
-
-constexpr int kLossBlocks = 32;
-
-struct SolveArgs {
-  ConvGeom G;
-  int K, P, L, p0, np, loss;
-  long long N;      // pixels per image
-  long long stage;  // P*K*N floats between consecutive history entries
-  long long qstage; // P*N floats between consecutive Qh entries
-  const float* tab;
-  const float* A;
-  const float* lam;
-  const float* obs;
-  float* Bh;
-  float* Qh;  // standard domain: q^s per histogram, L x P x N
-  float* Ch;
-  float* Fh;
-  float* V0;
-  float* V1;
-  float* R0;
-  float* R1;
-  float* GD;
-  double* GL;
-  float* Q;
-  float* G0;
-  double* lossPart;
-  const float* bInit;
-  float* bSave;
-  const int* Lp;      // per-histogram Sinkhorn iterations (<= L), or null for L
-  int weightsOnly;    // skip the dictionary-gradient chain
-  const float* logA;  // K x N log atoms (log-domain solver)
-  unsigned* events;   // lines whose spread exceeded kLogRange (log-domain solver)
-};
-
-// Workgroup -> (histogram, atom).  With 8+ histograms the K workgroups of a
-// histogram get block ids congruent mod 8, i.e. the same XCD.
-__device__ inline bool map_block(const SolveArgs& a, int& p, int& i) {
-  const int b = blockIdx.x;
-  int pl;
-  if (a.np >= 8) {
-    const int xcd = b & 7, slot = b >> 3;
-    pl = (slot / a.K) * 8 + xcd;
-    i = slot % a.K;
-  } else {
-    pl = b / a.K;
-    i = b % a.K;
-  }
-  if (pl >= a.np) return false;
-  p = a.p0 + pl;
-  return true;
-}
-
-__device__ inline int steps_of(const SolveArgs& a, int p) { return a.Lp ? a.Lp[p] : a.L; }
-
-// Launch t of the backward (t = L..1) is step t - (L - L_p) of histogram p;
-// histograms with fewer iterations join later.  Returns 0 when p has no step.
-__device__ inline int bwd_step(const SolveArgs& a, int p, int t) {
-  return t - (a.L - steps_of(a, p));
-}
-
-static int stage_grid(int np, int K) { return np >= 8 ? 8 * K * ((np + 7) / 8) : np * K; }
-
-// Backward: two workgroups (dictionary chain, weight chain) per (histogram,
-// atom), on block ids b and b + 8 (np >= 8) -- the same XCD, dispatched
-// together -- so the arrays both chains read come from one L2.
-__device__ inline bool map_block_bwd(const SolveArgs& a, int& p, int& i, int& chain) {
-  const int b = blockIdx.x;
-  int pl;
-  if (a.weightsOnly) {  // one workgroup per (histogram, atom), the weight chain
-    chain = 1;
-    return map_block(a, p, i);
-  }
-  if (a.np >= 8) {
-    const int xcd = b & 7, slot = b >> 3;
-    chain = slot & 1;
-    const int s2 = slot >> 1;
-    pl = (s2 / a.K) * 8 + xcd;
-    i = s2 % a.K;
-  } else {
-    chain = b & 1;
-    pl = (b >> 1) / a.K;
-    i = (b >> 1) % a.K;
-  }
-  if (pl >= a.np) return false;
-  p = a.p0 + pl;
-  return true;
-}
-
-__device__ inline float loss_grad(int loss, float q, float o) {
-  if (loss == WDL_LOSS_KL) return safe_logf(q / o);          // loss.h:83-89
-  if (loss == WDL_LOSS_TV) return (q - o > 0.f) ? 0.5f : -0.5f;  // loss.h:58-64
-  return 2.f * (q - o);                                       // loss.h:34-40
-}
-
-__device__ inline double loss_term(int loss, float q, float o) {
-  const double dq = q, dob = o;
-  if (loss == WDL_LOSS_KL) {                                  // loss.h:74-81
-    const double r = dq / dob;
-    return dq * std::log(r > 1e-250 ? r : 1e-250) - dq + dob;
-  }
-  if (loss == WDL_LOSS_TV) return 0.5 * std::fabs(dq - dob);  // loss.h:49-56
-  return (dq - dob) * (dq - dob);                             // loss.h:25-32
-}
-
-// Sum over the workgroup, result valid in thread 0.
-__device__ inline double block_sum(double v) {
-  __shared__ double red[kWaves];
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-  __syncthreads();
-  double t = 0;
-  if (threadIdx.x == 0)
-    for (int w = 0; w < kWaves; ++w) t += red[w];
-  __syncthreads();
-  return t;
-}
 
 // Forward iteration s (1..L): b^{s-1} (geometric-mean step of iteration s-1,
 // :577-592), c^{s-1} = K b^{s-1} (:568), a = p / c (:569-574), phi^s = K a (:575).
@@ -230,7 +113,7 @@ __global__ __launch_bounds__(kThreads) void k_final(SolveArgs a) {
   const float* F = a.Fh + (long long)(Lp - 1) * a.stage + hbase;
   float* QL = a.Qh + (long long)(Lp - 1) * a.qstage + (long long)p * N;
   const float* lam = a.lam + (long long)p * K;
-  double acc = 0;
+  double acc = 0, gs = 0;
   for (long long x = (long long)blockIdx.x * kThreads + threadIdx.x; x < N;
        x += (long long)gridDim.x * kThreads) {
     float lq = 0.f;
@@ -240,12 +123,17 @@ __global__ __launch_bounds__(kThreads) void k_final(SolveArgs a) {
     if (a.bSave)
       for (int j = 0; j < K; ++j) a.bSave[hbase + j * N + x] = q / F[j * N + x];
     const float o = a.obs[(long long)p * N + x];
+    const float g0 = loss_grad(a.loss, q, o);
     a.Q[(long long)p * N + x] = q;
-    a.G0[(long long)p * N + x] = loss_grad(a.loss, q, o);
+    a.G0[(long long)p * N + x] = g0;
     acc += loss_term(a.loss, q, o);
+    gs += (double)(g0 * q);  // g_L as the weight chain forms it
   }
-  const double t = block_sum(acc);
-  if (threadIdx.x == 0) a.lossPart[(long long)p * kLossBlocks + blockIdx.x] = t;
+  const double t = block_sum(acc), tg = block_sum(gs);
+  if (threadIdx.x == 0) {
+    a.lossPart[(long long)p * kLossBlocks + blockIdx.x] = t;
+    a.gsumPart[(long long)p * kLossBlocks + blockIdx.x] = tg;
+  }
 }
 
 // Backward step s (L..1).  chain 0: dictionary gradient (:600-629);
@@ -334,7 +222,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_backward(SolveArgs a, int t) {
     const float* Fs = a.Fh + (long long)(s - 1) * a.stage + img;  // phi^s
     const float* G0 = a.G0 + (long long)p * N;
     const float* Qp = a.Q + (long long)p * N;
-    double dot = 0;
+    double dot = 0, sf = 0, sg = 0;
     px_pass<VW>(
         G,
         [&](int gi) {
@@ -357,15 +245,18 @@ __global__ __launch_bounds__(kThreads, 4) void k_backward(SolveArgs a, int t) {
         [&](int li, int gi, bool ok, const Three& r) {
           V u = vsplat<VW>(0.f);
           if (ok) {
-            dot += vdot_d(vlogf(r.z), r.x);  // plain log (:649)
+            const V lf = vlogf(r.z);  // plain log (:649)
+            dot += vdot_d(lf, r.x);
+            sf += vsum_d(lf);
+            sg += vsum_d(r.x);
             u = (lpi * r.x - r.y) / r.z;
           }
           vst<VW>(buf + li, u);
         });
-    const double t = block_sum(dot);
+    const double t = block_sum(dot), tf = block_sum(sf), tg = block_sum(sg);
     if (threadIdx.x == 0) {
       double* gl = a.GL + (long long)p * K + i;
-      *gl = last ? t : *gl + t;
+      *gl = last ? t : *gl + (t - tf / (double)N * (tg - exact_gsum(a, p, s)));
     }
     if (s == 1) return;
     conv_lds(buf, tabs, G);
@@ -521,7 +412,7 @@ __global__ __launch_bounds__(kThreads) void k_final_log(SolveArgs a) {
   const double* F = (const double*)a.Fh + (long long)(Lp - 1) * a.stage + hbase;
   double* B = (double*)a.Bh + (long long)(Lp - 1) * a.stage + hbase;
   const float* lam = a.lam + (long long)p * K;
-  double acc = 0;
+  double acc = 0, gs = 0;
   for (long long x = (long long)blockIdx.x * kThreads + threadIdx.x; x < N;
        x += (long long)gridDim.x * kThreads) {
     double lq = 0.0;
@@ -533,12 +424,17 @@ __global__ __launch_bounds__(kThreads) void k_final_log(SolveArgs a) {
     }
     const float q = (float)exp(lq);
     const float o = a.obs[(long long)p * N + x];
+    const float g0 = loss_grad(a.loss, q, o);
     a.Q[(long long)p * N + x] = q;
-    a.G0[(long long)p * N + x] = loss_grad(a.loss, q, o);
+    a.G0[(long long)p * N + x] = g0;
     acc += loss_term(a.loss, q, o);
+    gs += (double)(g0 * q);
   }
-  const double t = block_sum(acc);
-  if (threadIdx.x == 0) a.lossPart[(long long)p * kLossBlocks + blockIdx.x] = t;
+  const double t = block_sum(acc), tg = block_sum(gs);
+  if (threadIdx.x == 0) {
+    a.lossPart[(long long)p * kLossBlocks + blockIdx.x] = t;
+    a.gsumPart[(long long)p * kLossBlocks + blockIdx.x] = tg;
+  }
 }
 
 template <int VW>
@@ -635,7 +531,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_backward_log(SolveArgs a, int t
     const double* Fs = (const double*)a.Fh + (long long)(s - 1) * a.stage + img;  // log phi^s
     const float* G0 = a.G0 + (long long)p * N;
     const float* Qp = a.Q + (long long)p * N;
-    double dot = 0;
+    double dot = 0, sf = 0, sg = 0;
     auto wload = [&](int gi) {
       Three r;  // g, r_i, log phi^s
       r.z = vldd<VW>(Fs + gi);
@@ -653,22 +549,27 @@ __global__ __launch_bounds__(kThreads, 4) void k_backward_log(SolveArgs a, int t
       }
       return r;
     };
+    auto acc_dot = [&](const Three& r) {
+      dot += vdot_dd(r.z, r.x);  // sum phi . g (:786)
+      sf += vsum_d(r.z);
+      sg += vsum_d(r.x);
+    };
     auto wmake = [&](int, int, int, const Three& r, bool second) {
-      if (second) dot += vdot_dd(r.z, r.x);  // sum phi . g (:786)
+      if (second) acc_dot(r);
       const V x = lpi * r.x + r.y;
       return SLog<VW>{vhlog(vabs(x)) - r.z, vsign(x)};
     };
     if (s == 1) {
       px_pass_r<VW>(G, wload, [&](int, int, int, bool ok, const Three& r) {
-        if (ok) dot += vdot_dd(r.z, r.x);
+        if (ok) acc_dot(r);
       });
     } else {
       log_conv_lds<VW>(sm.buf, sm.tabs, sm.L, G, a.events, wload, wmake);
     }
-    const double t = block_sum(dot);
+    const double t = block_sum(dot), tf = block_sum(sf), tg = block_sum(sg);
     if (threadIdx.x == 0) {
       double* gl = a.GL + (long long)p * K + i;
-      *gl = last ? t : *gl + t;
+      *gl = last ? t : *gl + (t - tf / (double)N * (tg - exact_gsum(a, p, s)));
     }
     if (s == 1) return;
     log_conv_lds<VW>(sm.buf, sm.tabs, sm.L, G, a.events, mid_load, mid_make);
@@ -725,7 +626,9 @@ struct wdl_solver {
   long long N = 0;
   int D = 0;
   DevBuf tab, A, lam, obs, Bh, Qh, Ch, Fh, V0, V1, R0, R1, GD, GL, Q, G0, lossPart, bStore,
-      bTemp, GDsum, logA, events, Lp;
+      bTemp, GDsum, logA, events, Lp, X0, X1, XS0, XS1, GLt, gsumPart, Mt;
+  bool tiled = false;  // 64 x 64 tiles with a halo (a side above kMaxSide)
+  TileGeom tg{};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool observed = false, problem = false, ran = false;
@@ -770,10 +673,19 @@ struct wdl_solver {
     const bool logd = flags & WDL_FLAG_LOG_DOMAIN;
     a.logA = logd ? logA.as<float>() : nullptr;
     a.events = logd ? events.as<unsigned>() : nullptr;
+    a.T = tg;
+    a.X0 = X0.as<float>();
+    a.X1 = X1.as<float>();
+    a.XS0 = XS0.as<unsigned char>();
+    a.XS1 = XS1.as<unsigned char>();
+    a.GLt = GLt.as<double>();
+    a.gsumPart = gsumPart.as<double>();
+    a.Mt = Mt.as<double>();
     return a;
   }
 
   int lds_bytes(const SolveArgs& a) const {
+    if (tiled) return tiled_lds_bytes(a);
     return conv_lds_bytes(a.G) + ((flags & WDL_FLAG_LOG_DOMAIN) ? kLogKeysBytes : 0);
   }
 
@@ -853,16 +765,26 @@ struct wdl_solver {
     const int lds = lds_bytes(a);
     const dim3 blk(kThreads);
     const int gs = stage_grid(np, K);
-    for (int s = 1; s <= L; ++s)
-      launch_forward(a, gs, lds, st, s);
+    int e;
+    for (int s = 1; s <= L; ++s) {
+      if (!tiled)
+        launch_forward(a, gs, lds, st, s);
+      else if ((e = tiled_forward(a, st, s)))
+        return e;
+    }
     // kLossBlocks blocks per histogram: every lossPart slot is written
     // (blocks past the last pixel write 0).
     if (a.logA)
       hipLaunchKernelGGL(k_final_log, dim3(kLossBlocks, np), blk, 0, st, a);
     else
       hipLaunchKernelGGL(k_final, dim3(kLossBlocks, np), blk, 0, st, a);
-    for (int s = L; s >= 1; --s)
-      launch_backward(a, gs, lds, st, s);
+    for (int s = L; s >= 1; --s) {
+      if (!tiled)
+        launch_backward(a, gs, lds, st, s);
+      else if ((e = tiled_backward(a, st, s)))
+        return e;
+    }
+    if (tiled && (e = tiled_reduce_gl(a, st))) return e;
     WDL_HIP(hipGetLastError());
     return (reduce && !(flags & WDL_FLAG_WEIGHTS_ONLY)) ? reduce_gd(st) : WDL_OK;
   }
@@ -885,17 +807,26 @@ static int set_lds_attr() {
   return WDL_OK;
 }
 
+static bool use_tiles(int W, int H, int flags) {
+  return needs_tiles(W, H) || (flags & WDL_FLAG_TILED);
+}
+
 static long long footprint(int W, int H, int K, int P, int L, int flags) {
   const long long N = (long long)W * H, PKN = (long long)P * K * N;
+  const bool logd = flags & WDL_FLAG_LOG_DOMAIN;
   long long f = (flags & WDL_FLAG_LOG_DOMAIN) ? 3LL * L * PKN * 8           // Bh, Ch, Fh (FP64)
                                              : 2LL * L * PKN * 4 + (long long)L * P * N * 4;  // Ch, Fh, Qh
   f += 5 * PKN * 4;                          // V0, V1, R0, R1, GD
   f += (long long)K * N * (4 + 8);           // A, GDsum
   f += (long long)P * N * 4 * 3;             // obs, Q, G0
-  f += (long long)P * K * (4 + 8) + (long long)P * kLossBlocks * 8;
+  f += (long long)P * K * (4 + 8) + 2LL * P * kLossBlocks * 8;
   if (flags & WDL_FLAG_WARM_RESTART) f += 2 * PKN * 4;
   if (flags & WDL_FLAG_LOG_DOMAIN) f += (long long)K * N * 4;  // log atoms
   f += kTabLen * 4;
+  if (use_tiles(W, H, flags)) {  // X0, X1 (+ sign bytes), per-tile dots
+    const long long nt = (long long)((W + kTile - 1) / kTile) * ((H + kTile - 1) / kTile);
+    f += 2 * PKN * (logd ? 9 : 4) + (long long)P * K * nt * 8 * (1 + L);
+  }
   return f;
 }
 
@@ -913,14 +844,23 @@ int wdl_solver_create(wdl_ctx* ctx, int W, int H, int K, int P, int L, double ga
             "wdl_solver_create: W, H, K, P, L must be positive");
   WDL_CHECK(gamma > 0 && std::isfinite(gamma), WDL_ERR_ARG,
             "wdl_solver_create: gamma must be positive and finite");
-  WDL_CHECK(W <= kMaxSide && H <= kMaxSide, WDL_ERR_UNSUPPORTED,
-            "wdl_solver_create: sides above " + std::to_string(kMaxSide) + " not supported");
   WDL_CHECK(loss == WDL_LOSS_QUADRATIC || loss == WDL_LOSS_KL || loss == WDL_LOSS_TV,
             WDL_ERR_UNSUPPORTED, "wdl_solver_create: loss must be TV (1), quadratic (2) or KL (4)");
+  const bool tiled = use_tiles(W, H, flags);
+  WDL_CHECK(!(tiled && (flags & WDL_FLAG_DENSE)), WDL_ERR_UNSUPPORTED,
+            "wdl_solver_create: WDL_FLAG_DENSE needs sides of at most " + std::to_string(kMaxSide) +
+                " and no WDL_FLAG_TILED");
+  std::vector<float> tab;
+  const int D = build_table(W, H, gamma, tab);
+  TileGeom tg{};
+  int st;
+  if (tiled && (st = make_tile_geom(W, H, D, gamma, flags & WDL_FLAG_LOG_DOMAIN, tg))) return st;
   WDL_HIP(hipSetDevice(ctx->device));
-  int st = set_lds_attr();
-  if (st) return st;
+  if ((st = set_lds_attr())) return st;
+  if ((st = tiled_set_lds_attr())) return st;
   wdl_solver* s = new wdl_solver();
+  s->tiled = tiled;
+  s->tg = tg;
   s->ctx = ctx;
   s->W = W;
   s->H = H;
@@ -931,9 +871,9 @@ int wdl_solver_create(wdl_ctx* ctx, int W, int H, int K, int P, int L, double ga
   s->flags = flags;
   s->gamma = gamma;
   s->N = (long long)W * H;
-  std::vector<float> tab;
-  s->D = build_table(W, H, gamma, tab);
+  s->D = D;
   const long long N = s->N, PKN = (long long)P * K * N;
+  const long long ntiles = (long long)tg.nx * tg.ny;
   const size_t f4 = sizeof(float);
   const size_t hsz = (flags & WDL_FLAG_LOG_DOMAIN) ? sizeof(double) : sizeof(float);  // histories
   struct {
@@ -963,6 +903,13 @@ int wdl_solver_create(wdl_ctx* ctx, int W, int H, int K, int P, int L, double ga
       {&s->logA, (flags & WDL_FLAG_LOG_DOMAIN) ? (size_t)K * N * f4 : 0},
       {&s->events, sizeof(unsigned)},
       {&s->Lp, (size_t)P * sizeof(int)},
+      {&s->X0, tiled ? (size_t)PKN * hsz : 0},
+      {&s->X1, tiled ? (size_t)PKN * hsz : 0},
+      {&s->XS0, tiled && (flags & WDL_FLAG_LOG_DOMAIN) ? (size_t)PKN : 0},
+      {&s->XS1, tiled && (flags & WDL_FLAG_LOG_DOMAIN) ? (size_t)PKN : 0},
+      {&s->GLt, tiled ? (size_t)P * K * ntiles * sizeof(double) : 0},
+      {&s->Mt, tiled ? (size_t)L * P * K * ntiles * sizeof(double) : 0},
+      {&s->gsumPart, (size_t)P * kLossBlocks * sizeof(double)},
   };
   for (auto& al : allocs) {
     if ((st = al.b->ensure(al.n))) {
@@ -1014,7 +961,7 @@ int wdl_solver_destroy(wdl_solver* s) {
   DevBuf* bufs[] = {&s->tab, &s->A,  &s->lam, &s->obs, &s->Bh, &s->Qh, &s->Ch,  &s->Fh,
                     &s->V0,  &s->V1, &s->R0,  &s->R1,  &s->GD, &s->GL,       &s->Q,
                     &s->G0,  &s->lossPart,    &s->bStore,      &s->bTemp,    &s->GDsum,
-                    &s->logA, &s->events, &s->Lp};
+                    &s->logA, &s->events, &s->Lp, &s->X0, &s->X1, &s->XS0, &s->XS1, &s->GLt, &s->Mt, &s->gsumPart};
   for (DevBuf* b : bufs) b->release();
   delete s;
   return WDL_OK;
@@ -1178,6 +1125,9 @@ int wdl_solver_run_stage(wdl_solver* s, int stage, int step) {
   const SolveArgs a = s->args(0, s->P);
   const int lds = s->lds_bytes(a);  // all P solves, on the context stream
   const int gs = stage_grid(s->P, s->K);
+  if (s->tiled)
+    return stage == WDL_STAGE_FORWARD ? tiled_forward(a, s->ctx->stream, step)
+                                      : tiled_backward(a, s->ctx->stream, step);
   if (stage == WDL_STAGE_FORWARD)
     launch_forward(a, gs, lds, s->ctx->stream, step);
   else

@@ -61,10 +61,14 @@ CONFIGS = {
                         sigma=(4, 16), scale_factor=100.0),
     "log_small_eps": dict(n=256, P=32, K=6, gamma=0.5, L=200, loss=4, curves=True,
                           scale_factor=100.0),
+    "projection": dict(n=128, P=4096, K=8, gamma=1.0, L=100, Lmin=20, loss=2, nblobs=(1, 5),
+                       sigma=(3, 20), scale_factor=100.0),
 }
 
 
 def make_problem(name: str, seed: int | None = None, P: int | None = None):
+    """Inputs of one workload.  The projection workload's atoms stand for a
+    learned dictionary (random blobs, as the configuration states)."""
     c = dict(CONFIGS[name])
     if P is not None:
         c["P"] = P
@@ -76,4 +80,6 @@ def make_problem(name: str, seed: int | None = None, P: int | None = None):
         obs = blob_histograms(rng, n, Pn, c["nblobs"], c["sigma"])
     atoms, w = init_atoms_weights(rng, K, Pn, n * n)
     c.update(obs=obs, atoms=atoms, weights=w)
+    if "Lmin" in c:  # per-request iteration counts, uniform in Lmin..L
+        c["iterations"] = rng.integers(c["Lmin"], c["L"] + 1, Pn).astype(np.int32)
     return c
