@@ -107,10 +107,12 @@ int wdl_solver_warm_commit(wdl_solver* s);
  * on its own stream (default min(4, P/16), at least 1).  No counterpart in
  * the reference; a tuning knob of the batched launch. */
 int wdl_solver_set_streams(wdl_solver* s, int nstreams);
-/* Per-histogram Sinkhorn iterations (1..L each; NULL = L for all): histogram p
+/* Per-histogram Sinkhorn iterations (0..L each; NULL = L for all): histogram p
  * runs iterations[p] forward steps and as many backward steps (the reference
  * has one Niters for all, inverseWasserstein.h:546; uneven counts serve
- * requests that arrive with their own budgets). */
+ * requests that arrive with their own budgets).  0 leaves histogram p idle:
+ * no launch touches it, its loss reads 0 and its other outputs keep their
+ * values. */
 int wdl_solver_set_iterations(wdl_solver* s, const int* iterations);
 /* Enqueue the forward L iterations and both unrolled backwards for all P
  * solves (compute_gradient_both, inverseWasserstein.h:544-674, or its log
@@ -136,6 +138,61 @@ int wdl_solver_get_results(wdl_solver* s, double* loss, double* grad_lambda,
 int wdl_solver_log_range_events(wdl_solver* s, long long* count);
 /* resultDic of every solve (P x K x N). */
 int wdl_solver_get_grad_dic(wdl_solver* s, double* grad_dic);
+/* ---- continuous batching: per-histogram slots ------------------------------
+ * No counterpart in the reference (its P solves are one serial loop,
+ * inverseWasserstein.h:1052); these let a caller change one histogram of a
+ * batched solver between runs, as a serving loop admits and retires
+ * requests. */
+/* Replaces histogram `slot`'s observed histogram (N values, NULL = keep) and
+ * weights (K values, normalised here; NULL = keep), sets its Sinkhorn
+ * iterations (0..L; 0 = idle: no launch touches it, its outputs keep their
+ * values) and its mirror-descent step (wdl_solver_projection_round), and resets
+ * its warm-restart store to the reference's start value 1 (:380-384).
+ * Synchronous: waits for work on the context stream first. */
+int wdl_solver_set_slot(wdl_solver* s, int slot, const double* observed, const double* lambdas,
+                        int iterations, double step);
+/* One projection round for every non-idle histogram (flags WDL_FLAG_WEIGHTS_ONLY |
+ * WDL_FLAG_WARM_RESTART): the batched weights-only solve (:635-673) seeded from the
+ * warm-restart store, then lambda_i <- lambda_i exp(-step g_i / max|g|) / Z with
+ * g = resultLa - <lambda, resultLa> (the projection of :1095-1099), then the
+ * per-histogram warm commit b_storage <- b_temp (:418-422).  Asynchronous. */
+int wdl_solver_projection_round(wdl_solver* s);
+/* Histogram `slot`'s current weights (K) and the loss of its last run.  Synchronous. */
+int wdl_solver_get_slot(wdl_solver* s, int slot, double* lambdas, double* loss);
+
+/* ---- served projection loop ----------------------------------------------
+ * No counterpart in the reference.  A scheduler thread serves requests that
+ * project histograms onto registered dictionaries (weights fitting by
+ * projection rounds), batching the requests of one dictionary into pools of
+ * slots that they join and leave between rounds (continuous batching),
+ * running pools of different dictionaries concurrently on their own streams,
+ * and admitting pools by device-memory footprint (wdl_solver_footprint)
+ * within hbm_budget_bytes. */
+typedef struct wdl_server wdl_server;
+int wdl_server_create(int device, long long hbm_budget_bytes, int max_slots_per_pool,
+                      wdl_server** out);
+int wdl_server_destroy(wdl_server* sv);
+/* A dictionary: K atoms (K x W x H, normalised per atom by the solver), gamma,
+ * loss, the largest iteration count a request may ask for, flags 0 or
+ * WDL_FLAG_LOG_DOMAIN.  Returns its id. */
+int wdl_server_add_dictionary(wdl_server* sv, int W, int H, int K, double gamma, int loss,
+                              int max_iterations, int flags, const double* atoms, int* dict_id);
+/* Queue a request (the histogram is copied): `rounds` projection rounds of
+ * `iterations` Sinkhorn iterations each, mirror-descent step `step`, weights
+ * starting uniform.  Returns immediately with the request id. */
+int wdl_server_submit(wdl_server* sv, int dict_id, const double* histogram, int iterations,
+                      int rounds, double step, long long* request_id);
+/* Wait up to timeout_s for a request; on completion writes its weights (K), the
+ * loss of its last round, its latency (submit to completion) and its queueing
+ * time (submit to admission), in ms, and forgets it.  Returns 1 on timeout. */
+int wdl_server_wait(wdl_server* sv, long long request_id, double timeout_s, double* weights,
+                    double* loss, double* latency_ms, double* queue_ms);
+/* out6: completed requests, pools created, pools released, requests that waited
+ * for device memory, device bytes held by pools, rounds launched. */
+int wdl_server_stats(wdl_server* sv, long long* out6);
+/* 0 when all work enqueued on the context has completed, 1 while some is pending. */
+int wdl_ctx_query(wdl_ctx* ctx);
+
 /* compute_gradient_both for every histogram: set_problem + run + get_results. */
 int wdl_gradient_both(wdl_solver* s, const double* atoms, const double* lambdas, double* loss,
                       double* grad_lambda, double* grad_dic_sum, double* barycenters);

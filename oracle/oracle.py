@@ -27,6 +27,11 @@ How each function is pinned (tests/test_oracle.py):
   state), and finite differences of f(x).
 * ``loss_*``: textbook formulas checked against the compiled reference
   through gradient_both with each loss.
+* ``project_weights`` (the serving loop's projection; no counterpart in the
+  reference): built from the pinned ``gradient_both`` / ``gradient_both_log``;
+  its update is pinned by step 0 (weights unchanged), K = 1 (weight 1), the
+  descent property (the loss falls over rounds; a sign error raises it) and
+  the simplex (weights stay positive and sum to 1).
 """
 from __future__ import annotations
 
@@ -362,3 +367,35 @@ def encode_variables(weights, atoms, scale_dic, exp_weight=True):
     w = np.asarray(weights, np.float64).ravel()
     xw = np.log(w) * scale_dic if exp_weight else w * scale_dic
     return np.concatenate([xw, np.log(np.asarray(atoms, np.float64).ravel())])
+
+
+# ----------------------------------------------------------------------------
+# The serving loop's projection (no counterpart in the reference)
+# ----------------------------------------------------------------------------
+def project_weights(atoms, obs, W, H, gamma, L, rounds, step, loss=LOSS_QUADRATIC,
+                    log_domain=False):
+    """Fit the weights of one histogram against a fixed dictionary, as the
+    projection rounds of the served loop do: `rounds` weight-gradient
+    evaluations (compute_gradient_both, inverseWasserstein.h:544-674, or its log
+    form), each warm-started from the previous b^L (:1057-1061, :1081-1085; the
+    store starts at 1, :380-384), each followed by the exponentiated-gradient step
+        lambda_i <- lambda_i exp(-step g_i / max_j |g_j|) / Z,
+        g = resultLa - <lambda, resultLa>            (the projection of :1095-1099).
+    Returns (weights after the last step, loss of the last evaluation)."""
+    atoms = np.asarray(atoms, np.float64)
+    K, N = atoms.shape
+    lam = np.full(K, 1.0 / K)
+    b = np.ones((K, N))
+    solve = gradient_both_log if log_domain else gradient_both
+    f = None
+    for _ in range(rounds):
+        r = solve(atoms, lam, obs, W, H, gamma, L, loss, b0=b)
+        b = r["b_last"]
+        f = loss_value(loss, r["bary"], obs)
+        G = r["grad_lambda"]
+        g = G - lam @ G
+        m = np.abs(g).max()
+        if m > 0 and np.isfinite(m):
+            lam = lam * np.exp(-step * g / m)
+            lam = lam / lam.sum()
+    return lam, f

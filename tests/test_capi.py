@@ -52,3 +52,14 @@ def test_null_arguments_are_errors():
     assert b"NULL" in L.wdl_last_error()
     assert L.wdl_solver_run(None, 0) == -1
     assert L.wdl_ctx_create(0, None) == -1
+    # serving entry points check their arguments before touching the device
+    assert L.wdl_solver_set_slot(None, 0, None, None, 0, 0.0) == -1
+    assert L.wdl_solver_projection_round(None) == -1
+    assert L.wdl_solver_get_slot(None, 0, None, None) == -1
+    assert L.wdl_ctx_query(None) == -1
+    assert L.wdl_server_create(0, 1 << 30, 4, None) == -1
+    assert L.wdl_server_create(0, 0, 4, None) == -1
+    assert L.wdl_server_submit(None, 0, None, 1, 1, 0.5, None) == -1
+    assert L.wdl_server_wait(None, 1, 0.0, None, None, None, None) == -1
+    assert L.wdl_server_stats(None, None) == -1
+    assert L.wdl_server_destroy(None) == 0

@@ -67,6 +67,18 @@ _SIGS = {
     "wdl_gradient_both": (_i, [_vp, _d, _d, _d, _d, _d, _d]),
     "wdl_solver_set_scale": (_i, [_vp, ctypes.c_double, _i]),
     "wdl_evaluate_both": (ctypes.c_double, [_vp, _d, _d, _i, ctypes.c_double]),
+    "wdl_solver_set_slot": (_i, [_vp, _i, _d, _d, _i, ctypes.c_double]),
+    "wdl_solver_projection_round": (_i, [_vp]),
+    "wdl_solver_get_slot": (_i, [_vp, _i, _d, _d]),
+    "wdl_ctx_query": (_i, [_vp]),
+    "wdl_server_create": (_i, [_i, ctypes.c_longlong, _i, ctypes.POINTER(_vp)]),
+    "wdl_server_destroy": (_i, [_vp]),
+    "wdl_server_add_dictionary": (_i, [_vp, _i, _i, _i, ctypes.c_double, _i, _i, _i, _d,
+                                       ctypes.POINTER(_i)]),
+    "wdl_server_submit": (_i, [_vp, _i, _d, _i, _i, ctypes.c_double,
+                               ctypes.POINTER(ctypes.c_longlong)]),
+    "wdl_server_wait": (_i, [_vp, ctypes.c_longlong, ctypes.c_double, _d, _d, _d, _d]),
+    "wdl_server_stats": (_i, [_vp, ctypes.POINTER(ctypes.c_longlong)]),
 }
 
 
@@ -302,9 +314,89 @@ class SinkhornSolver:
                 raise WdlError("wdl_evaluate_both: " + err)
         return f, g
 
+    def set_slot(self, slot, observed=None, lambdas=None, iterations=0, step=0.0):
+        o = None if observed is None else _f64(observed)
+        lm = None if lambdas is None else _f64(lambdas)
+        _check(lib().wdl_solver_set_slot(self.handle, slot, _dp(o), _dp(lm), int(iterations),
+                                         float(step)), "wdl_solver_set_slot")
+
+    def projection_round(self):
+        _check(lib().wdl_solver_projection_round(self.handle), "wdl_solver_projection_round")
+
+    def get_slot(self, slot):
+        w = np.zeros(self.K)
+        loss = ctypes.c_double()
+        _check(lib().wdl_solver_get_slot(self.handle, slot, _dp(w), ctypes.byref(loss)),
+               "wdl_solver_get_slot")
+        return w, loss.value
+
     def close(self):
         if self.handle:
             lib().wdl_solver_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        if not sys.is_finalizing():
+            try:
+                self.close()
+            except Exception:
+                pass
+
+
+class ProjectionServer:
+    """Served projection loop (wdl_server): requests fit the weights of one
+    histogram against a registered dictionary; a native scheduler thread
+    batches them continuously on the device."""
+
+    STATS = ("completed", "pools_created", "pools_released", "memory_waits", "bytes_in_use",
+             "rounds")
+
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 64 << 30,
+                 max_slots_per_pool: int = 16):
+        h = _vp()
+        _check(lib().wdl_server_create(device, int(hbm_budget_bytes), int(max_slots_per_pool),
+                                       ctypes.byref(h)), "wdl_server_create")
+        self.handle = h
+        self.K = {}
+
+    def add_dictionary(self, atoms, W, H, gamma, loss=LOSS_QUADRATIC, max_iterations=500,
+                       flags=0) -> int:
+        a = _f64(atoms)
+        K = a.size // (W * H)
+        d = ctypes.c_int()
+        _check(lib().wdl_server_add_dictionary(self.handle, W, H, K, float(gamma), loss,
+                                               int(max_iterations), flags, _dp(a),
+                                               ctypes.byref(d)), "wdl_server_add_dictionary")
+        self.K[d.value] = K
+        return d.value
+
+    def submit(self, dict_id, histogram, iterations, rounds, step=0.5) -> int:
+        h = _f64(histogram)
+        rid = ctypes.c_longlong()
+        _check(lib().wdl_server_submit(self.handle, dict_id, _dp(h), int(iterations), int(rounds),
+                                       float(step), ctypes.byref(rid)), "wdl_server_submit")
+        return (rid.value, dict_id)
+
+    def wait(self, request, timeout_s=60.0):
+        """(weights, loss, latency_ms, queue_ms) of a submitted request."""
+        rid, dict_id = request
+        w = np.zeros(self.K[dict_id])
+        loss, lat, qu = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        st = lib().wdl_server_wait(self.handle, rid, float(timeout_s), _dp(w), ctypes.byref(loss),
+                                   ctypes.byref(lat), ctypes.byref(qu))
+        if st == 1:
+            raise TimeoutError("request %d not done after %g s" % (rid, timeout_s))
+        _check(st, "wdl_server_wait")
+        return w, loss.value, lat.value, qu.value
+
+    def stats(self) -> dict:
+        out = (ctypes.c_longlong * 6)()
+        _check(lib().wdl_server_stats(self.handle, out), "wdl_server_stats")
+        return dict(zip(self.STATS, [int(v) for v in out]))
+
+    def close(self):
+        if self.handle:
+            lib().wdl_server_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

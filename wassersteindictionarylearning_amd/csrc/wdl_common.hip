@@ -129,6 +129,15 @@ int wdl_ctx_destroy(wdl_ctx* ctx) {
 
 void* wdl_ctx_stream(wdl_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
+int wdl_ctx_query(wdl_ctx* ctx) {
+  WDL_CHECK(ctx != nullptr, WDL_ERR_ARG, "wdl_ctx_query: ctx is NULL");
+  const hipError_t e = hipStreamQuery(ctx->stream);
+  if (e == hipSuccess) return 0;
+  if (e == hipErrorNotReady) return 1;
+  wdl::set_error(std::string("hipStreamQuery: ") + hipGetErrorString(e));
+  return WDL_ERR_HIP;
+}
+
 int wdl_ctx_synchronize(wdl_ctx* ctx) {
   WDL_CHECK(ctx != nullptr, WDL_ERR_ARG, "wdl_ctx_synchronize: ctx is NULL");
   WDL_HIP(hipStreamSynchronize(ctx->stream));

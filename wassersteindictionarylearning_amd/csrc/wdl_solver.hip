@@ -110,6 +110,13 @@ __global__ __launch_bounds__(kThreads) void k_final(SolveArgs a) {
   const int K = a.K;
   const long long hbase = (long long)p * K * N;
   const int Lp = steps_of(a, p);
+  if (Lp == 0) {  // idle histogram (continuous batching): no loss, no gradient
+    if (threadIdx.x == 0) {
+      a.lossPart[(long long)p * kLossBlocks + blockIdx.x] = 0.0;
+      a.gsumPart[(long long)p * kLossBlocks + blockIdx.x] = 0.0;
+    }
+    return;
+  }
   const float* F = a.Fh + (long long)(Lp - 1) * a.stage + hbase;
   float* QL = a.Qh + (long long)(Lp - 1) * a.qstage + (long long)p * N;
   const float* lam = a.lam + (long long)p * K;
@@ -409,6 +416,13 @@ __global__ __launch_bounds__(kThreads) void k_final_log(SolveArgs a) {
   const int K = a.K;
   const long long hbase = (long long)p * K * N;
   const int Lp = steps_of(a, p);
+  if (Lp == 0) {
+    if (threadIdx.x == 0) {
+      a.lossPart[(long long)p * kLossBlocks + blockIdx.x] = 0.0;
+      a.gsumPart[(long long)p * kLossBlocks + blockIdx.x] = 0.0;
+    }
+    return;
+  }
   const double* F = (const double*)a.Fh + (long long)(Lp - 1) * a.stage + hbase;
   double* B = (double*)a.Bh + (long long)(Lp - 1) * a.stage + hbase;
   const float* lam = a.lam + (long long)p * K;
@@ -585,6 +599,43 @@ __global__ __launch_bounds__(kThreads, 4) void k_backward_log(SolveArgs a, int t
   }
 }
 
+// ---- projection rounds (continuous batching) ------------------------------------
+// One mirror-descent step on the simplex per active histogram: with the
+// weight gradient G = resultLa (:635-673) and its simplex-tangent projection
+// g = G - <lambda, G> (as evaluate_both, :1095-1099),
+//   lambda_i <- lambda_i exp(-eta g_i / max_j |g_j|) / Z,
+// the exponentiated-gradient update with a step normalised by the largest
+// component (scale-free across losses and grid sizes).
+__global__ void k_project_update(float* __restrict__ lam, const double* __restrict__ GL,
+                                 const int* __restrict__ Lp, const float* __restrict__ eta, int P,
+                                 int K) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P || Lp[p] <= 0) return;
+  float* l = lam + (long long)p * K;
+  const double* g = GL + (long long)p * K;
+  double d = 0;
+  for (int i = 0; i < K; ++i) d += (double)l[i] * g[i];
+  double m = 0;
+  for (int i = 0; i < K; ++i) m = fmax(m, fabs(g[i] - d));
+  if (!(m > 0) || !isfinite(m)) return;
+  const double step = (double)eta[p] / m;
+  double z = 0;
+  for (int i = 0; i < K; ++i) z += (double)l[i] * exp(-step * (g[i] - d));
+  for (int i = 0; i < K; ++i) l[i] = (float)((double)l[i] * exp(-step * (g[i] - d)) / z);
+}
+
+// b_storage <- b_temp for the active histograms (the warm restart of :418-422,
+// per histogram).
+__global__ void k_warm_commit_active(float* __restrict__ store, const float* __restrict__ temp,
+                                     const int* __restrict__ Lp, long long KN) {
+  const int p = blockIdx.y;
+  if (Lp[p] <= 0) return;
+  const long long o = (long long)p * KN;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < KN;
+       e += (long long)gridDim.x * blockDim.x)
+    store[o + e] = temp[o + e];
+}
+
 // Vector width 4 when rows hold whole float4 groups.
 static void launch_forward(const SolveArgs& a, int grid, int lds, hipStream_t st, int s) {
   if (a.logA) {
@@ -626,7 +677,7 @@ struct wdl_solver {
   long long N = 0;
   int D = 0;
   DevBuf tab, A, lam, obs, Bh, Qh, Ch, Fh, V0, V1, R0, R1, GD, GL, Q, G0, lossPart, bStore,
-      bTemp, GDsum, logA, events, Lp, X0, X1, XS0, XS1, GLt, gsumPart, Mt;
+      bTemp, GDsum, logA, events, Lp, X0, X1, XS0, XS1, GLt, gsumPart, Mt, eta;
   bool tiled = false;  // 64 x 64 tiles with a halo (a side above kMaxSide)
   TileGeom tg{};
   hipGraph_t graph = nullptr;
@@ -819,7 +870,7 @@ static long long footprint(int W, int H, int K, int P, int L, int flags) {
   f += 5 * PKN * 4;                          // V0, V1, R0, R1, GD
   f += (long long)K * N * (4 + 8);           // A, GDsum
   f += (long long)P * N * 4 * 3;             // obs, Q, G0
-  f += (long long)P * K * (4 + 8) + 2LL * P * kLossBlocks * 8;
+  f += (long long)P * K * (4 + 8) + 2LL * P * kLossBlocks * 8 + (long long)P * 4;
   if (flags & WDL_FLAG_WARM_RESTART) f += 2 * PKN * 4;
   if (flags & WDL_FLAG_LOG_DOMAIN) f += (long long)K * N * 4;  // log atoms
   f += kTabLen * 4;
@@ -910,6 +961,7 @@ int wdl_solver_create(wdl_ctx* ctx, int W, int H, int K, int P, int L, double ga
       {&s->GLt, tiled ? (size_t)P * K * ntiles * sizeof(double) : 0},
       {&s->Mt, tiled ? (size_t)L * P * K * ntiles * sizeof(double) : 0},
       {&s->gsumPart, (size_t)P * kLossBlocks * sizeof(double)},
+      {&s->eta, (size_t)P * sizeof(float)},
   };
   for (auto& al : allocs) {
     if ((st = al.b->ensure(al.n))) {
@@ -961,7 +1013,7 @@ int wdl_solver_destroy(wdl_solver* s) {
   DevBuf* bufs[] = {&s->tab, &s->A,  &s->lam, &s->obs, &s->Bh, &s->Qh, &s->Ch,  &s->Fh,
                     &s->V0,  &s->V1, &s->R0,  &s->R1,  &s->GD, &s->GL,       &s->Q,
                     &s->G0,  &s->lossPart,    &s->bStore,      &s->bTemp,    &s->GDsum,
-                    &s->logA, &s->events, &s->Lp, &s->X0, &s->X1, &s->XS0, &s->XS1, &s->GLt, &s->Mt, &s->gsumPart};
+                    &s->logA, &s->events, &s->Lp, &s->X0, &s->X1, &s->XS0, &s->XS1, &s->GLt, &s->Mt, &s->gsumPart, &s->eta};
   for (DevBuf* b : bufs) b->release();
   delete s;
   return WDL_OK;
@@ -1187,8 +1239,8 @@ int wdl_solver_set_iterations(wdl_solver* s, const int* iterations) {
   std::vector<int> lp((size_t)s->P, s->L);
   if (iterations)
     for (int p = 0; p < s->P; ++p) {
-      WDL_CHECK(iterations[p] >= 1 && iterations[p] <= s->L, WDL_ERR_ARG,
-                "wdl_solver_set_iterations: iterations must be in 1..L");
+      WDL_CHECK(iterations[p] >= 0 && iterations[p] <= s->L, WDL_ERR_ARG,
+                "wdl_solver_set_iterations: iterations must be in 0..L");
       lp[p] = iterations[p];
     }
   WDL_HIP(hipSetDevice(s->ctx->device));
@@ -1209,6 +1261,92 @@ int wdl_solver_get_grad_dic(wdl_solver* s, double* grad_dic) {
   WDL_HIP(hipMemcpyAsync(h.data(), s->GD.p, n * 4, hipMemcpyDeviceToHost, s->ctx->stream));
   WDL_HIP(hipStreamSynchronize(s->ctx->stream));
   for (size_t e = 0; e < n; ++e) grad_dic[e] = h[e];
+  return WDL_OK;
+}
+
+int wdl_solver_set_slot(wdl_solver* s, int slot, const double* observed, const double* lambdas,
+                        int iterations, double step) {
+  WDL_CHECK(s, WDL_ERR_ARG, "wdl_solver_set_slot: NULL argument");
+  WDL_CHECK(slot >= 0 && slot < s->P, WDL_ERR_ARG, "wdl_solver_set_slot: slot out of 0..P-1");
+  WDL_CHECK(iterations >= 0 && iterations <= s->L, WDL_ERR_ARG,
+            "wdl_solver_set_slot: iterations must be in 0..L");
+  WDL_CHECK(std::isfinite(step) && step >= 0, WDL_ERR_ARG,
+            "wdl_solver_set_slot: step must be finite and >= 0");
+  const int K = s->K;
+  const long long N = s->N;
+  hipStream_t st = s->ctx->stream;
+  WDL_HIP(hipSetDevice(s->ctx->device));
+  WDL_HIP(hipStreamSynchronize(st));  // the slot's arrays may be in use by a running round
+  std::vector<float> ho, hl;
+  if (observed) {
+    ho.resize((size_t)N);
+    for (long long x = 0; x < N; ++x) ho[(size_t)x] = (float)observed[x];
+    WDL_HIP(hipMemcpyAsync(s->obs.as<float>() + (long long)slot * N, ho.data(), N * sizeof(float),
+                           hipMemcpyHostToDevice, st));
+    s->observed = true;
+  }
+  if (lambdas) {  // normalize_values (:53-74)
+    double t = 0;
+    for (int i = 0; i < K; ++i) t += lambdas[i];
+    WDL_CHECK(t > 0 && std::isfinite(t), WDL_ERR_ARG, "wdl_solver_set_slot: weights must sum > 0");
+    hl.resize((size_t)K);
+    for (int i = 0; i < K; ++i) hl[(size_t)i] = (float)(lambdas[i] / t);
+    WDL_HIP(hipMemcpyAsync(s->lam.as<float>() + (long long)slot * K, hl.data(), K * sizeof(float),
+                           hipMemcpyHostToDevice, st));
+  }
+  const float et = (float)step;
+  WDL_HIP(hipMemcpyAsync(s->Lp.as<int>() + slot, &iterations, sizeof(int), hipMemcpyHostToDevice, st));
+  WDL_HIP(hipMemcpyAsync(s->eta.as<float>() + slot, &et, sizeof(float), hipMemcpyHostToDevice, st));
+  if (s->flags & WDL_FLAG_WARM_RESTART) {
+    // the start value of b_storage, 1 in both builds of the reference (:380-384)
+    hipLaunchKernelGGL(k_fill, dim3(256), dim3(256), 0, st,
+                       s->bStore.as<float>() + (long long)slot * K * N, 1.f, (long long)K * N);
+    WDL_HIP(hipGetLastError());
+  }
+  WDL_HIP(hipStreamSynchronize(st));
+  return WDL_OK;
+}
+
+int wdl_solver_projection_round(wdl_solver* s) {
+  WDL_CHECK(s, WDL_ERR_ARG, "wdl_solver_projection_round: NULL argument");
+  const int need = WDL_FLAG_WEIGHTS_ONLY | WDL_FLAG_WARM_RESTART;
+  WDL_CHECK((s->flags & need) == need, WDL_ERR_STATE,
+            "wdl_solver_projection_round: needs WDL_FLAG_WEIGHTS_ONLY | WDL_FLAG_WARM_RESTART");
+  int st = wdl_solver_run(s, WDL_RUN_BATCHED);
+  if (st) return st;
+  hipStream_t str = s->ctx->stream;
+  hipLaunchKernelGGL(k_project_update, dim3((s->P + 63) / 64), dim3(64), 0, str, s->lam.as<float>(),
+                     s->GL.as<double>(), s->Lp.as<int>(), s->eta.as<float>(), s->P, s->K);
+  const long long KN = (long long)s->K * s->N;
+  hipLaunchKernelGGL(k_warm_commit_active, dim3((unsigned)std::min<long long>(64, (KN + 255) / 256),
+                     (unsigned)s->P), dim3(256), 0, str, s->bStore.as<float>(), s->bTemp.as<float>(),
+                     s->Lp.as<int>(), KN);
+  WDL_HIP(hipGetLastError());
+  return WDL_OK;
+}
+
+int wdl_solver_get_slot(wdl_solver* s, int slot, double* lambdas, double* loss) {
+  WDL_CHECK(s, WDL_ERR_ARG, "wdl_solver_get_slot: NULL argument");
+  WDL_CHECK(slot >= 0 && slot < s->P, WDL_ERR_ARG, "wdl_solver_get_slot: slot out of 0..P-1");
+  const int K = s->K;
+  std::vector<float> hl((size_t)K);
+  std::vector<double> lp(kLossBlocks);
+  hipStream_t st = s->ctx->stream;
+  WDL_HIP(hipSetDevice(s->ctx->device));
+  if (lambdas)
+    WDL_HIP(hipMemcpyAsync(hl.data(), s->lam.as<float>() + (long long)slot * K, K * sizeof(float),
+                           hipMemcpyDeviceToHost, st));
+  if (loss)
+    WDL_HIP(hipMemcpyAsync(lp.data(), s->lossPart.as<double>() + (long long)slot * kLossBlocks,
+                           kLossBlocks * sizeof(double), hipMemcpyDeviceToHost, st));
+  WDL_HIP(hipStreamSynchronize(st));
+  if (lambdas)
+    for (int i = 0; i < K; ++i) lambdas[i] = hl[(size_t)i];
+  if (loss) {
+    double t = 0;
+    for (int b = 0; b < kLossBlocks; ++b) t += lp[(size_t)b];
+    *loss = t;
+  }
   return WDL_OK;
 }
 
