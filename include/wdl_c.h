@@ -40,6 +40,8 @@ extern "C" {
 #define WDL_FLAG_NO_GRAPH 8     /* batched runs launch eagerly instead of replaying a hipGraph */
 #define WDL_FLAG_WEIGHTS_ONLY 16 /* only dL/dlambda (:635-673): projection onto a fixed dictionary */
 #define WDL_FLAG_TILED 32       /* 64x64-tile kernels at any size (taken anyway above 128 per side) */
+#define WDL_FLAG_EXACT_LSE 64   /* log domain: exact per-output maximum (kernels.h:1082-1091) for
+                                   any spread of log values; FP32 sums pruned at e^-40 */
 
 /* Run modes of wdl_solver_run. */
 #define WDL_RUN_BATCHED 0 /* all P solves advance together, one launch per Sinkhorn stage */
@@ -173,8 +175,8 @@ int wdl_server_create(int device, long long hbm_budget_bytes, int max_slots_per_
                       wdl_server** out);
 int wdl_server_destroy(wdl_server* sv);
 /* A dictionary: K atoms (K x W x H, normalised per atom by the solver), gamma,
- * loss, the largest iteration count a request may ask for, flags 0 or
- * WDL_FLAG_LOG_DOMAIN.  Returns its id. */
+ * loss, the largest iteration count a request may ask for, flags 0,
+ * WDL_FLAG_LOG_DOMAIN or WDL_FLAG_LOG_DOMAIN | WDL_FLAG_EXACT_LSE.  Returns its id. */
 int wdl_server_add_dictionary(wdl_server* sv, int W, int H, int K, double gamma, int loss,
                               int max_iterations, int flags, const double* atoms, int* dict_id);
 /* Queue a request (the histogram is copied): `rounds` projection rounds of

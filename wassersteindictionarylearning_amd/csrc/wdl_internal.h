@@ -55,6 +55,13 @@ ConvGeom make_geom(int W, int H, int D);
 // Returns WDL_ERR_UNSUPPORTED when the halo would exceed kHaloMax.
 int make_tile_geom(int W, int H, int D, double gamma, bool logdom, TileGeom& t);
 
+// Exact-shift log-domain convolution of nimg W x H images (wdl_xlse.hip):
+// signs one byte per value (nullptr: all non-negative); tmp / stmp hold the
+// transposed intermediate.
+int xlse_convolve(const double* in, const unsigned char* sin, double* out, unsigned char* sout,
+                  double* tmp, unsigned char* stmp, int W, int H, long long nimg, double gamma,
+                  hipStream_t st);
+
 // Images with a side above kMaxSide take the tiled kernels.
 inline bool needs_tiles(int W, int H) { return W > kMaxSide || H > kMaxSide; }
 

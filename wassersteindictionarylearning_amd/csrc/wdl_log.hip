@@ -82,6 +82,14 @@ __global__ void k_pack_signs(const unsigned char* __restrict__ bytes, unsigned c
   }
 }
 
+// Unpacks bits (LSB first) into one byte per value.
+__global__ void k_unpack_signs(const unsigned char* __restrict__ bits,
+                               unsigned char* __restrict__ bytes, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    bytes[i] = (bits[i >> 3] >> (i & 7)) & 1;
+}
+
 static int lse_launch_bytes(int len) { return len * (int)(sizeof(double) + sizeof(float)); }
 
 // Two passes over nvec images; signs == nullptr for the unsigned form.
@@ -98,6 +106,25 @@ static int log_convolve_dev(wdl_gauss2d* k, const double* d_in, double* d_out, u
   const int g1 = (int)std::min<long long>((long long)nvec * H, 65535);
   const int g2 = (int)std::min<long long>((long long)nvec * W, 65535);
   double* tmp = k->io_tmp.as<double>();
+  if (k->flags & WDL_FLAG_EXACT_LSE) {  // exact-shift passes, FP32 sums (wdl_xlse.hip)
+    WDL_CHECK(std::max(W, H) <= 1024, WDL_ERR_UNSUPPORTED,
+              "log_convolve: WDL_FLAG_EXACT_LSE takes sides up to 1024");
+    if (sgn && (rc = k->io_sign.ensure(3 * n))) return rc;
+    b1 = k->io_sign.as<unsigned char>();
+    b2 = b1 + n;
+    unsigned char* b0 = b1 + 2 * n;
+    const int gp = (int)std::min<long long>((n + 255) / 256, 8192);
+    if (sgn) hipLaunchKernelGGL(k_unpack_signs, dim3(gp), dim3(256), 0, st, d_bits, b0, n);
+    if ((rc = xlse_convolve(d_in, sgn ? b0 : nullptr, d_out, sgn ? b2 : nullptr, tmp,
+                            sgn ? b1 : nullptr, W, H, nvec, k->gamma, st)))
+      return rc;
+    if (sgn) {
+      const int gq = (int)std::min<long long>(((n + 7) / 8 + 255) / 256, 4096);
+      hipLaunchKernelGGL(k_pack_signs, dim3(gq), dim3(256), 0, st, b2, d_bits, n);
+    }
+    WDL_HIP(hipGetLastError());
+    return WDL_OK;
+  }
   if (sgn) {
     hipLaunchKernelGGL((k_lse_lines<true, true>), dim3(g1), dim3(kLineThreads), lse_launch_bytes(W),
                        st, d_in, d_bits, tmp, b1, H, W, nvec, k->gamma);

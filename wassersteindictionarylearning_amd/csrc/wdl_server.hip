@@ -354,8 +354,8 @@ int wdl_server_add_dictionary(wdl_server* sv, int W, int H, int K, double gamma,
   WDL_CHECK(sv && atoms && dict_id, WDL_ERR_ARG, "wdl_server_add_dictionary: NULL argument");
   WDL_CHECK(W > 0 && H > 0 && K > 0 && max_iterations > 0, WDL_ERR_ARG,
             "wdl_server_add_dictionary: W, H, K, max_iterations must be positive");
-  WDL_CHECK((flags & ~WDL_FLAG_LOG_DOMAIN) == 0, WDL_ERR_ARG,
-            "wdl_server_add_dictionary: only WDL_FLAG_LOG_DOMAIN may be given");
+  WDL_CHECK((flags & ~(WDL_FLAG_LOG_DOMAIN | WDL_FLAG_EXACT_LSE)) == 0, WDL_ERR_ARG,
+            "wdl_server_add_dictionary: flags may be WDL_FLAG_LOG_DOMAIN, WDL_FLAG_EXACT_LSE");
   const long long slot = wdl_solver_footprint(W, H, K, 1, max_iterations, flags | kPoolFlags);
   WDL_CHECK(slot > 0, WDL_ERR_ARG, "wdl_server_add_dictionary: bad sizes");
   WDL_CHECK(slot <= hbm_budget_of(sv), WDL_ERR_NOMEM,

@@ -50,6 +50,14 @@ struct SolveArgs {
   double* GLt;
   double* gsumPart;  // [P][kLossBlocks] partial sums of g_L = loss'(q) q (k_final)
   double* Mt;        // tiled: [L][P][K][tiles] partial sums of log phi (forward)
+  // exact-shift log domain: gamma, FP64 scratch images and sign bytes
+  double gamma;
+  double* xa;
+  double* xb;
+  double* xt;
+  unsigned char* sga;
+  unsigned char* sgb;
+  unsigned char* sgt;
 };
 
 // Workgroup -> (histogram, atom).  With 8+ histograms the K workgroups of a
@@ -149,6 +157,12 @@ int tiled_set_lds_attr();
 int tiled_forward(const SolveArgs& a, hipStream_t st, int s);
 int tiled_backward(const SolveArgs& a, hipStream_t st, int t);
 int tiled_reduce_gl(const SolveArgs& a, hipStream_t st);
+
+// Exact-shift log-domain stages (wdl_xlse.hip).
+int xlse_set_lds_attr(int maxlen);
+int exact_forward(const SolveArgs& a, hipStream_t st, int s);
+int exact_backward(const SolveArgs& a, hipStream_t st, int t);
+constexpr int kXMaxLen = 1024;  // longest line of the exact-shift passes
 
 // Sum over the workgroup, result valid in thread 0.
 __device__ inline double block_sum(double v) {

@@ -677,8 +677,10 @@ struct wdl_solver {
   long long N = 0;
   int D = 0;
   DevBuf tab, A, lam, obs, Bh, Qh, Ch, Fh, V0, V1, R0, R1, GD, GL, Q, G0, lossPart, bStore,
-      bTemp, GDsum, logA, events, Lp, X0, X1, XS0, XS1, GLt, gsumPart, Mt, eta;
+      bTemp, GDsum, logA, events, Lp, X0, X1, XS0, XS1, GLt, gsumPart, Mt, eta, xa, xb, xt, sga,
+      sgb, sgt;
   bool tiled = false;  // 64 x 64 tiles with a halo (a side above kMaxSide)
+  bool exact = false;  // log domain with the exact-shift line passes (WDL_FLAG_EXACT_LSE)
   TileGeom tg{};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -732,6 +734,13 @@ struct wdl_solver {
     a.GLt = GLt.as<double>();
     a.gsumPart = gsumPart.as<double>();
     a.Mt = Mt.as<double>();
+    a.gamma = gamma;
+    a.xa = xa.as<double>();
+    a.xb = xb.as<double>();
+    a.xt = xt.as<double>();
+    a.sga = sga.as<unsigned char>();
+    a.sgb = sgb.as<unsigned char>();
+    a.sgt = sgt.as<unsigned char>();
     return a;
   }
 
@@ -818,10 +827,13 @@ struct wdl_solver {
     const int gs = stage_grid(np, K);
     int e;
     for (int s = 1; s <= L; ++s) {
-      if (!tiled)
+      if (exact) {
+        if ((e = exact_forward(a, st, s))) return e;
+      } else if (!tiled) {
         launch_forward(a, gs, lds, st, s);
-      else if ((e = tiled_forward(a, st, s)))
+      } else if ((e = tiled_forward(a, st, s))) {
         return e;
+      }
     }
     // kLossBlocks blocks per histogram: every lossPart slot is written
     // (blocks past the last pixel write 0).
@@ -830,12 +842,15 @@ struct wdl_solver {
     else
       hipLaunchKernelGGL(k_final, dim3(kLossBlocks, np), blk, 0, st, a);
     for (int s = L; s >= 1; --s) {
-      if (!tiled)
+      if (exact) {
+        if ((e = exact_backward(a, st, s))) return e;
+      } else if (!tiled) {
         launch_backward(a, gs, lds, st, s);
-      else if ((e = tiled_backward(a, st, s)))
+      } else if ((e = tiled_backward(a, st, s))) {
         return e;
+      }
     }
-    if (tiled && (e = tiled_reduce_gl(a, st))) return e;
+    if (tiled && !exact && (e = tiled_reduce_gl(a, st))) return e;
     WDL_HIP(hipGetLastError());
     return (reduce && !(flags & WDL_FLAG_WEIGHTS_ONLY)) ? reduce_gd(st) : WDL_OK;
   }
@@ -858,8 +873,12 @@ static int set_lds_attr() {
   return WDL_OK;
 }
 
+static bool use_exact(int flags) {
+  return (flags & WDL_FLAG_EXACT_LSE) && (flags & WDL_FLAG_LOG_DOMAIN);
+}
+
 static bool use_tiles(int W, int H, int flags) {
-  return needs_tiles(W, H) || (flags & WDL_FLAG_TILED);
+  return !use_exact(flags) && (needs_tiles(W, H) || (flags & WDL_FLAG_TILED));
 }
 
 static long long footprint(int W, int H, int K, int P, int L, int flags) {
@@ -874,6 +893,7 @@ static long long footprint(int W, int H, int K, int P, int L, int flags) {
   if (flags & WDL_FLAG_WARM_RESTART) f += 2 * PKN * 4;
   if (flags & WDL_FLAG_LOG_DOMAIN) f += (long long)K * N * 4;  // log atoms
   f += kTabLen * 4;
+  if (use_exact(flags)) f += 3LL * 2 * PKN * (8 + 1);  // FP64 scratch images + sign bytes
   if (use_tiles(W, H, flags)) {  // X0, X1 (+ sign bytes), per-tile dots
     const long long nt = (long long)((W + kTile - 1) / kTile) * ((H + kTile - 1) / kTile);
     f += 2 * PKN * (logd ? 9 : 4) + (long long)P * K * nt * 8 * (1 + L);
@@ -897,6 +917,10 @@ int wdl_solver_create(wdl_ctx* ctx, int W, int H, int K, int P, int L, double ga
             "wdl_solver_create: gamma must be positive and finite");
   WDL_CHECK(loss == WDL_LOSS_QUADRATIC || loss == WDL_LOSS_KL || loss == WDL_LOSS_TV,
             WDL_ERR_UNSUPPORTED, "wdl_solver_create: loss must be TV (1), quadratic (2) or KL (4)");
+  WDL_CHECK(!(flags & WDL_FLAG_EXACT_LSE) || (flags & WDL_FLAG_LOG_DOMAIN), WDL_ERR_ARG,
+            "wdl_solver_create: WDL_FLAG_EXACT_LSE needs WDL_FLAG_LOG_DOMAIN");
+  WDL_CHECK(!use_exact(flags) || std::max(W, H) <= kXMaxLen, WDL_ERR_UNSUPPORTED,
+            "wdl_solver_create: exact-shift passes take sides up to " + std::to_string(kXMaxLen));
   const bool tiled = use_tiles(W, H, flags);
   WDL_CHECK(!(tiled && (flags & WDL_FLAG_DENSE)), WDL_ERR_UNSUPPORTED,
             "wdl_solver_create: WDL_FLAG_DENSE needs sides of at most " + std::to_string(kMaxSide) +
@@ -909,8 +933,10 @@ int wdl_solver_create(wdl_ctx* ctx, int W, int H, int K, int P, int L, double ga
   WDL_HIP(hipSetDevice(ctx->device));
   if ((st = set_lds_attr())) return st;
   if ((st = tiled_set_lds_attr())) return st;
+  if (use_exact(flags) && (st = xlse_set_lds_attr(std::max(W, H)))) return st;
   wdl_solver* s = new wdl_solver();
   s->tiled = tiled;
+  s->exact = use_exact(flags);
   s->tg = tg;
   s->ctx = ctx;
   s->W = W;
@@ -962,6 +988,12 @@ int wdl_solver_create(wdl_ctx* ctx, int W, int H, int K, int P, int L, double ga
       {&s->Mt, tiled ? (size_t)L * P * K * ntiles * sizeof(double) : 0},
       {&s->gsumPart, (size_t)P * kLossBlocks * sizeof(double)},
       {&s->eta, (size_t)P * sizeof(float)},
+      {&s->xa, use_exact(flags) ? (size_t)2 * PKN * sizeof(double) : 0},
+      {&s->xb, use_exact(flags) ? (size_t)2 * PKN * sizeof(double) : 0},
+      {&s->xt, use_exact(flags) ? (size_t)2 * PKN * sizeof(double) : 0},
+      {&s->sga, use_exact(flags) ? (size_t)2 * PKN : 0},
+      {&s->sgb, use_exact(flags) ? (size_t)2 * PKN : 0},
+      {&s->sgt, use_exact(flags) ? (size_t)2 * PKN : 0},
   };
   for (auto& al : allocs) {
     if ((st = al.b->ensure(al.n))) {
@@ -1013,7 +1045,8 @@ int wdl_solver_destroy(wdl_solver* s) {
   DevBuf* bufs[] = {&s->tab, &s->A,  &s->lam, &s->obs, &s->Bh, &s->Qh, &s->Ch,  &s->Fh,
                     &s->V0,  &s->V1, &s->R0,  &s->R1,  &s->GD, &s->GL,       &s->Q,
                     &s->G0,  &s->lossPart,    &s->bStore,      &s->bTemp,    &s->GDsum,
-                    &s->logA, &s->events, &s->Lp, &s->X0, &s->X1, &s->XS0, &s->XS1, &s->GLt, &s->Mt, &s->gsumPart, &s->eta};
+                    &s->logA, &s->events, &s->Lp, &s->X0, &s->X1, &s->XS0, &s->XS1, &s->GLt, &s->Mt, &s->gsumPart, &s->eta, &s->xa, &s->xb, &s->xt, &s->sga, &s->sgb,
+                    &s->sgt};
   for (DevBuf* b : bufs) b->release();
   delete s;
   return WDL_OK;
@@ -1177,6 +1210,9 @@ int wdl_solver_run_stage(wdl_solver* s, int stage, int step) {
   const SolveArgs a = s->args(0, s->P);
   const int lds = s->lds_bytes(a);  // all P solves, on the context stream
   const int gs = stage_grid(s->P, s->K);
+  if (s->exact)
+    return stage == WDL_STAGE_FORWARD ? exact_forward(a, s->ctx->stream, step)
+                                      : exact_backward(a, s->ctx->stream, step);
   if (s->tiled)
     return stage == WDL_STAGE_FORWARD ? tiled_forward(a, s->ctx->stream, step)
                                       : tiled_backward(a, s->ctx->stream, step);

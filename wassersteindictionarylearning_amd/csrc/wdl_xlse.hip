@@ -1,0 +1,391 @@
+// Exact-shift log-domain convolution, and the log-domain solver stages built
+// on it (WDL_FLAG_EXACT_LSE).
+//
+// Each 1-D pass computes, for every output j of a line x,
+//     m_j = max(-1e99, max_k(x_k - (j-k)^2/gamma))               (kernels.h:1082-1091)
+//     out_j = m_j + log |sum_k s_k exp(x_k - (j-k)^2/gamma - m_j)|
+// with the reference's exact per-output maximum, so any spread of log values
+// along a line is handled (the shifted MFMA form of wdl_logconv.h needs every
+// line within ~80 nats).  The line is split into 16-pixel chunks with their
+// maxima; a chunk whose bound max_c - d_min^2/gamma cannot raise m_j is not
+// visited for the maximum, and one that lies more than 40 nats below m_j is
+// not visited for the sum (its terms are below e^-40 each: at most len * 4e-18
+// relative).  Arguments are formed in FP64, the terms exponentiated and summed
+// in FP32 (each term <= 1).  Cost per output: one bound per chunk plus the
+// visited chunks' terms -- O(len) per line in the worst case, a few chunks when
+// the log values are smooth.
+//
+// The solver stages (exact_forward / exact_backward) run the log-domain
+// recursions of compute_gradient_both_log_signArray (inverseWasserstein.h:
+// 678-814) as elementwise kernels around these passes, with FP64 scratch
+// images between them.
+#include <algorithm>
+#include <cmath>
+
+#include "wdl_solve_common.h"
+
+namespace wdl {
+
+constexpr int kXWaves = 4;  // lines per workgroup, one wave each
+constexpr double kXPrune = 40.0;
+
+__host__ __device__ inline int xlse_lds_bytes(int len) {
+  const int nch = (len + 15) / 16;
+  return kXWaves * (len * (8 + 8 + 4 + 1) + nch * 8 + 16);
+}
+
+// One pass over `nlines` lines of length `len` (line-major input), writing the
+// transposed layout (line index fastest) of each image of `rows` lines.  A line
+// is skipped (output untouched) when active_thr > 0 and Lp[p0 + image / ipp] <
+// active_thr.
+template <bool SIGNED>
+__global__ __launch_bounds__(kXWaves * 64) void k_xlse_pass(
+    const double* __restrict__ in, const unsigned char* __restrict__ sin, double* __restrict__ out,
+    unsigned char* __restrict__ sout, int rows, int len, long long nlines, double inv_gamma,
+    const int* __restrict__ Lp, int p0, int ipp, int active_thr) {
+  extern __shared__ __align__(16) double xs[];
+  const int nch = (len + 15) >> 4;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // per wave: its line X, outputs O (+ signs SO), chunk maxima CM, input signs SG;
+  // O0 / SO0 address the outputs of all waves for the transposed store
+  double* X = xs + wave * len;
+  double* O0 = xs + kXWaves * len;
+  double* O = O0 + wave * len;
+  double* CM = xs + 2 * kXWaves * len + wave * nch;
+  float* SG = reinterpret_cast<float*>(xs + 2 * kXWaves * len + kXWaves * nch) + wave * len;
+  unsigned char* SO0 = reinterpret_cast<unsigned char*>(
+      reinterpret_cast<float*>(xs + 2 * kXWaves * len + kXWaves * nch) + kXWaves * len);
+  unsigned char* SO = SO0 + wave * len;
+  for (long long l0 = (long long)blockIdx.x * kXWaves; l0 < nlines;
+       l0 += (long long)gridDim.x * kXWaves) {
+    const long long line = l0 + wave;
+    bool active = line < nlines;
+    if (active && active_thr > 0) {
+      const long long img = line / rows;
+      active = Lp[p0 + (int)(img / ipp)] >= active_thr;
+    }
+    if (active) {
+      const double* src = in + line * len;
+      for (int k = lane; k < len; k += 64) {
+        X[k] = src[k];
+        if (SIGNED) SG[k] = sin[line * len + k] ? 1.f : -1.f;  // getSign (signArray.h:19-22)
+      }
+    }
+    __syncthreads();
+    if (active)
+      for (int c = lane; c < nch; c += 64) {
+        double m = -INFINITY;
+        for (int k = c * 16; k < min(len, c * 16 + 16); ++k) m = fmax(m, X[k]);
+        CM[c] = m;
+      }
+    __syncthreads();
+    if (active)
+      for (int j = lane; j < len; j += 64) {
+        const int cj = j >> 4;
+        double m = -INFINITY;
+        for (int k = cj * 16; k < min(len, cj * 16 + 16); ++k)
+          m = fmax(m, X[k] - (double)((j - k) * (j - k)) * inv_gamma);
+        for (int c = 0; c < nch; ++c) {
+          if (c == cj) continue;
+          const int d = c < cj ? j - (c * 16 + 15) : c * 16 - j;
+          if (CM[c] - (double)(d * d) * inv_gamma <= m) continue;
+          for (int k = c * 16; k < min(len, c * 16 + 16); ++k)
+            m = fmax(m, X[k] - (double)((j - k) * (j - k)) * inv_gamma);
+        }
+        m = fmax(m, -1e99);
+        float s = 0.f;
+        for (int c = 0; c < nch; ++c) {
+          const int d = c == cj ? 0 : (c < cj ? j - (c * 16 + 15) : c * 16 - j);
+          if (CM[c] - (double)(d * d) * inv_gamma - m < -kXPrune) continue;
+          for (int k = c * 16; k < min(len, c * 16 + 16); ++k) {
+            const float e = expf((float)(X[k] - (double)((j - k) * (j - k)) * inv_gamma - m));
+            s += SIGNED ? SG[k] * e : e;
+          }
+        }
+        O[j] = m + hlog(fabsf(s));
+        if (SIGNED) SO[j] = s >= 0.f ? 1 : 0;
+      }
+    __syncthreads();
+    // transposed store: the lines of one workgroup are consecutive, so each
+    // output position j gets up to kXWaves adjacent values
+    for (int e = threadIdx.x; e < kXWaves * len; e += blockDim.x) {
+      const int w = e % kXWaves, j = e / kXWaves;
+      const long long ln = l0 + w;
+      if (ln >= nlines) continue;
+      const long long img = ln / rows;
+      if (active_thr > 0 && Lp[p0 + (int)(img / ipp)] < active_thr) continue;
+      const int r = (int)(ln - img * rows);
+      const long long o = img * (long long)rows * len + (long long)j * rows + r;
+      out[o] = O0[w * len + j];
+      if (SIGNED) sout[o] = SO0[w * len + j];
+    }
+    __syncthreads();
+  }
+}
+
+// Two passes (along x, then along y) over nimg W x H images; tmp holds the
+// transposed intermediate.  Signs: none (sin == nullptr) or one byte per value.
+static int xlse_conv(const double* in, const unsigned char* sin, double* out, unsigned char* sout,
+                     double* tmp, unsigned char* stmp, int W, int H, long long nimg, double gamma,
+                     const int* Lp, int p0, int ipp, int thr, hipStream_t st) {
+  const double ig = 1.0 / gamma;
+  const int thr_ = Lp ? thr : 0;
+  auto launch = [&](const double* i, const unsigned char* si, double* o, unsigned char* so,
+                    int rows, int len) {
+    const long long nl = nimg * rows;
+    const int grid = (int)std::min<long long>((nl + kXWaves - 1) / kXWaves, 1 << 20);
+    const int lds = xlse_lds_bytes(len);
+    if (si)
+      hipLaunchKernelGGL(k_xlse_pass<true>, dim3(grid), dim3(kXWaves * 64), lds, st, i, si, o, so,
+                         rows, len, nl, ig, Lp, p0, ipp, thr_);
+    else
+      hipLaunchKernelGGL(k_xlse_pass<false>, dim3(grid), dim3(kXWaves * 64), lds, st, i, si, o,
+                         so, rows, len, nl, ig, Lp, p0, ipp, thr_);
+  };
+  launch(in, sin, tmp, sin ? stmp : nullptr, H, W);   // along x: lines are rows
+  launch(tmp, sin ? stmp : nullptr, out, sout, W, H);  // along y: lines are columns
+  WDL_HIP(hipGetLastError());
+  return WDL_OK;
+}
+
+int xlse_set_lds_attr(int maxlen) {
+  static int done = 0;
+  if (done >= maxlen) return WDL_OK;
+  const int b = xlse_lds_bytes(maxlen);
+  WDL_HIP(hipFuncSetAttribute((const void*)k_xlse_pass<true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, b));
+  WDL_HIP(hipFuncSetAttribute((const void*)k_xlse_pass<false>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, b));
+  done = maxlen;
+  return WDL_OK;
+}
+
+int xlse_convolve(const double* in, const unsigned char* sin, double* out, unsigned char* sout,
+                  double* tmp, unsigned char* stmp, int W, int H, long long nimg, double gamma,
+                  hipStream_t st) {
+  int e = xlse_set_lds_attr(std::max(W, H));
+  if (e) return e;
+  return xlse_conv(in, sin, out, sout, tmp, stmp, W, H, nimg, gamma, nullptr, 0, 1, 0, st);
+}
+
+// ---- solver stages -------------------------------------------------------------
+// Scratch layout: image ((p * nch + chain slot) * K + i) of N values, nch = 2
+// (atom chain, weight chain) or 1 (forward; weights-only backward), so the
+// images of histograms p0..p0+np-1 are contiguous.
+
+struct XEl {
+  int p, i, cs;
+  long long x, img;  // pixel, (p, i) image offset into the P x K x N arrays
+};
+
+__device__ inline XEl xel(const SolveArgs& a, int nch, long long e) {
+  XEl r;
+  const long long N = a.N;
+  r.x = e % N;
+  const long long im = e / N;
+  r.i = (int)(im % a.K);
+  const long long pc = im / a.K;
+  r.cs = (int)(pc % nch);
+  r.p = a.p0 + (int)(pc / nch);
+  r.img = ((long long)r.p * a.K + r.i) * N;
+  return r;
+}
+
+__device__ inline long long xoff(const SolveArgs& a, int nch, const XEl& r) {
+  return (((long long)r.p * nch + r.cs) * a.K + r.i) * a.N + r.x;
+}
+
+#define XLOOP(total)                                                                     \
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < (total);      \
+       e += (long long)gridDim.x * blockDim.x)
+
+// forward, input of the first convolution: log b^{s-1} (:711-724; b^0: 0 or the store)
+__global__ void k_xf1(SolveArgs a, int s) {
+  const long long tot = (long long)a.np * a.K * a.N;
+  XLOOP(tot) {
+    const XEl r = xel(a, 1, e);
+    double b;
+    if (s > steps_of(a, r.p)) continue;
+    if (s == 1) {
+      b = a.bInit ? (double)a.bInit[r.img + r.x] : 0.0;
+    } else {
+      const double* Fp = (const double*)a.Fh + (long long)(s - 2) * a.stage +
+                         (long long)r.p * a.K * a.N + r.x;
+      double lq = 0;
+      for (int j = 0; j < a.K; ++j) lq += (double)a.lam[(long long)r.p * a.K + j] * Fp[j * a.N];
+      b = lq - Fp[(long long)r.i * a.N];
+      ((double*)a.Bh)[(long long)(s - 2) * a.stage + r.img + r.x] = b;
+    }
+    a.xa[xoff(a, 1, r)] = b;
+  }
+}
+
+// forward, input of the second convolution: log p - log c^{s-1} (:704-707)
+__global__ void k_xf2(SolveArgs a, int s) {
+  const long long tot = (long long)a.np * a.K * a.N;
+  XLOOP(tot) {
+    const XEl r = xel(a, 1, e);
+    if (s > steps_of(a, r.p)) continue;
+    const double c = ((const double*)a.Ch)[(long long)(s - 1) * a.stage + r.img + r.x];
+    a.xa[xoff(a, 1, r)] = (double)a.logA[(long long)r.i * a.N + r.x] - c;
+  }
+}
+
+// backward step, first convolution's input: chain 0 (lam_i n + v_i) b^s, chain 1
+// (lam_i g + r_i) / phi^s, as log magnitude and sign (:740-749, :778-793)
+__global__ void k_xb1(SolveArgs a, int t, int nch) {
+  const long long tot = (long long)a.np * nch * a.K * a.N;
+  XLOOP(tot) {
+    XEl r = xel(a, nch, e);
+    const int s = bwd_step(a, r.p, t);
+    if (s < 1) continue;
+    const int chain = nch == 1 ? 1 : r.cs;
+    const bool last = s == steps_of(a, r.p);
+    const long long N = a.N, hb = (long long)r.p * a.K * N;
+    const float lpi = a.lam[(long long)r.p * a.K + r.i];
+    float acc = 0.f, own = 0.f;
+    const float* Pp = chain == 0 ? ((s & 1) ? a.V0 : a.V1) : ((s & 1) ? a.R0 : a.R1);
+    if (last) {
+      acc = a.G0[(long long)r.p * N + r.x];
+      if (chain == 1) acc *= a.Q[(long long)r.p * N + r.x];
+    } else {
+      for (int j = 0; j < a.K; ++j) {
+        const float v = Pp[hb + j * N + r.x];
+        acc -= v;  // n = -sum v (:764), g = -sum r (:808)
+        if (j == r.i) own = v;
+      }
+    }
+    const float xv = lpi * acc + own;
+    const long long h = (long long)(s - 1) * a.stage + r.img + r.x;
+    const double lh = chain == 0 ? ((const double*)a.Bh)[h] : -((const double*)a.Fh)[h];
+    const long long o = xoff(a, nch, r);
+    a.xa[o] = hlog(fabsf(xv)) + lh;
+    a.sga[o] = xv >= 0.f ? 1 : 0;
+  }
+}
+
+// backward step, after the first convolution: resultDic (chain 0, :750-752) and
+// the second convolution's input log p + c - 2 log c^{s-1}, sign of c (:754-756, :796)
+__global__ void k_xb2(SolveArgs a, int t, int nch) {
+  const long long tot = (long long)a.np * nch * a.K * a.N;
+  XLOOP(tot) {
+    XEl r = xel(a, nch, e);
+    const int s = bwd_step(a, r.p, t);
+    if (s < 1) continue;
+    const int chain = nch == 1 ? 1 : r.cs;
+    const long long o = xoff(a, nch, r);
+    const double c = a.xb[o];
+    const float sg = a.sgb[o] ? 1.f : -1.f;
+    const double cs = ((const double*)a.Ch)[(long long)(s - 1) * a.stage + r.img + r.x];
+    if (chain == 0) {
+      float* gd = a.GD + r.img + r.x;
+      const float prev = s == steps_of(a, r.p) ? 0.f : *gd;
+      *gd = prev + sg * expf((float)(c - cs));
+    }
+    if (s > 1) {
+      a.xa[o] = (double)a.logA[(long long)r.i * a.N + r.x] + c - 2.0 * cs;
+      a.sga[o] = a.sgb[o];
+    }
+  }
+}
+
+// backward step s >= 2, after the second convolution: -v = sign exp(res - log
+// phi^{s-1}) (:757-762), -r = sign exp(res + log b^{s-1}) (:797-805)
+__global__ void k_xb3(SolveArgs a, int t, int nch) {
+  const long long tot = (long long)a.np * nch * a.K * a.N;
+  XLOOP(tot) {
+    XEl r = xel(a, nch, e);
+    const int s = bwd_step(a, r.p, t);
+    if (s < 2) continue;
+    const int chain = nch == 1 ? 1 : r.cs;
+    const long long o = xoff(a, nch, r);
+    const double v = a.xb[o];
+    const float sg = a.sgb[o] ? 1.f : -1.f;
+    if (chain == 0) {
+      const double f = ((const double*)a.Fh)[(long long)(s - 2) * a.stage + r.img + r.x];
+      ((s & 1) ? a.V1 : a.V0)[r.img + r.x] = sg * expf((float)(v - f));
+    } else {
+      const double b = ((const double*)a.Bh)[(long long)(s - 2) * a.stage + r.img + r.x];
+      ((s & 1) ? a.R1 : a.R0)[r.img + r.x] = sg * expf((float)(v + b));
+    }
+  }
+}
+
+// weight-gradient dot of backward step s, <log phi_i^s, g_s> (:786), centred as
+// in the other solvers (exact_gsum); one workgroup per (histogram, atom)
+__global__ __launch_bounds__(kThreads) void k_xdot(SolveArgs a, int t) {
+  const int p = a.p0 + (int)(blockIdx.x / a.K), i = (int)(blockIdx.x % a.K);
+  const int s = bwd_step(a, p, t);
+  if (s < 1) return;
+  const bool last = s == steps_of(a, p);
+  const long long N = a.N, hb = (long long)p * a.K * N;
+  const double* F = (const double*)a.Fh + (long long)(s - 1) * a.stage + hb + (long long)i * N;
+  const float* Rp = (s & 1) ? a.R0 : a.R1;
+  double sfg = 0, sf = 0, sg = 0;
+  for (long long x = threadIdx.x; x < N; x += blockDim.x) {
+    float g = 0.f;
+    if (last) {
+      g = a.G0[(long long)p * N + x] * a.Q[(long long)p * N + x];
+    } else {
+      for (int j = 0; j < a.K; ++j) g -= Rp[hb + j * N + x];
+    }
+    sfg += F[x] * (double)g;
+    sf += F[x];
+    sg += (double)g;
+  }
+  const double tfg = block_sum(sfg), tf = block_sum(sf), tg = block_sum(sg);
+  if (threadIdx.x == 0) {
+    double* gl = a.GL + (long long)p * a.K + i;
+    *gl = last ? tfg : *gl + (tfg - tf / (double)N * (tg - exact_gsum(a, p, s)));
+  }
+}
+
+static dim3 ew_grid(long long n) {
+  return dim3((unsigned)std::min<long long>((n + 255) / 256, 8192));
+}
+
+int exact_forward(const SolveArgs& a, hipStream_t st, int s) {
+  const long long nimg = (long long)a.np * a.K, n = nimg * a.N;
+  const long long off = (long long)a.p0 * a.K * a.N;
+  const int W = a.G.W, H = a.G.H;
+  const double g = a.gamma;
+  int e;
+  hipLaunchKernelGGL(k_xf1, ew_grid(n), dim3(256), 0, st, a, s);
+  if ((e = xlse_conv(a.xa + off, nullptr, (double*)a.Ch + (long long)(s - 1) * a.stage + off,
+                     nullptr, a.xt + off, nullptr, W, H, nimg, g, a.Lp, a.p0, a.K, s, st)))
+    return e;
+  hipLaunchKernelGGL(k_xf2, ew_grid(n), dim3(256), 0, st, a, s);
+  if ((e = xlse_conv(a.xa + off, nullptr, (double*)a.Fh + (long long)(s - 1) * a.stage + off,
+                     nullptr, a.xt + off, nullptr, W, H, nimg, g, a.Lp, a.p0, a.K, s, st)))
+    return e;
+  WDL_HIP(hipGetLastError());
+  return WDL_OK;
+}
+
+int exact_backward(const SolveArgs& a, hipStream_t st, int t) {
+  const int nch = a.weightsOnly ? 1 : 2;
+  const long long nimg = (long long)a.np * nch * a.K, n = nimg * a.N;
+  const long long off = (long long)a.p0 * nch * a.K * a.N;
+  const int W = a.G.W, H = a.G.H;
+  const double g = a.gamma;
+  int e;
+  hipLaunchKernelGGL(k_xdot, dim3((unsigned)(a.np * a.K)), dim3(kThreads), 0, st, a, t);
+  hipLaunchKernelGGL(k_xb1, ew_grid(n), dim3(256), 0, st, a, t, nch);
+  // histogram p runs this step when L_p >= L - t + 1, the second convolution when >= L - t + 2
+  if ((e = xlse_conv(a.xa + off, a.sga + off, a.xb + off, a.sgb + off, a.xt + off, a.sgt + off, W, H,
+                     nimg, g, a.Lp, a.p0, nch * a.K, a.L - t + 1, st)))
+    return e;
+  hipLaunchKernelGGL(k_xb2, ew_grid(n), dim3(256), 0, st, a, t, nch);
+  if (t >= 2) {
+    if ((e = xlse_conv(a.xa + off, a.sga + off, a.xb + off, a.sgb + off, a.xt + off, a.sgt + off, W,
+                       H, nimg, g, a.Lp, a.p0, nch * a.K, a.L - t + 2, st)))
+      return e;
+    hipLaunchKernelGGL(k_xb3, ew_grid(n), dim3(256), 0, st, a, t, nch);
+  }
+  WDL_HIP(hipGetLastError());
+  return WDL_OK;
+}
+
+#undef XLOOP
+
+}  // namespace wdl
