@@ -285,7 +285,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-form timing")
     ap.add_argument("--dense", action="store_true", help="do not skip all-zero factor chunks")
     ap.add_argument("--streams", type=int, default=0,
-                    help="histogram groups on separate streams (0: the library default)")
+                    help="histogram groups on separate streams (0: autotuned)")
     ap.add_argument("--no-projection", action="store_true", help="skip the projection leg")
     ap.add_argument("--no-log", action="store_true", help="skip the 256x256 log-domain leg")
     ap.add_argument("--no-served", action="store_true", help="skip the served projection leg")
@@ -313,6 +313,10 @@ def main():
     s.set_problem(c["atoms"], c["weights"])
     if args.streams:
         s.set_streams(args.streams)
+        nstreams = args.streams
+    else:
+        nstreams = s.autotune(3)
+        log("autotune: %d histogram groups (streams)" % nstreams)
     stream = ctx.stream
 
     for _ in range(args.warmup):
@@ -424,7 +428,8 @@ def main():
                 "uniform random atoms and weights, one seed",
         "config": {"workload": "batched standard domain: compute_gradient_both for all P "
                                "histograms per step", "n": n, "P": P, "K": K, "L": L,
-                   "gamma": g, "loss": "quadratic", "band_skip": not args.dense},
+                   "gamma": g, "loss": "quadratic", "band_skip": not args.dense,
+                   "streams": nstreams},
         "latency_ms": {"p50": round(float(np.percentile(step_ms, 50)), 4),
                        "p99": round(float(np.percentile(step_ms, 99)), 4),
                        "what": "per-step (whole batch) latency"},

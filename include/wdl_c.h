@@ -109,6 +109,11 @@ int wdl_solver_warm_commit(wdl_solver* s);
  * on its own stream (default min(4, P/16), at least 1).  No counterpart in
  * the reference; a tuning knob of the batched launch. */
 int wdl_solver_set_streams(wdl_solver* s, int nstreams);
+/* Times `reps` batched runs for each histogram-group count 1, 2, 4, 8 (at most P)
+ * and keeps the fastest; writes it to *nstreams (may be NULL).  Runs the solve,
+ * so results of earlier runs are overwritten (the warm-restart store is not
+ * committed).  No counterpart in the reference; the batched launch's tuner. */
+int wdl_solver_autotune(wdl_solver* s, int reps, int* nstreams);
 /* Per-histogram Sinkhorn iterations (0..L each; NULL = L for all): histogram p
  * runs iterations[p] forward steps and as many backward steps (the reference
  * has one Niters for all, inverseWasserstein.h:546; uneven counts serve

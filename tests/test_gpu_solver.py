@@ -226,3 +226,26 @@ def test_uneven_iterations_and_weights_only(ctx, flags):
         assert rel(full["bary"][p], o["bary"]) < TOL, p
         b_last = o["b_last"]
         assert rel(full["b_temp"][p], b_last) < TOL, p
+
+
+def test_autotune_keeps_results(ctx):
+    # The tuner only regroups histograms over streams: results stay bit-identical.
+    W = H = 32
+    K, P, L, g = 3, 20, 8, 1.0
+    print("autotune P=%d" % P, flush=True)
+    rng = np.random.default_rng(9)
+    obs = synthetic.blob_histograms(rng, W, P, (1, 3), (2, 6))
+    atoms, lam = synthetic.init_atoms_weights(rng, K, P, W * H)
+    s = wdl.SinkhornSolver(ctx, W, H, K, P, L, g)
+    s.set_observed(obs)
+    s.set_problem(atoms, lam)
+    s.set_streams(1)
+    s.run()
+    ref = s.results()
+    n = s.autotune(2)
+    assert n in (1, 2, 4, 8)
+    s.run()
+    r = s.results()
+    s.close()
+    for k in ("grad_lambda", "grad_dic_sum", "bary", "loss"):
+        np.testing.assert_array_equal(r[k], ref[k], err_msg=k)

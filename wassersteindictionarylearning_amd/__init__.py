@@ -61,6 +61,7 @@ _SIGS = {
     "wdl_solver_run": (_i, [_vp, _i]),
     "wdl_solver_run_stage": (_i, [_vp, _i, _i]),
     "wdl_solver_set_streams": (_i, [_vp, _i]),
+    "wdl_solver_autotune": (_i, [_vp, _i, ctypes.POINTER(_i)]),
     "wdl_solver_set_iterations": (_i, [_vp, ctypes.POINTER(_i)]),
     "wdl_solver_get_results": (_i, [_vp, _d, _d, _d, _d]),
     "wdl_solver_get_grad_dic": (_i, [_vp, _d]),
@@ -276,6 +277,12 @@ class SinkhornSolver:
 
     def set_streams(self, nstreams: int):
         _check(lib().wdl_solver_set_streams(self.handle, nstreams), "wdl_solver_set_streams")
+
+    def autotune(self, reps: int = 3) -> int:
+        """Picks the histogram-group (stream) count by timing; returns it."""
+        n = ctypes.c_int()
+        _check(lib().wdl_solver_autotune(self.handle, reps, ctypes.byref(n)), "wdl_solver_autotune")
+        return n.value
 
     def run_stage(self, stage: int, step: int):
         _check(lib().wdl_solver_run_stage(self.handle, stage, step), "wdl_solver_run_stage")

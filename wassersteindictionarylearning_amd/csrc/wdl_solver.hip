@@ -1300,6 +1300,43 @@ int wdl_solver_get_grad_dic(wdl_solver* s, double* grad_dic) {
   return WDL_OK;
 }
 
+int wdl_solver_autotune(wdl_solver* s, int reps, int* nstreams) {
+  WDL_CHECK(s, WDL_ERR_ARG, "wdl_solver_autotune: NULL argument");
+  WDL_CHECK(reps >= 1, WDL_ERR_ARG, "wdl_solver_autotune: reps must be >= 1");
+  WDL_CHECK(s->observed && s->problem, WDL_ERR_STATE,
+            "wdl_solver_autotune: set_observed and set_problem must be called first");
+  WDL_HIP(hipSetDevice(s->ctx->device));
+  hipStream_t st = s->ctx->stream;
+  hipEvent_t e0, e1;
+  WDL_HIP(hipEventCreate(&e0));
+  WDL_HIP(hipEventCreate(&e1));
+  int best = s->ngroups, rc = WDL_OK;
+  float bestMs = 0.f;
+  // candidate histogram-group counts: each group advances on its own stream
+  for (int n = 1; n <= 8 && n <= s->P && rc == WDL_OK; n *= 2) {
+    if ((rc = s->set_groups(n))) break;
+    if ((rc = wdl_solver_run(s, WDL_RUN_BATCHED))) break;  // graph capture and warm-up
+    float ms = 0.f;
+    if (hipEventRecord(e0, st) != hipSuccess) rc = WDL_ERR_HIP;
+    for (int r = 0; r < reps && rc == WDL_OK; ++r) rc = wdl_solver_run(s, WDL_RUN_BATCHED);
+    if (rc == WDL_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                         hipEventElapsedTime(&ms, e0, e1) != hipSuccess)) {
+      set_error("wdl_solver_autotune: event timing failed");
+      rc = WDL_ERR_HIP;
+    }
+    if (rc == WDL_OK && (bestMs == 0.f || ms < bestMs)) {
+      bestMs = ms;
+      best = n;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  if ((rc = s->set_groups(best))) return rc;
+  if (nstreams) *nstreams = best;
+  return WDL_OK;
+}
+
 int wdl_solver_set_slot(wdl_solver* s, int slot, const double* observed, const double* lambdas,
                         int iterations, double step) {
   WDL_CHECK(s, WDL_ERR_ARG, "wdl_solver_set_slot: NULL argument");
