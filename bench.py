@@ -243,10 +243,11 @@ def cpu_baseline(c, gpu, budget_s=15.0):
     a bounded sample of the same workload, and the GPU error against it."""
     from oracle import ref
     if not ref.available():
-        return None, None
+        return None, None, []
     n, K, L, g, loss = c["n"], c["K"], c["L"], c["gamma"], c["loss"]
     out = {"cores": ref.num_threads(), "kind": "reference"}
     errs = {"grad_lambda": [], "grad_dic": [], "bary": []}
+    refs = []
     t_sse, done = 0.0, 0
     for p in range(c["P"]):
         t0 = time.perf_counter()
@@ -254,6 +255,7 @@ def cpu_baseline(c, gpu, budget_s=15.0):
                               variant="sse")
         t_sse += time.perf_counter() - t0
         done += 1
+        refs.append((p, r))
         for k, gk in (("grad_lambda", gpu["grad_lambda"][p]), ("grad_dic", gpu["grad_dic"][p]),
                       ("bary", gpu["bary"][p])):
             errs[k].append(float(np.linalg.norm(gk.ravel() - r[k].ravel())
@@ -273,7 +275,67 @@ def cpu_baseline(c, gpu, budget_s=15.0):
     err = {k: max(v) for k, v in errs.items()}
     err["histograms_checked"] = done
     err["against"] = "reference compute_gradient_both (FP64, CPU)"
-    return out, err
+    return out, err, refs
+
+
+def errors_against(refs, res):
+    e = {"grad_lambda": 0.0, "grad_dic": 0.0, "bary": 0.0}
+    for p, r in refs:
+        for k in e:
+            e[k] = max(e[k], float(np.linalg.norm(res[k][p].ravel() - r[k].ravel())
+                                   / np.linalg.norm(r[k].ravel())))
+    return e
+
+
+def bf16x3_leg(ctx, c, steps):
+    """The same batched workload with the convolution passes in split-BF16x3,
+    and the bare convolution in both precisions."""
+    n, P, K, L, g, loss = c["n"], c["P"], c["K"], c["L"], c["gamma"], c["loss"]
+    s = wdl.SinkhornSolver(ctx, n, n, K, P, L, g, loss, wdl.FLAG_BF16X3)
+    s.set_observed(c["obs"])
+    s.set_problem(c["atoms"], c["weights"])
+    s.autotune(2)
+    stream = ctx.stream
+    e0, e1 = _hip.Event(), _hip.Event()
+    s.run()
+    e0.record(stream)
+    for _ in range(steps):
+        s.run()
+    e1.record(stream)
+    e1.synchronize()
+    ms = e0.elapsed_ms(e1) / steps
+    res = s.results()
+    res["grad_dic"] = s.grad_dic()
+    s.run_stage(wdl.STAGE_BACKWARD, L // 2)
+    e0.record(stream)
+    for _ in range(20):
+        s.run_stage(wdl.STAGE_BACKWARD, L // 2)
+    e1.record(stream)
+    e1.synchronize()
+    bwd_ms = e0.elapsed_ms(e1) / 20
+    s.close()
+    nimg = 4 * P * K
+    dbuf_in = _hip.DeviceArray((nimg, n, n))
+    dbuf_out = _hip.DeviceArray((nimg, n, n))
+    dbuf_in.upload(np.random.default_rng(0).uniform(0, 1, (nimg, n, n)))
+    conv = {}
+    for name, f in (("fp32", 0), ("bf16x3", wdl.FLAG_BF16X3)):
+        kern = wdl.HipGaussian2DKernel(ctx, g, n, n, f)
+        kern.convolve_device(dbuf_in.ptr, dbuf_out.ptr, nimg)
+        e0.record(stream)
+        for _ in range(20):
+            kern.convolve_device(dbuf_in.ptr, dbuf_out.ptr, nimg)
+        e1.record(stream)
+        e1.synchronize()
+        conv[name + "_launch_ms"] = round(e0.elapsed_ms(e1) / 20, 4)
+        kern.close()
+    dbuf_in.free()
+    dbuf_out.free()
+    e0.destroy()
+    e1.destroy()
+    return {"ms_per_step": round(ms, 4), "value": round(P * L / (ms / 1e3), 1),
+            "unit": "Sinkhorn-iters/s", "backward_launch_ms": round(bwd_ms, 4),
+            "conv_only": conv}, res
 
 
 def main():
@@ -289,6 +351,7 @@ def main():
     ap.add_argument("--no-projection", action="store_true", help="skip the projection leg")
     ap.add_argument("--no-log", action="store_true", help="skip the 256x256 log-domain leg")
     ap.add_argument("--no-served", action="store_true", help="skip the served projection leg")
+    ap.add_argument("--no-bf16x3", action="store_true", help="skip the split-BF16x3 leg")
     ap.add_argument("--projection-requests", type=int, default=512,
                     help="how many of the 4096 projection requests to run")
     ap.add_argument("--served-seconds", type=float, default=20.0,
@@ -438,7 +501,7 @@ def main():
         "batched_over_plain": (round(value / plain["value"], 2) if plain else None),
         "roofline": roofline,
     }
-    if not args.no_projection:
+    if not args.no_projection and args.projection_requests > 0:
         line["projection"] = projection_leg(ctx, args.projection_requests)
         log("projection: %s iters/s" % line["projection"]["value"])
     if not args.no_log:
@@ -447,10 +510,16 @@ def main():
     if not args.no_served:
         line["served"] = served_leg(args.served_seconds)
         log("served: %s" % json.dumps(line["served"]["latency_ms"]))
+    bres = None
+    if not args.no_bf16x3:
+        line["bf16x3"], bres = bf16x3_leg(ctx, c, args.steps)
+        log("bf16x3: %s iters/s" % line["bf16x3"]["value"])
     if not args.no_cpu:
-        cb, err = cpu_baseline(c, res)
+        cb, err, refs = cpu_baseline(c, res)
         line["cpu_baseline"] = cb
         line["err_vs_fp64"] = err
+        if bres is not None and refs:
+            line["bf16x3"]["err_vs_fp64"] = errors_against(refs, bres)
     if args.copy_calib:
         # A plain copy of known size: FETCH_SIZE / WRITE_SIZE of this copy kernel
         # scale the counters read for the solver kernels to bytes.

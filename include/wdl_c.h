@@ -42,6 +42,8 @@ extern "C" {
 #define WDL_FLAG_TILED 32       /* 64x64-tile kernels at any size (taken anyway above 128 per side) */
 #define WDL_FLAG_EXACT_LSE 64   /* log domain: exact per-output maximum (kernels.h:1082-1091) for
                                    any spread of log values; FP32 sums pruned at e^-40 */
+#define WDL_FLAG_BF16X3 128     /* standard domain, sides <= 128: the convolution's MFMA passes in
+                                   split-BF16x3 (hi.hi + hi.lo + lo.hi) instead of FP32 */
 
 /* Run modes of wdl_solver_run. */
 #define WDL_RUN_BATCHED 0 /* all P solves advance together, one launch per Sinkhorn stage */

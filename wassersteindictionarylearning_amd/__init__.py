@@ -23,6 +23,7 @@ LOSS_TV, LOSS_QUADRATIC, LOSS_WASSERSTEIN, LOSS_KL = 1, 2, 3, 4
 FLAG_LOG_DOMAIN, FLAG_WARM_RESTART, FLAG_DENSE, FLAG_NO_GRAPH, FLAG_WEIGHTS_ONLY = 1, 2, 4, 8, 16
 FLAG_TILED = 32
 FLAG_EXACT_LSE = 64
+FLAG_BF16X3 = 128
 RUN_BATCHED, RUN_PLAIN = 0, 1
 STAGE_FORWARD, STAGE_BACKWARD = 0, 1
 

@@ -48,12 +48,12 @@ int build_table(int W, int H, double gamma, std::vector<float>& tab) {
 
 static int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
-ConvGeom make_geom(int W, int H, int D) {
+ConvGeom make_geom(int W, int H, int D, int pad) {
   ConvGeom g;
   g.W = W;
   g.H = H;
-  g.Wp = round_up(W, 16);
-  g.Hp = round_up(H, 16);
+  g.Wp = round_up(W, pad);
+  g.Hp = round_up(H, pad);
   // Row strides congruent to 4 (mod 64 floats): the 16 lanes of a quarter-wave
   // reading 16 consecutive rows at one column then hit distinct LDS banks.
   g.S1 = round_up(g.Wp, 64) + 4;

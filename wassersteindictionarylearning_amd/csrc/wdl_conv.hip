@@ -11,19 +11,19 @@ namespace wdl {
 // One workgroup per image (grid-stride over images).  T is FP32 (device form)
 // or FP64 (host drop-in: values are rounded to FP32 for the MFMA passes and
 // widened back on store).
-template <class T>
+template <class T, bool BF3>
 __global__ __launch_bounds__(kThreads, 4) void k_conv(const T* __restrict__ in,
                                                       T* __restrict__ out, int nvec, ConvGeom G,
                                                       const float* __restrict__ gtab) {
   extern __shared__ __align__(16) float smem[];
-  float* tabs = smem;
-  float* buf = smem + 4 * kTabLen;
-  load_tables(tabs, gtab);
+  void* tabs = smem;
+  float* buf = smem + ConvPrec<BF3>::kTabBytes / 4;
+  ConvPrec<BF3>::load(tabs, gtab);
   const size_t N = (size_t)G.W * G.H;
   for (int v = blockIdx.x; v < nvec; v += gridDim.x) {
     const T* src = in + (size_t)v * N;
     for_each_px(G, [&](int li, int gi, bool ok) { buf[li] = ok ? (float)src[gi] : 0.f; });
-    conv_lds(buf, tabs, G);
+    ConvPrec<BF3>::conv(buf, tabs, G);
     T* dst = out + (size_t)v * N;
     for_each_px(G, [&](int li, int gi, bool ok) {
       if (ok) dst[gi] = (T)buf[li];
@@ -33,20 +33,21 @@ __global__ __launch_bounds__(kThreads, 4) void k_conv(const T* __restrict__ in,
 }
 
 // FP32 form with float4 loads and stores (rows of whole float4 groups).
+template <bool BF3>
 __global__ __launch_bounds__(kThreads, 4) void k_conv4(const float* __restrict__ in,
                                                        float* __restrict__ out, int nvec,
                                                        ConvGeom G, const float* __restrict__ gtab) {
   extern __shared__ __align__(16) float smem[];
-  float* tabs = smem;
-  float* buf = smem + 4 * kTabLen;
-  load_tables(tabs, gtab);
+  void* tabs = smem;
+  float* buf = smem + ConvPrec<BF3>::kTabBytes / 4;
+  ConvPrec<BF3>::load(tabs, gtab);
   const size_t N = (size_t)G.W * G.H;
   for (int v = blockIdx.x; v < nvec; v += gridDim.x) {
     const float* src = in + (size_t)v * N;
     px_pass<4>(
         G, [&](int gi) { return vld<4>(src + gi); },
         [&](int li, int gi, bool ok, floatx4 x) { vst<4>(buf + li, ok ? x : vsplat<4>(0.f)); });
-    conv_lds(buf, tabs, G);
+    ConvPrec<BF3>::conv(buf, tabs, G);
     float* dst = out + (size_t)v * N;
     px_pass<4>(
         G, [&](int gi) { return 0; },
@@ -126,23 +127,36 @@ template <class T>
 static int launch_conv(const wdl_gauss2d* k, const T* in, T* out, int nvec, hipStream_t s) {
   if (nvec <= 0) return WDL_OK;
   if (k->tiled) return launch_conv_tiled<T>(k, in, out, nvec, s);
+  const bool bf3 = k->flags & WDL_FLAG_BF16X3;
   const ConvGeom G = k->geom(!(k->flags & WDL_FLAG_DENSE));
-  const int lds = conv_lds_bytes(G);
   static bool attr_set = false;
   if (!attr_set) {
-    const int bytes = conv_lds_bytes(make_geom(kMaxSide, kMaxSide, 0));
-    WDL_HIP(hipFuncSetAttribute((const void*)k_conv<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                bytes));
-    WDL_HIP(hipFuncSetAttribute((const void*)k_conv4, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                bytes));
+    const int img = conv_lds_bytes(make_geom(kMaxSide, kMaxSide, 0, 32)) - 4 * kTabLen * 4;
+    const void* f32[] = {(const void*)k_conv<T, false>, (const void*)k_conv4<false>};
+    const void* fbf[] = {(const void*)k_conv<T, true>, (const void*)k_conv4<true>};
+    for (const void* f : f32)
+      WDL_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  img + ConvPrec<false>::kTabBytes));
+    for (const void* f : fbf)
+      WDL_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  img + ConvPrec<true>::kTabBytes));
     attr_set = true;
   }
+  const int lds = conv_lds_bytes(G) - 4 * kTabLen * 4 +
+                  (bf3 ? ConvPrec<true>::kTabBytes : ConvPrec<false>::kTabBytes);
   const int grid = nvec;
-  if (sizeof(T) == sizeof(float) && G.W % 4 == 0)
-    hipLaunchKernelGGL(k_conv4, dim3(grid), dim3(kThreads), lds, s, (const float*)in, (float*)out,
-                       nvec, G, k->tab.as<float>());
+  const bool v4 = sizeof(T) == sizeof(float) && G.W % 4 == 0;
+  if (v4 && bf3)
+    hipLaunchKernelGGL(k_conv4<true>, dim3(grid), dim3(kThreads), lds, s, (const float*)in,
+                       (float*)out, nvec, G, k->tab.as<float>());
+  else if (v4)
+    hipLaunchKernelGGL(k_conv4<false>, dim3(grid), dim3(kThreads), lds, s, (const float*)in,
+                       (float*)out, nvec, G, k->tab.as<float>());
+  else if (bf3)
+    hipLaunchKernelGGL((k_conv<T, true>), dim3(grid), dim3(kThreads), lds, s, in, out, nvec, G,
+                       k->tab.as<float>());
   else
-    hipLaunchKernelGGL(k_conv<T>, dim3(grid), dim3(kThreads), lds, s, in, out, nvec, G,
+    hipLaunchKernelGGL((k_conv<T, false>), dim3(grid), dim3(kThreads), lds, s, in, out, nvec, G,
                        k->tab.as<float>());
   WDL_HIP(hipGetLastError());
   return WDL_OK;
@@ -160,7 +174,7 @@ int launch_conv_f64io(const wdl_gauss2d* k, const double* in, double* out, int n
 }  // namespace wdl
 
 wdl::ConvGeom wdl_gauss2d::geom(bool band) const {
-  return wdl::make_geom(W, H, band ? D : (1 << 20));
+  return wdl::make_geom(W, H, band ? D : (1 << 20), (flags & WDL_FLAG_BF16X3) ? 32 : 16);
 }
 
 extern "C" {
@@ -171,8 +185,8 @@ int wdl_gauss2d_create(wdl_ctx* ctx, int W, int H, double gamma, int flags, wdl_
   WDL_CHECK(gamma > 0 && std::isfinite(gamma), WDL_ERR_ARG,
             "wdl_gauss2d_create: gamma must be positive and finite");
   const bool tiled = wdl::needs_tiles(W, H);
-  WDL_CHECK(!(tiled && (flags & WDL_FLAG_DENSE)), WDL_ERR_UNSUPPORTED,
-            "wdl_gauss2d_create: WDL_FLAG_DENSE needs sides of at most " +
+  WDL_CHECK(!(tiled && (flags & (WDL_FLAG_DENSE | WDL_FLAG_BF16X3))), WDL_ERR_UNSUPPORTED,
+            "wdl_gauss2d_create: WDL_FLAG_DENSE and WDL_FLAG_BF16X3 need sides of at most " +
                 std::to_string(wdl::kMaxSide));
   std::vector<float> tab;
   const int D = wdl::build_table(W, H, gamma, tab);

@@ -135,6 +135,88 @@ __device__ inline void store_transposed(float* dst, int sdst, int nrows, int nco
   }
 }
 
+// ---- split-BF16x3 variant -------------------------------------------------------
+// Each FP32 operand x is split into bf16 hi = RN(x) and lo = RN(x - hi) (x - hi
+// - lo is within 2^-17 |x|), and a product is formed from three bf16 MFMAs,
+// hi.hi + hi.lo + lo.hi, accumulated in FP32 (v_mfma_f32_16x16x32_bf16: 16x16
+// outputs over 32 contraction indices; lane l holds A[l%16][8(l/16)..+7] and
+// B[8(l/16)..+7][l%16]).  The dropped lo.lo term and the split leave ~2^-16
+// relative error per product -- between FP32 and BF16, at a third of the
+// matrix-core time per FLOP of the FP32 MFMA.  Contraction lengths are
+// multiples of 32 (images padded to 32, make_geom).
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+constexpr int kTabBfCopies = 8;  // shifted copies: 8 consecutive bf16 = one 16-byte read
+constexpr int kTabBfBytes = 2 * kTabBfCopies * kTabLen * 2;  // hi and lo copies
+
+// copy s of hi (lo) holds the split of tab[x + s] at tb[s*kTabLen + x]
+// (tb[(8 + s)*kTabLen + x]).
+__device__ inline void load_tables_bf3(__bf16* tb, const float* __restrict__ gtab) {
+  for (int e = threadIdx.x; e < kTabBfCopies * kTabLen; e += kThreads) {
+    const int s = e / kTabLen, x = e - s * kTabLen;
+    const float v = (x + s < kTabLen) ? gtab[x + s] : 0.f;
+    const __bf16 h = (__bf16)v;
+    tb[e] = h;
+    tb[kTabBfCopies * kTabLen + e] = (__bf16)(v - (float)h);
+  }
+}
+
+__device__ inline floatx4 mfma_bf16(bf16x8 a, bf16x8 b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// rows_gemm with the split-BF16x3 products; same output layout in acc.
+__device__ inline void rows_gemm_bf3(const float* __restrict__ in, int sin, int nrows,
+                                     int ncontract, int nout, int off,
+                                     const __bf16* __restrict__ tb, int D,
+                                     floatx4 (&acc)[kMaxUnits][4]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int c = lane & 15, g = lane >> 4;
+  const int ncb = nout >> 4;
+  const int nck = ncontract >> 5;
+  const int nunits = ncb * ((nrows + 63) >> 6);
+#pragma unroll
+  for (int uu = 0; uu < kMaxUnits; ++uu) {
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) acc[uu][rb] = floatx4{0.f, 0.f, 0.f, 0.f};
+    const int u = wave + uu * kWaves;
+    if (u < nunits) {
+      const int cb = u % ncb, rg = u / ncb;
+      const int c0 = (cb << 4) + off, r0 = rg << 6;
+      const int nrb = min(4, (nrows - r0) >> 4);
+      const int lo = c0 - D;
+      const int kc_lo = lo > 0 ? (lo >> 5) : 0;
+      const int kc_hi = min(nck - 1, (c0 + 15 + D) >> 5);
+      const float* arow = in + (r0 + c) * sin + 8 * g;
+      for (int kc = kc_lo; kc <= kc_hi; ++kc) {
+        const int kk = kc << 5;
+        // B[k][j] = Kg(kk + 8g + t, c0 + c) = tab[kTabOff + kk + 8g + t - c0 - c], t = 0..7
+        const int a = kTabOff + kk + 8 * g - c0 - c;
+        const int sh = (a & 7) * kTabLen + (a & ~7);
+        const bf16x8 bh = *reinterpret_cast<const bf16x8*>(tb + sh);
+        const bf16x8 bl = *reinterpret_cast<const bf16x8*>(tb + kTabBfCopies * kTabLen + sh);
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) {
+          if (rb >= nrb) continue;
+          const float* ap = arow + rb * 16 * sin + kk;
+          const floatx4 x0 = *reinterpret_cast<const floatx4*>(ap);
+          const floatx4 x1 = *reinterpret_cast<const floatx4*>(ap + 4);
+          bf16x8 ah, al;
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            ah[t] = (__bf16)x0[t];
+            al[t] = (__bf16)(x0[t] - (float)ah[t]);
+            ah[t + 4] = (__bf16)x1[t];
+            al[t + 4] = (__bf16)(x1[t] - (float)ah[t + 4]);
+          }
+          acc[uu][rb] = mfma_bf16(al, bh, acc[uu][rb]);
+          acc[uu][rb] = mfma_bf16(ah, bl, acc[uu][rb]);
+          acc[uu][rb] = mfma_bf16(ah, bh, acc[uu][rb]);
+        }
+      }
+    }
+  }
+}
+
 // buf holds U (Hp x Wp, stride S1, zero outside H x W) on entry and
 // R = K_H . U . K_W (stride S1; values outside H x W are don't-care) on exit.
 // All threads of the workgroup must call it; it synchronises internally.
@@ -150,6 +232,38 @@ __device__ inline void conv_lds(float* buf, const float* tabs, const ConvGeom& G
   store_transposed(buf, G.S1, G.Wp, G.Hp, acc);        // buf = R (Hp x Wp)
   __syncthreads();
 }
+
+// conv_lds with the split-BF16x3 passes (Wp, Hp multiples of 32).
+__device__ inline void conv_lds_bf3(float* buf, const __bf16* tb, const ConvGeom& G) {
+  floatx4 acc[kMaxUnits][4];
+  __syncthreads();
+  rows_gemm_bf3(buf, G.S1, G.Hp, G.Wp, G.Wp, 0, tb, G.D, acc);
+  __syncthreads();
+  store_transposed(buf, G.S2, G.Hp, G.Wp, acc);
+  __syncthreads();
+  rows_gemm_bf3(buf, G.S2, G.Wp, G.Hp, G.Hp, 0, tb, G.D, acc);
+  __syncthreads();
+  store_transposed(buf, G.S1, G.Wp, G.Hp, acc);
+  __syncthreads();
+}
+
+// The table region and convolution of one precision variant: BF3 = false is
+// FP32 MFMA (4 shifted FP32 copies), true the split-BF16x3 form.
+template <bool BF3> struct ConvPrec;
+template <> struct ConvPrec<false> {
+  static constexpr int kTabBytes = 4 * kTabLen * (int)sizeof(float);
+  __device__ static void load(void* t, const float* g) { load_tables((float*)t, g); }
+  __device__ static void conv(float* buf, const void* t, const ConvGeom& G) {
+    conv_lds(buf, (const float*)t, G);
+  }
+};
+template <> struct ConvPrec<true> {
+  static constexpr int kTabBytes = kTabBfBytes;
+  __device__ static void load(void* t, const float* g) { load_tables_bf3((__bf16*)t, g); }
+  __device__ static void conv(float* buf, const void* t, const ConvGeom& G) {
+    conv_lds_bf3(buf, (const __bf16*)t, G);
+  }
+};
 
 // Visits every padded position of the LDS image; one wave per row, so global
 // accesses of a wave are 64 consecutive floats.  f(lds_index, pixel, valid).

@@ -48,7 +48,9 @@ struct DevBuf {
 // |d| whose FP32 factor is non-zero.
 int build_table(int W, int H, double gamma, std::vector<float>& tab);
 
-ConvGeom make_geom(int W, int H, int D);
+// pad: padded sides are multiples of it (16 for the FP32 passes, 32 for the
+// split-BF16x3 passes, whose contraction chunks are 32 wide).
+ConvGeom make_geom(int W, int H, int D, int pad = 16);
 
 // Geometry of the tiled convolution for band half-width D; the log domain
 // (logdom) widens the halo until the factors beyond it are below e^-264.

@@ -37,6 +37,7 @@ struct SolveArgs {
   float* bSave;
   const int* Lp;      // per-histogram Sinkhorn iterations (<= L), or null for L
   int weightsOnly;    // skip the dictionary-gradient chain
+  int bf3;            // split-BF16x3 convolution passes (standard domain)
   const float* logA;  // K x N log atoms (log-domain solver)
   unsigned* events;   // lines whose spread exceeded kLogRange (log-domain solver)
   // Tiled solver (a side above kMaxSide): geometry, the inputs of the second

@@ -29,15 +29,15 @@ namespace wdl {
 
 // Forward iteration s (1..L): b^{s-1} (geometric-mean step of iteration s-1,
 // :577-592), c^{s-1} = K b^{s-1} (:568), a = p / c (:569-574), phi^s = K a (:575).
-template <int VW>
+template <int VW, bool BF3>
 __global__ __launch_bounds__(kThreads, 4) void k_forward(SolveArgs a, int s) {
   typedef typename Vec<VW>::T V;
   int p, i;
   if (!map_block(a, p, i) || s > steps_of(a, p)) return;
   extern __shared__ __align__(16) float smem[];
-  float* tabs = smem;
-  float* buf = smem + 4 * kTabLen;
-  load_tables(tabs, a.tab);
+  void* tabs = smem;
+  float* buf = smem + ConvPrec<BF3>::kTabBytes / 4;
+  ConvPrec<BF3>::load(tabs, a.tab);
   const ConvGeom G = a.G;
   const long long N = a.N;
   const int K = a.K;
@@ -76,7 +76,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_forward(SolveArgs a, int s) {
           vst<VW>(buf + li, b);
         });
   }
-  conv_lds(buf, tabs, G);
+  ConvPrec<BF3>::conv(buf, tabs, G);
   {
     float* Cd = a.Ch + (long long)(s - 1) * a.stage + img;
     const float* Ai = a.A + (long long)i * N;
@@ -92,7 +92,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_forward(SolveArgs a, int s) {
           vst<VW>(buf + li, u);
         });
   }
-  conv_lds(buf, tabs, G);
+  ConvPrec<BF3>::conv(buf, tabs, G);
   float* Fd = a.Fh + (long long)(s - 1) * a.stage + img;
   px_pass<VW>(
       G, [&](int gi) { return 0; },
@@ -145,7 +145,7 @@ __global__ __launch_bounds__(kThreads) void k_final(SolveArgs a) {
 
 // Backward step s (L..1).  chain 0: dictionary gradient (:600-629);
 // chain 1: weight gradient (:635-673).
-template <int VW>
+template <int VW, bool BF3>
 __global__ __launch_bounds__(kThreads, 4) void k_backward(SolveArgs a, int t) {
   typedef typename Vec<VW>::T V;
   int p, i, chain;
@@ -153,9 +153,9 @@ __global__ __launch_bounds__(kThreads, 4) void k_backward(SolveArgs a, int t) {
   const int s = bwd_step(a, p, t);
   if (s < 1) return;
   extern __shared__ __align__(16) float smem[];
-  float* tabs = smem;
-  float* buf = smem + 4 * kTabLen;
-  load_tables(tabs, a.tab);
+  void* tabs = smem;
+  float* buf = smem + ConvPrec<BF3>::kTabBytes / 4;
+  ConvPrec<BF3>::load(tabs, a.tab);
   const ConvGeom G = a.G;
   const long long N = a.N;
   const int K = a.K;
@@ -195,7 +195,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_backward(SolveArgs a, int t) {
         [&](int li, int gi, bool ok, const Three& r) {
           vst<VW>(buf + li, ok ? (lpi * r.x - r.y) * r.z : vsplat<VW>(0.f));
         });
-    conv_lds(buf, tabs, G);
+    ConvPrec<BF3>::conv(buf, tabs, G);
     float* GDp = a.GD + img;
     px_pass<VW>(
         G,
@@ -216,7 +216,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_backward(SolveArgs a, int t) {
           vst<VW>(buf + li, t);
         });
     if (s == 1) return;
-    conv_lds(buf, tabs, G);
+    ConvPrec<BF3>::conv(buf, tabs, G);
     const float* Fm = a.Fh + (long long)(s - 2) * a.stage + img;  // phi^{s-1}
     px_pass<VW>(
         G, [&](int gi) { return vld<VW>(Fm + gi); },
@@ -266,7 +266,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_backward(SolveArgs a, int t) {
       *gl = last ? t : *gl + (t - tf / (double)N * (tg - exact_gsum(a, p, s)));
     }
     if (s == 1) return;
-    conv_lds(buf, tabs, G);
+    ConvPrec<BF3>::conv(buf, tabs, G);
     px_pass<VW>(
         G,
         [&](int gi) {
@@ -278,7 +278,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_backward(SolveArgs a, int t) {
         [&](int li, int gi, bool ok, const Two& r) {
           vst<VW>(buf + li, ok ? vld<VW>(buf + li) * (-r.y / (r.x * r.x)) : vsplat<VW>(0.f));
         });
-    conv_lds(buf, tabs, G);
+    ConvPrec<BF3>::conv(buf, tabs, G);
     const float* Qm = a.Qh + (long long)(s - 2) * a.qstage + (long long)p * N;  // q^{s-1}
     const float* Fm = a.Fh + (long long)(s - 2) * a.stage + img;                // phi^{s-1}
     px_pass<VW>(
@@ -645,10 +645,17 @@ static void launch_forward(const SolveArgs& a, int grid, int lds, hipStream_t st
       hipLaunchKernelGGL(k_forward_log<1>, dim3(grid), dim3(kThreads), lds, st, a, s);
     return;
   }
-  if (a.G.W % 4 == 0)
-    hipLaunchKernelGGL(k_forward<4>, dim3(grid), dim3(kThreads), lds, st, a, s);
-  else
-    hipLaunchKernelGGL(k_forward<1>, dim3(grid), dim3(kThreads), lds, st, a, s);
+  const bool v4 = a.G.W % 4 == 0;
+  if (a.bf3) {
+    if (v4)
+      hipLaunchKernelGGL((k_forward<4, true>), dim3(grid), dim3(kThreads), lds, st, a, s);
+    else
+      hipLaunchKernelGGL((k_forward<1, true>), dim3(grid), dim3(kThreads), lds, st, a, s);
+  } else if (v4) {
+    hipLaunchKernelGGL((k_forward<4, false>), dim3(grid), dim3(kThreads), lds, st, a, s);
+  } else {
+    hipLaunchKernelGGL((k_forward<1, false>), dim3(grid), dim3(kThreads), lds, st, a, s);
+  }
 }
 
 static void launch_backward(const SolveArgs& a, int grid1, int lds, hipStream_t st, int s) {
@@ -660,10 +667,17 @@ static void launch_backward(const SolveArgs& a, int grid1, int lds, hipStream_t 
       hipLaunchKernelGGL(k_backward_log<1>, dim3(blocks), dim3(kThreads), lds, st, a, s);
     return;
   }
-  if (a.G.W % 4 == 0)
-    hipLaunchKernelGGL(k_backward<4>, dim3(blocks), dim3(kThreads), lds, st, a, s);
-  else
-    hipLaunchKernelGGL(k_backward<1>, dim3(blocks), dim3(kThreads), lds, st, a, s);
+  const bool v4 = a.G.W % 4 == 0;
+  if (a.bf3) {
+    if (v4)
+      hipLaunchKernelGGL((k_backward<4, true>), dim3(blocks), dim3(kThreads), lds, st, a, s);
+    else
+      hipLaunchKernelGGL((k_backward<1, true>), dim3(blocks), dim3(kThreads), lds, st, a, s);
+  } else if (v4) {
+    hipLaunchKernelGGL((k_backward<4, false>), dim3(blocks), dim3(kThreads), lds, st, a, s);
+  } else {
+    hipLaunchKernelGGL((k_backward<1, false>), dim3(blocks), dim3(kThreads), lds, st, a, s);
+  }
 }
 
 }  // namespace wdl
@@ -691,7 +705,9 @@ struct wdl_solver {
 
   SolveArgs args(int p0, int np) const {
     SolveArgs a;
-    a.G = make_geom(W, H, (flags & WDL_FLAG_DENSE) ? (1 << 20) : D);
+    a.G = make_geom(W, H, (flags & WDL_FLAG_DENSE) ? (1 << 20) : D,
+                    (flags & WDL_FLAG_BF16X3) ? 32 : 16);
+    a.bf3 = (flags & WDL_FLAG_BF16X3) ? 1 : 0;
     a.K = K;
     a.P = P;
     a.L = L;
@@ -746,7 +762,8 @@ struct wdl_solver {
 
   int lds_bytes(const SolveArgs& a) const {
     if (tiled) return tiled_lds_bytes(a);
-    return conv_lds_bytes(a.G) + ((flags & WDL_FLAG_LOG_DOMAIN) ? kLogKeysBytes : 0);
+    return conv_lds_bytes(a.G) + ((flags & WDL_FLAG_LOG_DOMAIN) ? kLogKeysBytes : 0) +
+           (a.bf3 ? ConvPrec<true>::kTabBytes - ConvPrec<false>::kTabBytes : 0);
   }
 
   // Side streams for the batched form: histogram groups advance on their own
@@ -860,10 +877,15 @@ static int set_lds_attr() {
   static bool done = false;
   if (done) return WDL_OK;
   const int bytes = conv_lds_bytes(make_geom(kMaxSide, kMaxSide, 0));
-  const void* fns[] = {(const void*)k_forward<1>, (const void*)k_forward<4>,
-                       (const void*)k_backward<1>, (const void*)k_backward<4>};
+  const void* fns[] = {(const void*)k_forward<1, false>, (const void*)k_forward<4, false>,
+                       (const void*)k_backward<1, false>, (const void*)k_backward<4, false>};
   for (const void* f : fns)
     WDL_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+  const void* bfns[] = {(const void*)k_forward<1, true>, (const void*)k_forward<4, true>,
+                        (const void*)k_backward<1, true>, (const void*)k_backward<4, true>};
+  for (const void* f : bfns)
+    WDL_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                bytes + ConvPrec<true>::kTabBytes - ConvPrec<false>::kTabBytes));
   const void* lfns[] = {(const void*)k_forward_log<1>, (const void*)k_forward_log<4>,
                         (const void*)k_backward_log<1>, (const void*)k_backward_log<4>};
   for (const void* f : lfns)
@@ -917,6 +939,11 @@ int wdl_solver_create(wdl_ctx* ctx, int W, int H, int K, int P, int L, double ga
             "wdl_solver_create: gamma must be positive and finite");
   WDL_CHECK(loss == WDL_LOSS_QUADRATIC || loss == WDL_LOSS_KL || loss == WDL_LOSS_TV,
             WDL_ERR_UNSUPPORTED, "wdl_solver_create: loss must be TV (1), quadratic (2) or KL (4)");
+  WDL_CHECK(!(flags & WDL_FLAG_BF16X3) ||
+                (!(flags & (WDL_FLAG_LOG_DOMAIN | WDL_FLAG_TILED)) && !needs_tiles(W, H)),
+            WDL_ERR_UNSUPPORTED,
+            "wdl_solver_create: WDL_FLAG_BF16X3 is for the standard domain with sides <= " +
+                std::to_string(kMaxSide));
   WDL_CHECK(!(flags & WDL_FLAG_EXACT_LSE) || (flags & WDL_FLAG_LOG_DOMAIN), WDL_ERR_ARG,
             "wdl_solver_create: WDL_FLAG_EXACT_LSE needs WDL_FLAG_LOG_DOMAIN");
   WDL_CHECK(!use_exact(flags) || std::max(W, H) <= kXMaxLen, WDL_ERR_UNSUPPORTED,
