@@ -10,10 +10,10 @@
 // maxima; a chunk whose bound max_c - d_min^2/gamma cannot raise m_j is not
 // visited for the maximum, and one that lies more than 40 nats below m_j is
 // not visited for the sum (its terms are below e^-40 each: at most len * 4e-18
-// relative).  Arguments are formed in FP64, the terms exponentiated and summed
-// in FP32 (each term <= 1).  Cost per output: one bound per chunk plus the
-// visited chunks' terms -- O(len) per line in the worst case, a few chunks when
-// the log values are smooth.
+// relative).  Terms are formed in FP64, exponentiated and summed in FP32 (each
+// term <= ~1).  Cost per output: one bound per chunk plus the visited chunks'
+// terms -- O(len) per line in the worst case, a few chunks when the log values
+// are smooth.
 //
 // The solver stages (exact_forward / exact_backward) run the log-domain
 // recursions of compute_gradient_both_log_signArray (inverseWasserstein.h:
@@ -31,13 +31,34 @@ constexpr double kXPrune = 40.0;
 
 __host__ __device__ inline int xlse_lds_bytes(int len) {
   const int nch = (len + 15) / 16;
-  return kXWaves * (len * (8 + 8 + 4 + 1) + nch * 8 + 16);
+  return kXWaves * (len * (8 + 8 + 4 + 4 + 1) + nch * (8 + 4)) + 64;
+}
+
+// Reach, in chunks, of the 16 outputs of a chunk: with m the smallest output
+// maximum among them (lanes with `valid` false excluded), a chunk c with
+// |c - oc| > R has bound <= gmax - (16 (|c - oc| - 1))^2 / gamma < m - margin.
+__device__ inline int chunk_reach(float gmax, float m, bool valid, float margin, float gam,
+                                  int nch) {
+  float mm = valid ? m : INFINITY;
+  for (int sh = 1; sh < 16; sh <<= 1) mm = fminf(mm, __shfl_xor(mm, sh));
+  const float span = gmax - mm + margin;
+  if (!(span < 1e30f)) return nch;
+  if (span <= 0.f) return 1;
+  return min(nch, 1 + (int)ceilf(sqrtf(span * gam) * (1.f / 16.f)));
 }
 
 // One pass over `nlines` lines of length `len` (line-major input), writing the
 // transposed layout (line index fastest) of each image of `rows` lines.  A line
 // is skipped (output untouched) when active_thr > 0 and Lp[p0 + image / ipp] <
 // active_thr.
+//
+// One wave per line.  Outputs are taken 16 at a time (one output chunk): lane l
+// serves output j = 16 oc + (l % 16) over the quarter l / 16 of every visited
+// input chunk, and the four lanes of an output combine their maxima and sums
+// with cross-lane shuffles, so the 16 outputs of a chunk -- which need nearly
+// the same input chunks -- move through them together.  The maximum is found
+// in FP32 (any shift near the maximum serves: the output adds back exactly the
+// shift the terms were formed with); terms are formed in FP64.
 template <bool SIGNED>
 __global__ __launch_bounds__(kXWaves * 64) void k_xlse_pass(
     const double* __restrict__ in, const unsigned char* __restrict__ sin, double* __restrict__ out,
@@ -46,15 +67,20 @@ __global__ __launch_bounds__(kXWaves * 64) void k_xlse_pass(
   extern __shared__ __align__(16) double xs[];
   const int nch = (len + 15) >> 4;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  // per wave: its line X, outputs O (+ signs SO), chunk maxima CM, input signs SG;
-  // O0 / SO0 address the outputs of all waves for the transposed store
+  const int o = lane & 15, q4 = lane >> 4;
+  const float igf = (float)inv_gamma;
+  const float gam = (float)(1.0 / inv_gamma);
+  // per wave: its line X (FP64) and XF (FP32), chunk maxima CM / CMF, input
+  // signs SG, outputs O and their signs SO; O0 / SO0 address all waves' outputs
   double* X = xs + wave * len;
   double* O0 = xs + kXWaves * len;
   double* O = O0 + wave * len;
   double* CM = xs + 2 * kXWaves * len + wave * nch;
-  float* SG = reinterpret_cast<float*>(xs + 2 * kXWaves * len + kXWaves * nch) + wave * len;
-  unsigned char* SO0 = reinterpret_cast<unsigned char*>(
-      reinterpret_cast<float*>(xs + 2 * kXWaves * len + kXWaves * nch) + kXWaves * len);
+  float* F0 = reinterpret_cast<float*>(xs + 2 * kXWaves * len + kXWaves * nch);
+  float* SG = F0 + wave * len;
+  float* XF = F0 + kXWaves * len + wave * len;
+  float* CMF = F0 + 2 * kXWaves * len + wave * nch;
+  unsigned char* SO0 = reinterpret_cast<unsigned char*>(F0 + 2 * kXWaves * len + kXWaves * nch);
   unsigned char* SO = SO0 + wave * len;
   for (long long l0 = (long long)blockIdx.x * kXWaves; l0 < nlines;
        l0 += (long long)gridDim.x * kXWaves) {
@@ -67,43 +93,74 @@ __global__ __launch_bounds__(kXWaves * 64) void k_xlse_pass(
     if (active) {
       const double* src = in + line * len;
       for (int k = lane; k < len; k += 64) {
-        X[k] = src[k];
+        const double x = src[k];
+        X[k] = x;
+        XF[k] = (float)x;
         if (SIGNED) SG[k] = sin[line * len + k] ? 1.f : -1.f;  // getSign (signArray.h:19-22)
       }
     }
     __syncthreads();
-    if (active)
+    float gmax = -INFINITY;  // the line's maximum, in every lane
+    if (active) {
       for (int c = lane; c < nch; c += 64) {
         double m = -INFINITY;
         for (int k = c * 16; k < min(len, c * 16 + 16); ++k) m = fmax(m, X[k]);
         CM[c] = m;
+        CMF[c] = (float)m;
+        gmax = fmaxf(gmax, (float)m);
       }
+      for (int sh = 1; sh < 64; sh <<= 1) gmax = fmaxf(gmax, __shfl_xor(gmax, sh));
+    }
     __syncthreads();
     if (active)
-      for (int j = lane; j < len; j += 64) {
-        const int cj = j >> 4;
-        double m = -INFINITY;
-        for (int k = cj * 16; k < min(len, cj * 16 + 16); ++k)
-          m = fmax(m, X[k] - (double)((j - k) * (j - k)) * inv_gamma);
-        for (int c = 0; c < nch; ++c) {
-          if (c == cj) continue;
-          const int d = c < cj ? j - (c * 16 + 15) : c * 16 - j;
-          if (CM[c] - (double)(d * d) * inv_gamma <= m) continue;
-          for (int k = c * 16; k < min(len, c * 16 + 16); ++k)
-            m = fmax(m, X[k] - (double)((j - k) * (j - k)) * inv_gamma);
+      for (int oc = 0; oc < nch; ++oc) {
+        const int j = oc * 16 + o;
+        // maximum of x_k - (j-k)^2/gamma: own chunk, then every chunk whose
+        // bound can raise it (the four lanes of j decide alike)
+        float m = -INFINITY;
+        for (int t = 0; t < 4; ++t) {
+          const int k = oc * 16 + q4 * 4 + t;
+          if (k < len) m = fmaxf(m, XF[k] - (float)((j - k) * (j - k)) * igf);
         }
-        m = fmax(m, -1e99);
-        float s = 0.f;
-        for (int c = 0; c < nch; ++c) {
-          const int d = c == cj ? 0 : (c < cj ? j - (c * 16 + 15) : c * 16 - j);
-          if (CM[c] - (double)(d * d) * inv_gamma - m < -kXPrune) continue;
-          for (int k = c * 16; k < min(len, c * 16 + 16); ++k) {
-            const float e = expf((float)(X[k] - (double)((j - k) * (j - k)) * inv_gamma - m));
-            s += SIGNED ? SG[k] * e : e;
+        m = fmaxf(m, __shfl_xor(m, 16));
+        m = fmaxf(m, __shfl_xor(m, 32));
+        // chunks more than R away cannot raise the maximum of any of the 16
+        // outputs: their bound is at most gmax - (16 (R-1))^2 / gamma <= m
+        const int R = chunk_reach(gmax, m, j < len, 0.f, gam, nch);
+        for (int c = max(0, oc - R); c <= min(nch - 1, oc + R); ++c) {
+          if (c == oc) continue;
+          const int d = c < oc ? j - (c * 16 + 15) : c * 16 - j;
+          if (CMF[c] - (float)(d * d) * igf <= m) continue;
+          float mc = -INFINITY;
+          for (int t = 0; t < 4; ++t) {
+            const int k = c * 16 + q4 * 4 + t;
+            if (k < len) mc = fmaxf(mc, XF[k] - (float)((j - k) * (j - k)) * igf);
+          }
+          mc = fmaxf(mc, __shfl_xor(mc, 16));
+          mc = fmaxf(mc, __shfl_xor(mc, 32));
+          m = fmaxf(m, mc);
+        }
+        const double m64 = fmax((double)m, -1e99);
+        float sum = 0.f;
+        // chunks beyond Rs lie more than 40 nats below every output's maximum
+        const int Rs = chunk_reach(gmax, m, j < len, (float)kXPrune + 1.f, gam, nch);
+        for (int c = max(0, oc - Rs); c <= min(nch - 1, oc + Rs); ++c) {
+          const int d = c == oc ? 0 : (c < oc ? j - (c * 16 + 15) : c * 16 - j);
+          if (CM[c] - (double)(d * d) * inv_gamma - m64 < -kXPrune) continue;
+          for (int t = 0; t < 4; ++t) {
+            const int k = c * 16 + q4 * 4 + t;
+            if (k < len) {
+              const float e = expf((float)(X[k] - (double)((j - k) * (j - k)) * inv_gamma - m64));
+              sum += SIGNED ? SG[k] * e : e;
+            }
           }
         }
-        O[j] = m + hlog(fabsf(s));
-        if (SIGNED) SO[j] = s >= 0.f ? 1 : 0;
+        sum += __shfl_xor(sum, 16);
+        sum += __shfl_xor(sum, 32);
+        if (q4 == 0 && j < len) {
+          O[j] = m64 + hlog(fabsf(sum));
+          if (SIGNED) SO[j] = sum >= 0.f ? 1 : 0;
+        }
       }
     __syncthreads();
     // transposed store: the lines of one workgroup are consecutive, so each
@@ -115,9 +172,9 @@ __global__ __launch_bounds__(kXWaves * 64) void k_xlse_pass(
       const long long img = ln / rows;
       if (active_thr > 0 && Lp[p0 + (int)(img / ipp)] < active_thr) continue;
       const int r = (int)(ln - img * rows);
-      const long long o = img * (long long)rows * len + (long long)j * rows + r;
-      out[o] = O0[w * len + j];
-      if (SIGNED) sout[o] = SO0[w * len + j];
+      const long long oi = img * (long long)rows * len + (long long)j * rows + r;
+      out[oi] = O0[w * len + j];
+      if (SIGNED) sout[oi] = SO0[w * len + j];
     }
     __syncthreads();
   }
