@@ -272,10 +272,39 @@ def cpu_baseline(c, gpu, budget_s=15.0):
     ref.gradient_both(c["atoms"], c["weights"][0], c["obs"][0], n, n, g, L, loss, variant="plain")
     t_plain = time.perf_counter() - t0
     out["plain_kernel_value"] = L / t_plain
+    # one OpenMP thread, the SSE/AVX kernel, one histogram
+    ref.set_threads(1)
+    t0 = time.perf_counter()
+    ref.gradient_both(c["atoms"], c["weights"][0], c["obs"][0], n, n, g, L, loss, variant="sse")
+    out["one_thread_value"] = L / (time.perf_counter() - t0)
+    ref.set_threads(out["cores"])
+    out["cpu_model"] = cpu_model()
+    # the log domain (LogSignArrayGaussian2DKernel) on the 256x256 log workload:
+    # 2 Sinkhorn iterations of one histogram (its O(n^3) exact passes take ~minutes
+    # per histogram at L=200), as iterations/s per histogram
+    cl = synthetic.make_problem("log_small_eps", P=1)
+    t0 = time.perf_counter()
+    ref.gradient_both(cl["atoms"], cl["weights"][0], cl["obs"][0], cl["n"], cl["n"], cl["gamma"], 2,
+                      cl["loss"], variant="log")
+    out["log_domain"] = {"value": round(2 / (time.perf_counter() - t0), 3),
+                         "unit": "Sinkhorn-iters/s",
+                         "sample": "compute_gradient_both_log_signArray, 256x256, K=6, gamma=0.5, "
+                                   "KL, one curve histogram, 2 iterations (forward + backward)"}
     err = {k: max(v) for k, v in errs.items()}
     err["histograms_checked"] = done
     err["against"] = "reference compute_gradient_both (FP64, CPU)"
     return out, err, refs
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def errors_against(refs, res):

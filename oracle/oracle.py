@@ -172,7 +172,22 @@ def safe_log(x):
     return np.log(np.maximum(EPSILON, x))
 
 
-def loss_value(kind: int, q: np.ndarray, o: np.ndarray) -> float:
+def wasserstein_scalings(q, o, W, H, gamma, n_iter=50):
+    """WassersteinLoss's Bregman projections, loss.h:103-117 / :132-146: a, b from
+    1, n_iter times c = K b, a = q / c, c = K a, b = o / c."""
+    a = np.ones_like(q)
+    b = np.ones_like(q)
+    for _ in range(n_iter):
+        a = q / convolve(b, W, H, gamma)
+        b = o / convolve(a, W, H, gamma)
+    return a, b
+
+
+def loss_value(kind: int, q: np.ndarray, o: np.ndarray, W=None, H=None, gamma=None) -> float:
+    if kind == LOSS_WASSERSTEIN:         # loss.h:100-128
+        a, b = wasserstein_scalings(q, o, W, H, gamma)
+        c = convolve(b, W, H, gamma)
+        return float(gamma * np.sum(safe_log(a) * q + safe_log(b) * o - a * c))
     if kind == LOSS_QUADRATIC:           # loss.h:25-32
         return float(np.sum((q - o) ** 2))
     if kind == LOSS_TV:                  # loss.h:49-56
@@ -182,7 +197,10 @@ def loss_value(kind: int, q: np.ndarray, o: np.ndarray) -> float:
     raise ValueError("loss kind %d not in the oracle" % kind)
 
 
-def loss_gradient(kind: int, q: np.ndarray, o: np.ndarray) -> np.ndarray:
+def loss_gradient(kind: int, q: np.ndarray, o: np.ndarray, W=None, H=None, gamma=None) -> np.ndarray:
+    if kind == LOSS_WASSERSTEIN:         # loss.h:130-152
+        a, _ = wasserstein_scalings(q, o, W, H, gamma)
+        return gamma * safe_log(a)
     if kind == LOSS_QUADRATIC:           # loss.h:34-40
         return 2.0 * (q - o)
     if kind == LOSS_TV:                  # loss.h:58-64
@@ -223,7 +241,7 @@ def gradient_both(atoms, lam, obs, W, H, gamma, L, loss=LOSS_QUADRATIC, b0=None,
         f = conv(a); phi.append(f)                 # :575
         q = np.exp((lam[:, None] * safe_log(f)).sum(axis=0))   # :577-585
         b.append(q[None, :] / f)                   # :587-592
-    g = loss_gradient(loss, q, obs)                # :597
+    g = loss_gradient(loss, q, obs, W, H, gamma)                # :597
     # atom gradient :600-629
     n = g.copy()
     v = np.zeros((K, N))
@@ -276,7 +294,7 @@ def gradient_both_log(atoms, lam, obs, W, H, gamma, L, loss=LOSS_QUADRATIC, b0=N
         lq = (lam[:, None] * f).sum(axis=0)
         b.append(lq[None, :] - f)
     q = np.exp(lq)                                 # :727-728
-    g = loss_gradient(loss, q, obs)
+    g = loss_gradient(loss, q, obs, W, H, gamma)
     # atom gradient :736-770 ; v holds -v of the standard form
     n = g.copy()
     v = np.zeros((K, N))
@@ -347,7 +365,7 @@ def evaluate_both(x, observed, W, H, gamma, K, P, L, loss=LOSS_QUADRATIC, scale_
         res = solve(a, w[p], observed[p], W, H, gamma, L, loss, b0=b0)
         if b_temp is not None:
             b_temp[p] = res["b_last"]
-        f += loss_value(loss, res["bary"], observed[p])       # :1089-1091
+        f += loss_value(loss, res["bary"], observed[p], W, H, gamma)  # :1089-1091
         GL = res["grad_lambda"]
         g[p * K:(p + 1) * K] = GL - np.dot(w[p], GL)          # :1095-1099
         GD = res["grad_dic"]
@@ -391,7 +409,7 @@ def project_weights(atoms, obs, W, H, gamma, L, rounds, step, loss=LOSS_QUADRATI
     for _ in range(rounds):
         r = solve(atoms, lam, obs, W, H, gamma, L, loss, b0=b)
         b = r["b_last"]
-        f = loss_value(loss, r["bary"], obs)
+        f = loss_value(loss, r["bary"], obs, W, H, gamma)
         G = r["grad_lambda"]
         g = G - lam @ G
         m = np.abs(g).max()

@@ -40,6 +40,7 @@ def _lib(name: str):
                                               _d, _d]
             lib.ref_evaluate_both.restype = _f
             lib.ref_num_threads.restype = _i
+            lib.ref_set_threads.argtypes = [_i]
         if lib is not None and name == "libwdlref_log.so":
             lib.ref_evaluate_both_log.argtypes = [_i, _i, _f, _i, _i, _i, _i, _f, _i, _d, _d, _d, _d,
                                                   _d, _d]
@@ -144,3 +145,7 @@ def evaluate_both(x, observed, W, H, gamma, K, P, L, loss=2, scale_dic=1.0, exp_
 
 def num_threads() -> int:
     return int(_lib("libwdlref.so").ref_num_threads())
+
+
+def set_threads(n: int) -> None:
+    _lib("libwdlref.so").ref_set_threads(int(n))

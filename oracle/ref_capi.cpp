@@ -209,6 +209,15 @@ int ref_num_threads(void) {
 #endif
 }
 
+// OpenMP threads of the reference's parallel loops (for the 1-thread baseline).
+void ref_set_threads(int n) {
+#ifdef HAS_OPENMP
+  omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
 #else  // COMPUTE_BARYCENTER_LOG
 
 // evaluate_both built with COMPUTE_BARYCENTER_LOG (inverseWasserstein.h:1066-1068).

@@ -113,7 +113,36 @@ def main():
 
     for name, d in out.items():
         np.savez_compressed(os.path.join(HERE, name + ".npz"), **d)
+    wasserstein_fixtures()
     print("wrote %d fixtures to %s" % (len(out), HERE))
+
+
+def wasserstein_fixtures():
+    # the Wasserstein loss (loss.h:92-154), with a generator of its own so the
+    # fixtures above keep their values
+    rw = np.random.default_rng(303)
+    for j, (W, H, K, L, g, logd) in enumerate([(12, 12, 3, 6, 2.0, False),
+                                               (10, 12, 2, 5, 1.0, True)]):
+        N = W * H
+        atoms = rw.uniform(0, 1, (K, N))
+        lam = rw.uniform(0, 1, K)
+        obs = blob_hist(rw, W, H)
+        r = ref.gradient_both(atoms, lam, obs, W, H, g, L, 3, variant="log" if logd else "plain")
+        d = dict(W=W, H=H, K=K, L=L, gamma=g, loss=3, atoms=atoms, lam=lam, obs=obs, **r)
+        if not logd:
+            d["b0"] = np.zeros(0)
+        name = "glog_2" if logd else "grad_5"
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **d)
+    W, H, K, P, L, g = 8, 8, 2, 3, 4, 1.0
+    N = W * H
+    obs = np.stack([blob_hist(rw, W, H) for _ in range(P)])
+    xx = np.concatenate([np.log(rw.uniform(0.1, 1, P * K)) * 2.0,
+                         np.log(rw.uniform(0.1, 1, K * N))])
+    f, gr, _ = ref.evaluate_both(xx, obs, W, H, g, K, P, L, 3, 2.0, True)
+    np.savez_compressed(os.path.join(HERE, "eval_5.npz"),
+                        W=W, H=H, K=K, P=P, L=L, gamma=g, loss=3, scale=2.0, exp_weight=True,
+                        log_domain=False, x=xx, obs=obs, b_storage=np.zeros(0), f=f, g=gr,
+                        b_temp=np.zeros(0))
 
 
 if __name__ == "__main__":

@@ -47,7 +47,8 @@ def oracle_all(atoms, lam, obs, W, H, g, L, loss, b0=None):
                 grad_dic=np.stack([r["grad_dic"] for r in rs]),
                 bary=np.stack([r["bary"] for r in rs]),
                 b_last=np.stack([r["b_last"] for r in rs]),
-                loss=np.array([O.loss_value(loss, rs[p]["bary"], obs[p]) for p in range(len(rs))]))
+                loss=np.array([O.loss_value(loss, rs[p]["bary"], obs[p], W, H, g)
+                               for p in range(len(rs))]))
 
 
 def check(r, o):
@@ -61,7 +62,8 @@ def check(r, o):
 
 @pytest.mark.parametrize("W,H,K,P,L,g,loss", [
     (12, 12, 3, 2, 6, 2.0, 2), (16, 12, 2, 3, 10, 1.0, 4), (20, 20, 3, 9, 8, 0.5, 1),
-    (64, 64, 4, 4, 25, 2.0, 2), (128, 128, 5, 2, 100, 1.0, 2)])
+    (64, 64, 4, 4, 25, 2.0, 2), (128, 128, 5, 2, 100, 1.0, 2), (32, 32, 3, 3, 10, 2.0, 3),
+    (64, 64, 4, 2, 25, 2.0, 3)])
 def test_gradient_both_matches_oracle(ctx, W, H, K, P, L, g, loss):
     print("gradient_both W=%d H=%d K=%d P=%d L=%d gamma=%g loss=%d" % (W, H, K, P, L, g, loss),
           flush=True)

@@ -9,6 +9,7 @@
 namespace wdl {
 
 constexpr int kLossBlocks = 32;
+constexpr int kWassersteinIters = 50;  // WassersteinLoss's n_iter default (loss.h:95)
 
 struct SolveArgs {
   ConvGeom G;
@@ -51,6 +52,12 @@ struct SolveArgs {
   double* GLt;
   double* gsumPart;  // [P][kLossBlocks] partial sums of g_L = loss'(q) q (k_final)
   double* Mt;        // tiled: [L][P][K][tiles] partial sums of log phi (forward)
+  // Wasserstein loss: log a, log b of its inner Sinkhorn, a convolution result
+  // and the transposed intermediate, P x N FP64 each
+  double* wla;
+  double* wlb;
+  double* wlc;
+  double* wlt;
   // exact-shift log domain: gamma, FP64 scratch images and sign bytes
   double gamma;
   double* xa;
@@ -163,6 +170,8 @@ int tiled_reduce_gl(const SolveArgs& a, hipStream_t st);
 int xlse_set_lds_attr(int maxlen);
 int exact_forward(const SolveArgs& a, hipStream_t st, int s);
 int exact_backward(const SolveArgs& a, hipStream_t st, int t);
+// WassersteinLoss at the barycenters: fills G0, lossPart, gsumPart (wdl_xlse.hip).
+int wasserstein_loss(const SolveArgs& a, hipStream_t st);
 constexpr int kXMaxLen = 1024;  // longest line of the exact-shift passes
 
 // Sum over the workgroup, result valid in thread 0.

@@ -129,15 +129,16 @@ __global__ __launch_bounds__(kThreads) void k_final(SolveArgs a) {
     QL[x] = q;
     if (a.bSave)
       for (int j = 0; j < K; ++j) a.bSave[hbase + j * N + x] = q / F[j * N + x];
+    a.Q[(long long)p * N + x] = q;
+    if (a.loss == WDL_LOSS_WASSERSTEIN) continue;  // k_wloss_final
     const float o = a.obs[(long long)p * N + x];
     const float g0 = loss_grad(a.loss, q, o);
-    a.Q[(long long)p * N + x] = q;
     a.G0[(long long)p * N + x] = g0;
     acc += loss_term(a.loss, q, o);
     gs += (double)(g0 * q);  // g_L as the weight chain forms it
   }
   const double t = block_sum(acc), tg = block_sum(gs);
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0 && a.loss != WDL_LOSS_WASSERSTEIN) {
     a.lossPart[(long long)p * kLossBlocks + blockIdx.x] = t;
     a.gsumPart[(long long)p * kLossBlocks + blockIdx.x] = tg;
   }
@@ -437,15 +438,16 @@ __global__ __launch_bounds__(kThreads) void k_final_log(SolveArgs a) {
       if (a.bSave) a.bSave[hbase + j * N + x] = (float)b;
     }
     const float q = (float)exp(lq);
+    a.Q[(long long)p * N + x] = q;
+    if (a.loss == WDL_LOSS_WASSERSTEIN) continue;  // k_wloss_final
     const float o = a.obs[(long long)p * N + x];
     const float g0 = loss_grad(a.loss, q, o);
-    a.Q[(long long)p * N + x] = q;
     a.G0[(long long)p * N + x] = g0;
     acc += loss_term(a.loss, q, o);
     gs += (double)(g0 * q);
   }
   const double t = block_sum(acc), tg = block_sum(gs);
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0 && a.loss != WDL_LOSS_WASSERSTEIN) {
     a.lossPart[(long long)p * kLossBlocks + blockIdx.x] = t;
     a.gsumPart[(long long)p * kLossBlocks + blockIdx.x] = tg;
   }
@@ -692,7 +694,7 @@ struct wdl_solver {
   int D = 0;
   DevBuf tab, A, lam, obs, Bh, Qh, Ch, Fh, V0, V1, R0, R1, GD, GL, Q, G0, lossPart, bStore,
       bTemp, GDsum, logA, events, Lp, X0, X1, XS0, XS1, GLt, gsumPart, Mt, eta, xa, xb, xt, sga,
-      sgb, sgt;
+      sgb, sgt, wla, wlb, wlc, wlt;
   bool tiled = false;  // 64 x 64 tiles with a halo (a side above kMaxSide)
   bool exact = false;  // log domain with the exact-shift line passes (WDL_FLAG_EXACT_LSE)
   TileGeom tg{};
@@ -751,6 +753,10 @@ struct wdl_solver {
     a.gsumPart = gsumPart.as<double>();
     a.Mt = Mt.as<double>();
     a.gamma = gamma;
+    a.wla = wla.as<double>();
+    a.wlb = wlb.as<double>();
+    a.wlc = wlc.as<double>();
+    a.wlt = wlt.as<double>();
     a.xa = xa.as<double>();
     a.xb = xb.as<double>();
     a.xt = xt.as<double>();
@@ -858,6 +864,7 @@ struct wdl_solver {
       hipLaunchKernelGGL(k_final_log, dim3(kLossBlocks, np), blk, 0, st, a);
     else
       hipLaunchKernelGGL(k_final, dim3(kLossBlocks, np), blk, 0, st, a);
+    if (loss == WDL_LOSS_WASSERSTEIN && (e = wasserstein_loss(a, st))) return e;
     for (int s = L; s >= 1; --s) {
       if (exact) {
         if ((e = exact_backward(a, st, s))) return e;
@@ -915,6 +922,7 @@ static long long footprint(int W, int H, int K, int P, int L, int flags) {
   if (flags & WDL_FLAG_WARM_RESTART) f += 2 * PKN * 4;
   if (flags & WDL_FLAG_LOG_DOMAIN) f += (long long)K * N * 4;  // log atoms
   f += kTabLen * 4;
+  f += 4LL * P * N * 8;  // the Wasserstein loss's log scalings and convolution buffers
   if (use_exact(flags)) f += 3LL * 2 * PKN * (8 + 1);  // FP64 scratch images + sign bytes
   if (use_tiles(W, H, flags)) {  // X0, X1 (+ sign bytes), per-tile dots
     const long long nt = (long long)((W + kTile - 1) / kTile) * ((H + kTile - 1) / kTile);
@@ -937,8 +945,12 @@ int wdl_solver_create(wdl_ctx* ctx, int W, int H, int K, int P, int L, double ga
             "wdl_solver_create: W, H, K, P, L must be positive");
   WDL_CHECK(gamma > 0 && std::isfinite(gamma), WDL_ERR_ARG,
             "wdl_solver_create: gamma must be positive and finite");
-  WDL_CHECK(loss == WDL_LOSS_QUADRATIC || loss == WDL_LOSS_KL || loss == WDL_LOSS_TV,
-            WDL_ERR_UNSUPPORTED, "wdl_solver_create: loss must be TV (1), quadratic (2) or KL (4)");
+  WDL_CHECK(loss == WDL_LOSS_QUADRATIC || loss == WDL_LOSS_KL || loss == WDL_LOSS_TV ||
+                loss == WDL_LOSS_WASSERSTEIN,
+            WDL_ERR_UNSUPPORTED, "wdl_solver_create: loss must be 1, 2, 3 or 4 (main_dictionary_learning.cpp:189-197)");
+  WDL_CHECK(loss != WDL_LOSS_WASSERSTEIN || std::max(W, H) <= kXMaxLen, WDL_ERR_UNSUPPORTED,
+            "wdl_solver_create: the Wasserstein loss takes sides up to " +
+                std::to_string(kXMaxLen));
   WDL_CHECK(!(flags & WDL_FLAG_BF16X3) ||
                 (!(flags & (WDL_FLAG_LOG_DOMAIN | WDL_FLAG_TILED)) && !needs_tiles(W, H)),
             WDL_ERR_UNSUPPORTED,
@@ -960,7 +972,9 @@ int wdl_solver_create(wdl_ctx* ctx, int W, int H, int K, int P, int L, double ga
   WDL_HIP(hipSetDevice(ctx->device));
   if ((st = set_lds_attr())) return st;
   if ((st = tiled_set_lds_attr())) return st;
-  if (use_exact(flags) && (st = xlse_set_lds_attr(std::max(W, H)))) return st;
+  if ((use_exact(flags) || loss == WDL_LOSS_WASSERSTEIN) &&
+      (st = xlse_set_lds_attr(std::max(W, H))))
+    return st;
   wdl_solver* s = new wdl_solver();
   s->tiled = tiled;
   s->exact = use_exact(flags);
@@ -1021,6 +1035,10 @@ int wdl_solver_create(wdl_ctx* ctx, int W, int H, int K, int P, int L, double ga
       {&s->sga, use_exact(flags) ? (size_t)2 * PKN : 0},
       {&s->sgb, use_exact(flags) ? (size_t)2 * PKN : 0},
       {&s->sgt, use_exact(flags) ? (size_t)2 * PKN : 0},
+      {&s->wla, (size_t)P * N * sizeof(double)},
+      {&s->wlb, (size_t)P * N * sizeof(double)},
+      {&s->wlc, (size_t)P * N * sizeof(double)},
+      {&s->wlt, (size_t)P * N * sizeof(double)},
   };
   for (auto& al : allocs) {
     if ((st = al.b->ensure(al.n))) {
@@ -1073,7 +1091,8 @@ int wdl_solver_destroy(wdl_solver* s) {
                     &s->V0,  &s->V1, &s->R0,  &s->R1,  &s->GD, &s->GL,       &s->Q,
                     &s->G0,  &s->lossPart,    &s->bStore,      &s->bTemp,    &s->GDsum,
                     &s->logA, &s->events, &s->Lp, &s->X0, &s->X1, &s->XS0, &s->XS1, &s->GLt, &s->Mt, &s->gsumPart, &s->eta, &s->xa, &s->xb, &s->xt, &s->sga, &s->sgb,
-                    &s->sgt};
+                    &s->sgt, &s->wla, &s->wlb, &s->wlc,
+                    &s->wlt};
   for (DevBuf* b : bufs) b->release();
   delete s;
   return WDL_OK;

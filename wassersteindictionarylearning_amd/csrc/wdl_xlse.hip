@@ -443,6 +443,80 @@ int exact_backward(const SolveArgs& a, hipStream_t st, int t) {
   return WDL_OK;
 }
 
+// ---- the Wasserstein loss (loss.h:92-154) ----------------------------------------
+// Its Bregman projections between the barycenter q and the observed o, 50
+// iterations from b = 1: c = K b, a = q / c, c = K a, b = o / c (:106-117), run
+// on log a, log b with the exact-shift passes (the scalings leave FP32 range at
+// small gamma); loss gamma * sum(safe_log(a) q + safe_log(b) o - a K b) (:119-127)
+// and gradient gamma * safe_log(a) (:148-150).
+
+__global__ void k_wl_init(SolveArgs a) {
+  const long long tot = (long long)a.np * a.N, off = (long long)a.p0 * a.N;
+  XLOOP(tot) a.wlb[off + e] = 0.0;
+}
+
+// log a = log q - conv (second = false) or log b = log o - conv (second = true)
+__global__ void k_wl_update(SolveArgs a, int second) {
+  const long long tot = (long long)a.np * a.N, off = (long long)a.p0 * a.N;
+  XLOOP(tot) {
+    const long long i = off + e;
+    const float v = second ? a.obs[i] : a.Q[i];
+    const double lv = v > 0.f ? (double)logf(v) : -INFINITY;
+    (second ? a.wlb : a.wla)[i] = lv - a.wlc[i];
+  }
+}
+
+__device__ inline double safe_log_d(double l) { return l > -575.64627324851142 ? l : -575.64627324851142; }
+
+// one workgroup per histogram: loss, gradient and sum of g_L = gradient * q
+__global__ __launch_bounds__(kThreads) void k_wl_final(SolveArgs a) {
+  const int p = a.p0 + (int)blockIdx.x;
+  const long long N = a.N, off = (long long)p * N;
+  if (steps_of(a, p) == 0) {
+    if (threadIdx.x < kLossBlocks) {
+      a.lossPart[(long long)p * kLossBlocks + threadIdx.x] = 0.0;
+      a.gsumPart[(long long)p * kLossBlocks + threadIdx.x] = 0.0;
+    }
+    return;
+  }
+  double l = 0, gs = 0;
+  for (long long x = threadIdx.x; x < N; x += blockDim.x) {
+    const double la = safe_log_d(a.wla[off + x]), lb = safe_log_d(a.wlb[off + x]);
+    const float q = a.Q[off + x], o = a.obs[off + x];
+    const float g = (float)(a.gamma * la);
+    a.G0[off + x] = g;
+    l += la * q + lb * o - exp(a.wla[off + x] + a.wlc[off + x]);
+    gs += (double)(g * q);
+  }
+  const double tl = block_sum(l), tg = block_sum(gs);
+  if (threadIdx.x < kLossBlocks) {
+    a.lossPart[(long long)p * kLossBlocks + threadIdx.x] = threadIdx.x == 0 ? a.gamma * tl : 0.0;
+    a.gsumPart[(long long)p * kLossBlocks + threadIdx.x] = threadIdx.x == 0 ? tg : 0.0;
+  }
+}
+
+int wasserstein_loss(const SolveArgs& a, hipStream_t st) {
+  const long long n = (long long)a.np * a.N, off = (long long)a.p0 * a.N;
+  const int W = a.G.W, H = a.G.H;
+  int e;
+  hipLaunchKernelGGL(k_wl_init, ew_grid(n), dim3(256), 0, st, a);
+  for (int it = 0; it <= kWassersteinIters; ++it) {
+    // c = K b; the 51st convolution is the one the loss evaluates (:120)
+    if ((e = xlse_conv(a.wlb + off, nullptr, a.wlc + off, nullptr, a.wlt + off, nullptr, W, H,
+                       a.np, a.gamma, a.Lp, a.p0, 1, 1, st)))
+      return e;
+    if (it == kWassersteinIters) break;
+    hipLaunchKernelGGL(k_wl_update, ew_grid(n), dim3(256), 0, st, a, 0);
+    if ((e = xlse_conv(a.wla + off, nullptr, a.wlc + off, nullptr, a.wlt + off, nullptr, W, H,
+                       a.np, a.gamma, a.Lp, a.p0, 1, 1, st)))
+      return e;
+    hipLaunchKernelGGL(k_wl_update, ew_grid(n), dim3(256), 0, st, a, 1);
+  }
+  hipLaunchKernelGGL(k_wl_final, dim3((unsigned)a.np), dim3(kThreads), 0, st, a);
+  WDL_HIP(hipGetLastError());
+  return WDL_OK;
+}
+
 #undef XLOOP
 
 }  // namespace wdl
