@@ -139,11 +139,14 @@ int wdl_solver_run_stage(wdl_solver* s, int stage, int step);
  * barycenters (P x N).  Any pointer may be NULL. */
 int wdl_solver_get_results(wdl_solver* s, double* loss, double* grad_lambda,
                            double* grad_dic_sum, double* barycenters);
-/* Log-domain solver: number of image lines, in the last run, whose spread of
- * log values exceeded 80 nats, the range within which the shifted FP32 MFMA
- * pass loses no term that matters; 0 means every log-sum-exp was computed to
- * FP32 accuracy.
- * No counterpart in the reference (its per-output max has no such limit). */
+/* Range events of the last run.  Log domain (shifted MFMA form): image lines
+ * whose spread of log values exceeded 80 nats, the range within which the
+ * shifted FP32 MFMA pass loses no term that matters (the exact-shift form,
+ * WDL_FLAG_EXACT_LSE, has no such limit and counts none).  Standard domain:
+ * pixels whose final scalings phi or barycenter were not finite and positive,
+ * i.e. the Sinkhorn scalings left the FP32 range (run such inputs in the log
+ * domain).  0 means every value stayed in range.
+ * No counterpart in the reference (FP64, exact per-output max). */
 int wdl_solver_log_range_events(wdl_solver* s, long long* count);
 /* resultDic of every solve (P x K x N). */
 int wdl_solver_get_grad_dic(wdl_solver* s, double* grad_dic);

@@ -128,8 +128,12 @@ def test_exact_solver_far_transport(ctx):
     lam = np.full((P, K), 1.0 / K)
     mf = run(ctx, atoms, lam, obs, W, H, g, L, 2, wdl.FLAG_LOG_DOMAIN)
     ex = run(ctx, atoms, lam, obs, W, H, g, L, 2, EXACT)
-    print("shifted-MFMA range events:", mf["events"], flush=True)
+    sd = run(ctx, atoms, lam, obs, W, H, g, L, 2, 0)
+    print("range events: shifted MFMA %d, standard domain %d, exact %d"
+          % (mf["events"], sd["events"], ex["events"]), flush=True)
     assert mf["events"] > 0  # log b spreads over ~200 nats along a line here
+    assert sd["events"] > 0  # b ~ e^200 is beyond FP32
+    assert ex["events"] == 0
     # dL/dlambda is ill-conditioned here: the dot products sum terms up to ~9e2
     # times larger than a component (0.0011 from terms summing to 0.74), so the
     # FP32 rounding of g (~1e-7) shows at ~1e-5; its bound here is 3e-5

@@ -34,6 +34,7 @@ def solve_gpu(ctx, atoms, lam, obs, W, H, g, L, loss, flags=0, mode=wdl.RUN_BATC
     s.run(mode)
     r = s.results()
     r["grad_dic"] = s.grad_dic()
+    r["events"] = s.log_range_events()
     if b0 is not None:
         r["b_temp"] = s.get_warm_state()
     s.close()
@@ -74,6 +75,7 @@ def test_gradient_both_matches_oracle(ctx, W, H, K, P, L, g, loss):
     atoms, lam = synthetic.init_atoms_weights(rng, K, P, W * H)
     r = solve_gpu(ctx, atoms, lam, obs, W, H, g, L, loss)
     check(r, oracle_all(atoms, lam, obs, W, H, g, L, loss))
+    assert r["events"] == 0  # every scaling stayed in FP32 range
 
 
 @pytest.mark.parametrize("name,d", fixtures("grad"))

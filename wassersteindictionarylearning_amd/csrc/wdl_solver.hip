@@ -120,28 +120,36 @@ __global__ __launch_bounds__(kThreads) void k_final(SolveArgs a) {
   const float* F = a.Fh + (long long)(Lp - 1) * a.stage + hbase;
   float* QL = a.Qh + (long long)(Lp - 1) * a.qstage + (long long)p * N;
   const float* lam = a.lam + (long long)p * K;
-  double acc = 0, gs = 0;
+  double acc = 0, gs = 0, bad = 0;
   for (long long x = (long long)blockIdx.x * kThreads + threadIdx.x; x < N;
        x += (long long)gridDim.x * kThreads) {
     float lq = 0.f;
-    for (int j = 0; j < K; ++j) lq += lam[j] * safe_logf(F[j * N + x]);
+    bool ok = true;
+    for (int j = 0; j < K; ++j) {
+      const float f = F[j * N + x];
+      ok = ok && f > 0.f && f < INFINITY;
+      lq += lam[j] * safe_logf(f);
+    }
     const float q = expf(lq);
+    ok = ok && q > 0.f && q < INFINITY;
+    if (!ok) bad += 1.0;  // the FP32 scalings left their range at this pixel
     QL[x] = q;
     if (a.bSave)
       for (int j = 0; j < K; ++j) a.bSave[hbase + j * N + x] = q / F[j * N + x];
     a.Q[(long long)p * N + x] = q;
-    if (a.loss == WDL_LOSS_WASSERSTEIN) continue;  // k_wloss_final
+    if (a.loss == WDL_LOSS_WASSERSTEIN) continue;  // set by wasserstein_loss
     const float o = a.obs[(long long)p * N + x];
     const float g0 = loss_grad(a.loss, q, o);
     a.G0[(long long)p * N + x] = g0;
     acc += loss_term(a.loss, q, o);
     gs += (double)(g0 * q);  // g_L as the weight chain forms it
   }
-  const double t = block_sum(acc), tg = block_sum(gs);
+  const double t = block_sum(acc), tg = block_sum(gs), tb = block_sum(bad);
   if (threadIdx.x == 0 && a.loss != WDL_LOSS_WASSERSTEIN) {
     a.lossPart[(long long)p * kLossBlocks + blockIdx.x] = t;
     a.gsumPart[(long long)p * kLossBlocks + blockIdx.x] = tg;
   }
+  if (threadIdx.x == 0 && tb > 0 && a.events) atomicAdd(a.events, (unsigned)tb);
 }
 
 // Backward step s (L..1).  chain 0: dictionary gradient (:600-629);
@@ -439,7 +447,7 @@ __global__ __launch_bounds__(kThreads) void k_final_log(SolveArgs a) {
     }
     const float q = (float)exp(lq);
     a.Q[(long long)p * N + x] = q;
-    if (a.loss == WDL_LOSS_WASSERSTEIN) continue;  // k_wloss_final
+    if (a.loss == WDL_LOSS_WASSERSTEIN) continue;  // set by wasserstein_loss
     const float o = a.obs[(long long)p * N + x];
     const float g0 = loss_grad(a.loss, q, o);
     a.G0[(long long)p * N + x] = g0;
@@ -743,7 +751,7 @@ struct wdl_solver {
     a.weightsOnly = (flags & WDL_FLAG_WEIGHTS_ONLY) ? 1 : 0;
     const bool logd = flags & WDL_FLAG_LOG_DOMAIN;
     a.logA = logd ? logA.as<float>() : nullptr;
-    a.events = logd ? events.as<unsigned>() : nullptr;
+    a.events = events.as<unsigned>();
     a.T = tg;
     a.X0 = X0.as<float>();
     a.X1 = X1.as<float>();
@@ -819,8 +827,7 @@ struct wdl_solver {
   // All P solves; with several groups, fork onto the side streams and join
   // back on the context stream before the final reduction.
   int enqueue_batched() {
-    if (flags & WDL_FLAG_LOG_DOMAIN)
-      hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, ctx->stream, events.as<unsigned>());
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, ctx->stream, events.as<unsigned>());
     if (ngroups <= 1) return enqueue(0, P, true, ctx->stream);
     WDL_HIP(hipEventRecord(fork, ctx->stream));
     int st;
@@ -1220,8 +1227,7 @@ int wdl_solver_run(wdl_solver* s, int mode) {
   WDL_HIP(hipSetDevice(s->ctx->device));
   int st;
   if (mode == WDL_RUN_PLAIN) {
-    if (s->flags & WDL_FLAG_LOG_DOMAIN)
-      hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s->ctx->stream, s->events.as<unsigned>());
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s->ctx->stream, s->events.as<unsigned>());
     for (int p = 0; p < s->P; ++p)
       if ((st = s->enqueue(p, 1, p == s->P - 1, s->ctx->stream))) return st;
   } else if (s->flags & WDL_FLAG_NO_GRAPH) {
