@@ -315,18 +315,32 @@ __device__ inline void px_pass(const ConvGeom& G, Load&& load, Apply&& apply) {
   const int q = G.Wp / VW;
   const int total = G.Hp * q;
   typedef decltype(load(0)) V;
+  // Group e = threadIdx.x + k kThreads sits at row e / q, column VW (e % q);
+  // a thread's successive groups advance by (drow, dcol) rows and groups, plus
+  // one row when the column wraps -- no per-group integer division.
+  const int drow = kThreads / q, dcol = kThreads - drow * q;
+  int row = threadIdx.x / q, col = threadIdx.x - row * q;
+  int lrow = row * G.S1 + col * VW, grow = row * G.W + col * VW;
   for (int base = threadIdx.x; base < total; base += kThreads * kUnroll) {
     V v[kUnroll];
     int li[kUnroll], gi[kUnroll];
     bool ok[kUnroll];
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
-      const int e = base + u * kThreads;
-      const int row = e / q, c = (e - row * q) * VW;
-      li[u] = row * G.S1 + c;
-      gi[u] = row * G.W + c;
-      ok[u] = e < total && row < G.H && c < G.W;
+      li[u] = lrow;
+      gi[u] = grow;
+      ok[u] = row < G.H && col * VW < G.W;
       if (ok[u]) v[u] = load(gi[u]);
+      row += drow;
+      col += dcol;
+      lrow += drow * G.S1 + dcol * VW;
+      grow += drow * G.W + dcol * VW;
+      if (col >= q) {
+        col -= q;
+        ++row;
+        lrow += G.S1 - G.Wp;
+        grow += G.W - G.Wp;
+      }
     }
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u)
@@ -340,6 +354,25 @@ __device__ inline void px_pass(const ConvGeom& G, Load&& load, Apply&& apply) {
 __device__ inline float safe_logf(float x) { return x > 0.f ? logf(x) : -575.64627324851142f; }
 __device__ inline floatx4 safe_logf(floatx4 x) {
   return floatx4{safe_logf(x.x), safe_logf(x.y), safe_logf(x.z), safe_logf(x.w)};
+}
+// x / y of the elementwise steps: the hardware reciprocal (1 ulp) and one
+// Newton-Raphson correction of the quotient, q1 = q0 + r (x - y q0) with
+// q0 = x r (Markstein, IBM J. Res. Develop. 34(1), 1990): within 1 ulp, and
+// mostly the rounded quotient, when y and x / y are normal FP32 numbers --
+// 4 instructions against ~10 for the IEEE expansion.  Out of that range
+// (y = 0, +-inf, subnormal) the result may be NaN where IEEE gives +-inf or a
+// huge finite value; such scalings have left the FP32 range anyway, and the
+// solver counts them as range events.
+__device__ inline float fdiv(float x, float y) {
+  const float r = __builtin_amdgcn_rcpf(y);
+  const float q = x * r;
+  return fmaf(fmaf(-y, q, x), r, q);
+}
+__device__ inline floatx4 fdiv(floatx4 x, floatx4 y) {
+  return floatx4{fdiv(x.x, y.x), fdiv(x.y, y.y), fdiv(x.z, y.z), fdiv(x.w, y.w)};
+}
+__device__ inline floatx4 fdiv(float x, floatx4 y) {
+  return floatx4{fdiv(x, y.x), fdiv(x, y.y), fdiv(x, y.z), fdiv(x, y.w)};
 }
 __device__ inline float vexpf(float x) { return expf(x); }
 __device__ inline floatx4 vexpf(floatx4 x) { return floatx4{expf(x.x), expf(x.y), expf(x.z), expf(x.w)}; }

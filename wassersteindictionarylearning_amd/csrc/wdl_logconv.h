@@ -94,19 +94,31 @@ __device__ inline void px_pass_r(const ConvGeom& G, Load&& load, Apply&& apply) 
   const int q = G.Wp / VW;
   const int total = G.Hp * q;
   typedef decltype(load(0)) V;
+  // the incremental group walk of px_pass
+  const int drow = kThreads / q, dcol = kThreads - drow * q;
+  int r = threadIdx.x / q, col = threadIdx.x - r * q;
+  int lrow = r * G.S1 + col * VW, grow = r * G.W + col * VW;
   for (int base = threadIdx.x; base < total; base += kThreads * kUnroll) {
     V v[kUnroll];
     int li[kUnroll], gi[kUnroll], row[kUnroll];
     bool ok[kUnroll];
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
-      const int e = base + u * kThreads;
-      row[u] = e / q;
-      const int c = (e - row[u] * q) * VW;
-      li[u] = row[u] * G.S1 + c;
-      gi[u] = row[u] * G.W + c;
-      ok[u] = e < total && row[u] < G.H && c < G.W;
+      row[u] = r;
+      li[u] = lrow;
+      gi[u] = grow;
+      ok[u] = r < G.H && col * VW < G.W;
       if (ok[u]) v[u] = load(gi[u]);
+      r += drow;
+      col += dcol;
+      lrow += drow * G.S1 + dcol * VW;
+      grow += drow * G.W + dcol * VW;
+      if (col >= q) {
+        col -= q;
+        ++r;
+        lrow += G.S1 - G.Wp;
+        grow += G.W - G.Wp;
+      }
     }
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u)

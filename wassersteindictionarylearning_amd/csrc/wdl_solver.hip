@@ -71,7 +71,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_forward(SolveArgs a, int s) {
           if (ok) {
             const V q = vexpf(r.lq);
             if (Qd) vst<VW>(Qd + gi, q);
-            b = q / r.fi;
+            b = fdiv(q, r.fi);
           }
           vst<VW>(buf + li, b);
         });
@@ -87,7 +87,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_forward(SolveArgs a, int s) {
           if (ok) {
             const V c = vld<VW>(buf + li);
             vst<VW>(Cd + gi, c);
-            u = av / c;
+            u = fdiv(av, c);
           }
           vst<VW>(buf + li, u);
         });
@@ -135,7 +135,7 @@ __global__ __launch_bounds__(kThreads) void k_final(SolveArgs a) {
     if (!ok) bad += 1.0;  // the FP32 scalings left their range at this pixel
     QL[x] = q;
     if (a.bSave)
-      for (int j = 0; j < K; ++j) a.bSave[hbase + j * N + x] = q / F[j * N + x];
+      for (int j = 0; j < K; ++j) a.bSave[hbase + j * N + x] = fdiv(q, F[j * N + x]);
     a.Q[(long long)p * N + x] = q;
     if (a.loss == WDL_LOSS_WASSERSTEIN) continue;  // set by wasserstein_loss
     const float o = a.obs[(long long)p * N + x];
@@ -186,7 +186,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_backward(SolveArgs a, int t) {
         G,
         [&](int gi) {
           Three r;  // n, v_i, b^s = q^s / phi^s
-          r.z = vld<VW>(Qs + gi) / vld<VW>(Fs + gi);
+          r.z = fdiv(vld<VW>(Qs + gi), vld<VW>(Fs + gi));
           if (last) {
             r.x = vld<VW>(G0 + gi);
             r.y = vsplat<VW>(0.f);
@@ -219,8 +219,9 @@ __global__ __launch_bounds__(kThreads, 4) void k_backward(SolveArgs a, int t) {
           V t = vsplat<VW>(0.f);
           if (ok) {
             const V c = vld<VW>(buf + li);
-            vst<VW>(GDp + gi, r.y + c / r.x);
-            t = -r.z * c / (r.x * r.x);
+            const V cq = fdiv(c, r.x);
+            vst<VW>(GDp + gi, r.y + cq);
+            t = fdiv(-r.z * cq, r.x);
           }
           vst<VW>(buf + li, t);
         });
@@ -230,7 +231,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_backward(SolveArgs a, int t) {
     px_pass<VW>(
         G, [&](int gi) { return vld<VW>(Fm + gi); },
         [&](int li, int gi, bool ok, V f) {
-          if (ok) vst<VW>(Vc + img + gi, vld<VW>(buf + li) / f);
+          if (ok) vst<VW>(Vc + img + gi, fdiv(vld<VW>(buf + li), f));
         });
   } else {
     const float* Rp = (s & 1) ? a.R0 : a.R1;
@@ -265,7 +266,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_backward(SolveArgs a, int t) {
             dot += vdot_d(lf, r.x);
             sf += vsum_d(lf);
             sg += vsum_d(r.x);
-            u = (lpi * r.x - r.y) / r.z;
+            u = fdiv(lpi * r.x - r.y, r.z);
           }
           vst<VW>(buf + li, u);
         });
@@ -285,13 +286,13 @@ __global__ __launch_bounds__(kThreads, 4) void k_backward(SolveArgs a, int t) {
           return r;
         },
         [&](int li, int gi, bool ok, const Two& r) {
-          vst<VW>(buf + li, ok ? vld<VW>(buf + li) * (-r.y / (r.x * r.x)) : vsplat<VW>(0.f));
+          vst<VW>(buf + li, ok ? vld<VW>(buf + li) * fdiv(fdiv(-r.y, r.x), r.x) : vsplat<VW>(0.f));
         });
     ConvPrec<BF3>::conv(buf, tabs, G);
     const float* Qm = a.Qh + (long long)(s - 2) * a.qstage + (long long)p * N;  // q^{s-1}
     const float* Fm = a.Fh + (long long)(s - 2) * a.stage + img;                // phi^{s-1}
     px_pass<VW>(
-        G, [&](int gi) { return vld<VW>(Qm + gi) / vld<VW>(Fm + gi); },  // b^{s-1}
+        G, [&](int gi) { return fdiv(vld<VW>(Qm + gi), vld<VW>(Fm + gi)); },  // b^{s-1}
         [&](int li, int gi, bool ok, V b) {
           if (ok) vst<VW>(Rc + img + gi, vld<VW>(buf + li) * b);
         });

@@ -147,7 +147,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_tfwd1(SolveArgs a, int s) {
           return r;
         },
         [&](int li, int, bool ok, const Two& r) {
-          vst<VW>(sm.buf + li, ok ? r.q / r.f : vsplat<VW>(0.f));
+          vst<VW>(sm.buf + li, ok ? fdiv(r.q, r.f) : vsplat<VW>(0.f));
         });
   }
   tile_conv(sm.buf, sm.tabs, T);
@@ -184,7 +184,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_tfwd2(SolveArgs a, int s) {
         return r;
       },
       [&](int li, int, bool ok, const Two& r) {
-        vst<VW>(sm.buf + li, ok ? r.a / r.c : vsplat<VW>(0.f));
+        vst<VW>(sm.buf + li, ok ? fdiv(r.a, r.c) : vsplat<VW>(0.f));
       });
   tile_conv(sm.buf, sm.tabs, T);
   float* Fd = a.Fh + (long long)(s - 1) * a.stage + img;
@@ -235,7 +235,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_tbwd1(SolveArgs a, int t) {
         T, tx, ty,
         [&](int gi) {
           Three r;  // n, v_i, b^s = q^s / phi^s
-          r.z = vld<VW>(Qs + gi) / vld<VW>(Fs + gi);
+          r.z = fdiv(vld<VW>(Qs + gi), vld<VW>(Fs + gi));
           if (last) {
             r.x = vld<VW>(G0 + gi);
             r.y = vsplat<VW>(0.f);
@@ -268,8 +268,9 @@ __global__ __launch_bounds__(kThreads, 4) void k_tbwd1(SolveArgs a, int t) {
         [&](int li, int gi, bool ok, const Three& r) {
           if (!ok) return;
           const V c = vld<VW>(sm.buf + li);
-          vst<VW>(GDp + gi, r.y + c / r.x);
-          if (s > 1) vst<VW>(X + gi, -r.z * c / (r.x * r.x));
+          const V cq = fdiv(c, r.x);
+          vst<VW>(GDp + gi, r.y + cq);
+          if (s > 1) vst<VW>(X + gi, fdiv(-r.z * cq, r.x));
         });
   } else {
     const float* Rp = (s & 1) ? a.R0 : a.R1;
@@ -306,7 +307,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_tbwd1(SolveArgs a, int t) {
         V u = vsplat<VW>(0.f);
         if (ok) {
           if (tile_inner(T, li)) acc_dot(r);
-          u = (lpi * r.x - r.y) / r.z;
+          u = fdiv(lpi * r.x - r.y, r.z);
         }
         vst<VW>(sm.buf + li, u);
       });
@@ -329,7 +330,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_tbwd1(SolveArgs a, int t) {
           return r;
         },
         [&](int li, int gi, bool ok, const Two& r) {
-          if (ok) vst<VW>(X + gi, vld<VW>(sm.buf + li) * (-r.y / (r.x * r.x)));
+          if (ok) vst<VW>(X + gi, vld<VW>(sm.buf + li) * fdiv(fdiv(-r.y, r.x), r.x));
         });
   }
 }
@@ -361,13 +362,13 @@ __global__ __launch_bounds__(kThreads, 4) void k_tbwd2(SolveArgs a, int t) {
     tile_inner_pass<VW>(
         T, tx, ty, [&](int gi) { return vld<VW>(Fm + gi); },
         [&](int li, int gi, bool ok, V f) {
-          if (ok) vst<VW>(Vc + gi, vld<VW>(sm.buf + li) / f);
+          if (ok) vst<VW>(Vc + gi, fdiv(vld<VW>(sm.buf + li), f));
         });
   } else {
     float* Rc = ((s & 1) ? a.R1 : a.R0) + img;
     const float* Qm = a.Qh + (long long)(s - 2) * a.qstage + (long long)p * N;  // q^{s-1}
     tile_inner_pass<VW>(
-        T, tx, ty, [&](int gi) { return vld<VW>(Qm + gi) / vld<VW>(Fm + gi); },  // b^{s-1}
+        T, tx, ty, [&](int gi) { return fdiv(vld<VW>(Qm + gi), vld<VW>(Fm + gi)); },  // b^{s-1}
         [&](int li, int gi, bool ok, V b) {
           if (ok) vst<VW>(Rc + gi, vld<VW>(sm.buf + li) * b);
         });
