@@ -9,7 +9,8 @@
 //   (log domain: Bh [L][P][K][N] FP64, log b^s)
 //   Ch  [L][P][K][N]  Ch[s-1] = c^{s-1} = K b^{s-1}  history conv_b
 //   Fh  [L][P][K][N]  Fh[s-1] = phi^s = K a^s        history phi
-//   V0/V1, R0/R1 [P][K][N]  ping-pong v (atom chain) and r (weight chain)
+//   V0/V1, R0/R1 [P][K][N]  ping-pong v (atom chain) and r (weight chain); during
+//                     the forward V0/V1 carry safe_log phi^s between stages
 //   GD  [P][K][N]  resultDic per histogram;  GL [P][K] (FP64) resultLa
 //   Q, G0 [P][N]   barycenter q^L and the loss gradient at q^L
 //
@@ -49,7 +50,10 @@ __global__ __launch_bounds__(kThreads, 4) void k_forward(SolveArgs a, int s) {
         G, [&](int gi) { return bi ? vld<VW>(bi + gi) : vsplat<VW>(1.f); },
         [&](int li, int gi, bool ok, V b) { vst<VW>(buf + li, ok ? b : vsplat<VW>(0.f)); });
   } else {
-    const float* Fp = a.Fh + (long long)(s - 2) * a.stage + hbase;  // phi^{s-1}
+    // safe_log phi_j^{s-1} of every atom j, written by stage s-1 (each
+    // workgroup logs its own phi once instead of all K of them)
+    const float* Lf = (((s - 1) & 1) ? a.V1 : a.V0) + hbase;
+    const float* Fi = a.Fh + (long long)(s - 2) * a.stage + img;  // phi_i^{s-1}
     float* Qd = i == 0 ? a.Qh + (long long)(s - 2) * a.qstage + (long long)p * N : nullptr;
     const float* lam = a.lam + (long long)p * K;
     struct LdF { V lq, fi; };
@@ -58,12 +62,8 @@ __global__ __launch_bounds__(kThreads, 4) void k_forward(SolveArgs a, int s) {
         [&](int gi) {
           LdF r;
           r.lq = vsplat<VW>(0.f);
-          r.fi = vsplat<VW>(1.f);
-          for (int j = 0; j < K; ++j) {
-            const V f = vld<VW>(Fp + (long long)j * N + gi);
-            r.lq += lam[j] * safe_logf(f);
-            if (j == i) r.fi = f;
-          }
+          for (int j = 0; j < K; ++j) r.lq += lam[j] * vld<VW>(Lf + (long long)j * N + gi);
+          r.fi = vld<VW>(Fi + gi);
           return r;
         },
         [&](int li, int gi, bool ok, const LdF& r) {
@@ -94,10 +94,16 @@ __global__ __launch_bounds__(kThreads, 4) void k_forward(SolveArgs a, int s) {
   }
   ConvPrec<BF3>::conv(buf, tabs, G);
   float* Fd = a.Fh + (long long)(s - 1) * a.stage + img;
+  // the next stage's geometric mean reads safe_log phi^s (V0 / V1 are free
+  // until the backward writes them; k_final logs phi^L itself)
+  float* Ld = s < steps_of(a, p) ? ((s & 1) ? a.V1 : a.V0) + img : nullptr;
   px_pass<VW>(
       G, [&](int gi) { return 0; },
       [&](int li, int gi, bool ok, int) {
-        if (ok) vst<VW>(Fd + gi, vld<VW>(buf + li));
+        if (!ok) return;
+        const V f = vld<VW>(buf + li);
+        vst<VW>(Fd + gi, f);
+        if (Ld) vst<VW>(Ld + gi, safe_logf(f));
       });
 }
 
