@@ -307,10 +307,14 @@ __device__ inline double vdot_d(floatx4 a, floatx4 b) {
 }
 
 constexpr int kUnroll = 4;
+// px_pass's default: measured on the resident 128x128 stages, 2 groups per thread
+// in flight beat 1, 4 and 8 (more registers in flight slowed the backward stage).
+constexpr int kUnrollPx = 2;
 
 // load(gi) -> any value type; apply(li, gi, valid, value).  li / gi are the
-// LDS and pixel indices of the group's first pixel.
-template <int VW, class Load, class Apply>
+// LDS and pixel indices of the group's first pixel.  U groups per thread have
+// their loads in flight together (passes that load few arrays take more).
+template <int VW, int U = kUnrollPx, class Load, class Apply>
 __device__ inline void px_pass(const ConvGeom& G, Load&& load, Apply&& apply) {
   const int q = G.Wp / VW;
   const int total = G.Hp * q;
@@ -321,12 +325,12 @@ __device__ inline void px_pass(const ConvGeom& G, Load&& load, Apply&& apply) {
   const int drow = kThreads / q, dcol = kThreads - drow * q;
   int row = threadIdx.x / q, col = threadIdx.x - row * q;
   int lrow = row * G.S1 + col * VW, grow = row * G.W + col * VW;
-  for (int base = threadIdx.x; base < total; base += kThreads * kUnroll) {
-    V v[kUnroll];
-    int li[kUnroll], gi[kUnroll];
-    bool ok[kUnroll];
+  for (int base = threadIdx.x; base < total; base += kThreads * U) {
+    V v[U];
+    int li[U], gi[U];
+    bool ok[U];
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
+    for (int u = 0; u < U; ++u) {
       li[u] = lrow;
       gi[u] = grow;
       ok[u] = row < G.H && col * VW < G.W;
@@ -343,7 +347,7 @@ __device__ inline void px_pass(const ConvGeom& G, Load&& load, Apply&& apply) {
       }
     }
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u)
+    for (int u = 0; u < U; ++u)
       if (base + u * kThreads < total) apply(li[u], gi[u], ok[u], v[u]);
   }
 }
