@@ -296,6 +296,39 @@ def cpu_baseline(c, gpu, budget_s=15.0):
     return out, err, refs
 
 
+def evaluate_leg(s, c, solve_ms, reps=5):
+    """evaluate_both (inverseWasserstein.h:986-1145) around the batched solve:
+    the reparametrisation, loss sum, projections and chain rule on the device.
+    Timed with x and g resident in HBM (wdl_evaluate_both_device) and as the
+    lbfgs callback with x and g in host memory (PCIe-inclusive)."""
+    P, K, N = c["P"], c["K"], c["n"] * c["n"]
+    scale = (N / P) / c["scale_factor"]
+    # x as regress_both builds it (inverseWasserstein.h:389-401): log weights
+    # times the dictionary scale, then log atoms
+    x = np.concatenate([np.log(np.asarray(c["weights"], np.float64)).ravel() * scale,
+                        np.log(np.asarray(c["atoms"], np.float64)).ravel()])
+    s.set_scale(scale, True)
+    dx = _hip.DeviceArray.from_host(x, np.float64)
+    dg = _hip.DeviceArray(x.shape, np.float64)
+    s.evaluate_both_device(dx.ptr, dg.ptr)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        f = s.evaluate_both_device(dx.ptr, dg.ptr)
+    dev_ms = (time.perf_counter() - t0) / reps * 1e3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fh, _g = s.evaluate_both(x)
+    host_ms = (time.perf_counter() - t0) / reps * 1e3
+    dx.free()
+    dg.free()
+    return {"device_ms": round(dev_ms, 3), "callback_ms": round(host_ms, 3),
+            "solve_ms": round(solve_ms, 3), "f": f, "f_callback": fh,
+            "pcie_bytes_per_call": int(2 * x.nbytes),
+            "what": "one evaluation: reparametrisation + batched solve + loss sum, projections "
+                    "and chain rule, all on the device; device_ms with x and g in HBM, "
+                    "callback_ms with x and g in host memory"}
+
+
 def cpu_model() -> str:
     try:
         with open("/proc/cpuinfo") as f:
@@ -530,6 +563,8 @@ def main():
         "batched_over_plain": (round(value / plain["value"], 2) if plain else None),
         "roofline": roofline,
     }
+    line["evaluate_both"] = evaluate_leg(s, c, total_ms / args.steps)
+    log("evaluate_both: %s" % json.dumps(line["evaluate_both"]))
     if not args.no_projection and args.projection_requests > 0:
         line["projection"] = projection_leg(ctx, args.projection_requests)
         log("projection: %s iters/s" % line["projection"]["value"])

@@ -216,7 +216,17 @@ int wdl_gradient_both(wdl_solver* s, const double* atoms, const double* lambdas,
  * as the lbfgs_evaluate_t.  scale_dictionary and exp_weight play the roles
  * of WassersteinRegression::scaleDictionary and exp_weight. */
 int wdl_solver_set_scale(wdl_solver* s, double scale_dictionary, int exp_weight);
+/* The reparametrisation (:998-1031), the batched solve and the loss sum,
+ * simplex-tangent projections and chain rule (:1089-1136) all run on the
+ * device; x is copied in and g out (P*K + K*N doubles each), nothing else
+ * crosses PCIe.  Returns f, NaN on error. */
 double wdl_evaluate_both(void* instance, const double* x, double* g, int n, double step);
+/* evaluate_both with x and g in device memory (FP64, P*K weights then K*N
+ * atoms, ordered on the context stream); *f (host) receives the objective.
+ * Synchronous.  No counterpart in the reference, whose x and g are host
+ * arrays of lbfgs (lbfgs.h:378-384); for callers that keep the optimiser's
+ * state on the device. */
+int wdl_evaluate_both_device(wdl_solver* s, const double* d_x, double* d_g, double* f);
 
 #ifdef __cplusplus
 }

@@ -190,6 +190,42 @@ def test_evaluate_both_config_shape(ctx):
     assert rel(gr[P * K:], go[P * K:]) < TOL
 
 
+@pytest.mark.parametrize("flags,expw", [(0, True), (0, False), (wdl.FLAG_LOG_DOMAIN, True),
+                                        (wdl.FLAG_WEIGHTS_ONLY, True)])
+def test_evaluate_both_device_entry(ctx, flags, expw):
+    # x and g in device memory (wdl_evaluate_both_device).  The host callback
+    # runs the same device path plus the copies of x and g, so the two agree
+    # to the bit; both against the FP64 oracle's evaluate_both within TOL.
+    from wassersteindictionarylearning_amd import _hip
+    c = synthetic.make_problem("plain_small", P=3)
+    n, P, K, L, g = c["n"], c["P"], c["K"], c["L"], c["gamma"]
+    N = n * n
+    scale = (N / P) / c["scale_factor"]
+    print("evaluate_both_device flags=%d exp_weight=%d" % (flags, expw), flush=True)
+    x = O.encode_variables(c["weights"], c["atoms"], scale, expw)
+    s = wdl.SinkhornSolver(ctx, n, n, K, P, L, g, c["loss"], flags)
+    s.set_observed(c["obs"])
+    s.set_scale(scale, expw)
+    f_host, g_host = s.evaluate_both(x)
+    dx = _hip.DeviceArray.from_host(x, np.float64)
+    dg = _hip.DeviceArray(x.shape, np.float64)
+    f_dev = s.evaluate_both_device(dx.ptr, dg.ptr)
+    g_dev = dg.download()
+    dx.free()
+    dg.free()
+    s.close()
+    assert f_dev == f_host
+    np.testing.assert_array_equal(g_dev, g_host)
+    fo, go, _ = O.evaluate_both(x, c["obs"], n, n, g, K, P, L, c["loss"], scale, expw,
+                                log_domain=bool(flags & wdl.FLAG_LOG_DOMAIN))
+    assert abs(f_dev - fo) <= TOL * abs(fo)
+    assert rel(g_dev[:P * K], go[:P * K]) < TOL
+    if flags & wdl.FLAG_WEIGHTS_ONLY:
+        assert not np.any(g_dev[P * K:])  # no atom gradients in a weights-only solve
+    else:
+        assert rel(g_dev[P * K:], go[P * K:]) < TOL
+
+
 @pytest.mark.parametrize("flags", [0, wdl.FLAG_LOG_DOMAIN])
 def test_uneven_iterations_and_weights_only(ctx, flags):
     # Each histogram runs its own number of Sinkhorn iterations (and as many

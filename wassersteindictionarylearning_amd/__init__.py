@@ -70,6 +70,7 @@ _SIGS = {
     "wdl_gradient_both": (_i, [_vp, _d, _d, _d, _d, _d, _d]),
     "wdl_solver_set_scale": (_i, [_vp, ctypes.c_double, _i]),
     "wdl_evaluate_both": (ctypes.c_double, [_vp, _d, _d, _i, ctypes.c_double]),
+    "wdl_evaluate_both_device": (_i, [_vp, _vp, _vp, ctypes.POINTER(ctypes.c_double)]),
     "wdl_solver_set_slot": (_i, [_vp, _i, _d, _d, _i, ctypes.c_double]),
     "wdl_solver_projection_round": (_i, [_vp]),
     "wdl_solver_get_slot": (_i, [_vp, _i, _d, _d]),
@@ -322,6 +323,14 @@ class SinkhornSolver:
             if err:
                 raise WdlError("wdl_evaluate_both: " + err)
         return f, g
+
+    def evaluate_both_device(self, d_x: int, d_g: int) -> float:
+        """evaluate_both with x and g in device memory (FP64 device pointers,
+        P*K + K*N values each); returns f."""
+        f = ctypes.c_double()
+        _check(lib().wdl_evaluate_both_device(self.handle, d_x, d_g, ctypes.byref(f)),
+               "wdl_evaluate_both_device")
+        return f.value
 
     def set_slot(self, slot, observed=None, lambdas=None, iterations=0, step=0.0):
         o = None if observed is None else _f64(observed)

@@ -174,6 +174,16 @@ int exact_backward(const SolveArgs& a, hipStream_t st, int t);
 int wasserstein_loss(const SolveArgs& a, hipStream_t st);
 constexpr int kXMaxLen = 1024;  // longest line of the exact-shift passes
 
+// evaluate_both around the batched solve, on the device (wdl_evaluate.hip):
+// x -> normalised weights / atoms (FP64 w64, a64 and the solve's FP32 copies),
+// and the loss sum, projections and chain rule into g and *f (device).
+int evaluate_prologue(const double* x, double* w64, double* a64, float* lam, float* A,
+                      float* logA, int P, int K, long long N, double scale, int expw,
+                      hipStream_t st);
+int evaluate_epilogue(const double* GL, const double* GDsum, const double* lossPart,
+                      const double* w64, const double* a64, double* g, double* f, int P, int K,
+                      long long N, double scale, int expw, int weightsOnly, hipStream_t st);
+
 // Sum over the workgroup, result valid in thread 0.
 __device__ inline double block_sum(double v) {
   __shared__ double red[kWaves];

@@ -709,7 +709,7 @@ struct wdl_solver {
   int D = 0;
   DevBuf tab, A, lam, obs, Bh, Qh, Ch, Fh, V0, V1, R0, R1, GD, GL, Q, G0, lossPart, bStore,
       bTemp, GDsum, logA, events, Lp, X0, X1, XS0, XS1, GLt, gsumPart, Mt, eta, xa, xb, xt, sga,
-      sgb, sgt, wla, wlb, wlc, wlt;
+      sgb, sgt, wla, wlb, wlc, wlt, evX, evG, evW, evA, evF;
   bool tiled = false;  // 64 x 64 tiles with a halo (a side above kMaxSide)
   bool exact = false;  // log domain with the exact-shift line passes (WDL_FLAG_EXACT_LSE)
   TileGeom tg{};
@@ -937,6 +937,7 @@ static long long footprint(int W, int H, int K, int P, int L, int flags) {
   if (flags & WDL_FLAG_LOG_DOMAIN) f += (long long)K * N * 4;  // log atoms
   f += kTabLen * 4;
   f += 4LL * P * N * 8;  // the Wasserstein loss's log scalings and convolution buffers
+  f += (3LL * P * K + 3LL * K * N + 1) * 8;  // evaluate_both: x, g, weights, atoms (FP64), f
   if (use_exact(flags)) f += 3LL * 2 * PKN * (8 + 1);  // FP64 scratch images + sign bytes
   if (use_tiles(W, H, flags)) {  // X0, X1 (+ sign bytes), per-tile dots
     const long long nt = (long long)((W + kTile - 1) / kTile) * ((H + kTile - 1) / kTile);
@@ -1053,6 +1054,11 @@ int wdl_solver_create(wdl_ctx* ctx, int W, int H, int K, int P, int L, double ga
       {&s->wlb, (size_t)P * N * sizeof(double)},
       {&s->wlc, (size_t)P * N * sizeof(double)},
       {&s->wlt, (size_t)P * N * sizeof(double)},
+      {&s->evX, (size_t)(P * K + K * N) * sizeof(double)},
+      {&s->evG, (size_t)(P * K + K * N) * sizeof(double)},
+      {&s->evW, (size_t)P * K * sizeof(double)},
+      {&s->evA, (size_t)K * N * sizeof(double)},
+      {&s->evF, sizeof(double)},
   };
   for (auto& al : allocs) {
     if ((st = al.b->ensure(al.n))) {
@@ -1106,7 +1112,7 @@ int wdl_solver_destroy(wdl_solver* s) {
                     &s->G0,  &s->lossPart,    &s->bStore,      &s->bTemp,    &s->GDsum,
                     &s->logA, &s->events, &s->Lp, &s->X0, &s->X1, &s->XS0, &s->XS1, &s->GLt, &s->Mt, &s->gsumPart, &s->eta, &s->xa, &s->xb, &s->xt, &s->sga, &s->sgb,
                     &s->sgt, &s->wla, &s->wlb, &s->wlc,
-                    &s->wlt};
+                    &s->wlt, &s->evX, &s->evG, &s->evW, &s->evA, &s->evF};
   for (DevBuf* b : bufs) b->release();
   delete s;
   return WDL_OK;
@@ -1499,6 +1505,37 @@ int wdl_solver_set_scale(wdl_solver* s, double scale_dictionary, int exp_weight)
   return WDL_OK;
 }
 
+// evaluate_both with x and g in device memory: prologue, batched solve and
+// epilogue on the context stream; *f (host) after a synchronisation.
+static int evaluate_device(wdl_solver* s, const double* dx, double* dg, double* f) {
+  WDL_CHECK(s->observed, WDL_ERR_STATE, "evaluate_both: set_observed must be called first");
+  const int P = s->P, K = s->K;
+  const long long N = s->N;
+  hipStream_t st = s->ctx->stream;
+  WDL_HIP(hipSetDevice(s->ctx->device));
+  const bool logd = s->flags & WDL_FLAG_LOG_DOMAIN;
+  WDL_CHECK(evaluate_prologue(dx, s->evW.as<double>(), s->evA.as<double>(), s->lam.as<float>(),
+                              s->A.as<float>(), logd ? s->logA.as<float>() : nullptr, P, K, N,
+                              s->scaleDic, s->expWeight, st) == 0,
+            WDL_ERR_HIP, "evaluate_both: prologue launch failed");
+  s->problem = true;
+  int rc = wdl_solver_run(s, WDL_RUN_BATCHED);
+  if (rc) return rc;
+  WDL_CHECK(evaluate_epilogue(s->GL.as<double>(), s->GDsum.as<double>(), s->lossPart.as<double>(),
+                              s->evW.as<double>(), s->evA.as<double>(), dg, s->evF.as<double>(), P,
+                              K, N, s->scaleDic, s->expWeight,
+                              (s->flags & WDL_FLAG_WEIGHTS_ONLY) ? 1 : 0, st) == 0,
+            WDL_ERR_HIP, "evaluate_both: epilogue launch failed");
+  WDL_HIP(hipMemcpyAsync(f, s->evF.p, sizeof(double), hipMemcpyDeviceToHost, st));
+  WDL_HIP(hipStreamSynchronize(st));
+  return WDL_OK;
+}
+
+int wdl_evaluate_both_device(wdl_solver* s, const double* d_x, double* d_g, double* f) {
+  WDL_CHECK(s && d_x && d_g && f, WDL_ERR_ARG, "wdl_evaluate_both_device: NULL argument");
+  return evaluate_device(s, d_x, d_g, f);
+}
+
 double wdl_evaluate_both(void* instance, const double* x, double* g, int n, double step) {
   (void)step;
   const double nan = std::numeric_limits<double>::quiet_NaN();
@@ -1507,51 +1544,29 @@ double wdl_evaluate_both(void* instance, const double* x, double* g, int n, doub
     set_error("wdl_evaluate_both: NULL argument");
     return nan;
   }
-  const int P = s->P, K = s->K;
-  const long long N = s->N;
-  if ((long long)n != (long long)P * K + (long long)K * N) {
+  const long long nn = (long long)s->P * s->K + (long long)s->K * s->N;
+  if ((long long)n != nn) {
     set_error("wdl_evaluate_both: n != P*K + K*N");
     return nan;
   }
-  const double scale = s->scaleDic;
-  // Back to the original domain and normalised (inverseWasserstein.h:998-1031).
-  std::vector<double> v2(x, x + n);
-  for (int e = 0; e < P * K; ++e) v2[e] = s->expWeight ? std::exp(x[e] / scale) : x[e] / scale;
-  for (long long e = 0; e < (long long)K * N; ++e) v2[P * K + e] = std::exp(x[P * K + e]);
-  for (int p = 0; p < P; ++p) {
-    double t = 0;
-    for (int i = 0; i < K; ++i) t += v2[p * K + i];
-    for (int i = 0; i < K; ++i) v2[p * K + i] /= t;
-  }
-  for (int i = 0; i < K; ++i) {
-    double t = 0;
-    for (long long j = 0; j < N; ++j) t += v2[P * K + i * N + j];
-    for (long long j = 0; j < N; ++j) v2[P * K + i * N + j] /= t;
-  }
-  std::vector<double> lossv(P), gl((size_t)P * K), gd((size_t)K * N);
-  if (wdl_gradient_both(s, &v2[P * K], &v2[0], lossv.data(), gl.data(), gd.data(), nullptr))
+  // Only x and g cross PCIe; the reparametrisation (:998-1031), the batched
+  // solve and the loss sum, projections and chain rule (:1089-1136) run on
+  // the device.
+  hipStream_t st = s->ctx->stream;
+  double f = nan;
+  if (hipSetDevice(s->ctx->device) != hipSuccess ||
+      hipMemcpyAsync(s->evX.p, x, (size_t)nn * sizeof(double), hipMemcpyHostToDevice, st) !=
+          hipSuccess) {
+    set_error("wdl_evaluate_both: copy of x failed");
     return nan;
-  double f = 0;
-  for (int p = 0; p < P; ++p) f += lossv[p];  // :1089-1091
-  // Simplex-tangent projections (:1095-1108); the atom projection is linear in
-  // the per-histogram gradients, so it is applied to their sum (:1110-1112).
-  for (int p = 0; p < P; ++p) {
-    double d = 0;
-    for (int i = 0; i < K; ++i) d += v2[p * K + i] * gl[(size_t)p * K + i];
-    for (int i = 0; i < K; ++i) g[p * K + i] = gl[(size_t)p * K + i] - d;
   }
-  for (int i = 0; i < K; ++i) {
-    double d = 0;
-    for (long long j = 0; j < N; ++j) d += v2[P * K + i * N + j] * gd[(size_t)i * N + j];
-    for (long long j = 0; j < N; ++j) g[P * K + i * N + j] = gd[(size_t)i * N + j] - d;
+  if (evaluate_device(s, s->evX.as<double>(), s->evG.as<double>(), &f)) return nan;
+  if (hipMemcpyAsync(g, s->evG.p, (size_t)nn * sizeof(double), hipMemcpyDeviceToHost, st) !=
+          hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess) {
+    set_error("wdl_evaluate_both: copy of g failed");
+    return nan;
   }
-  // Chain rule of the exp / scaling reparametrisation (:1125-1136).
-  if (s->expWeight) {
-    for (int e = 0; e < n; ++e) g[e] *= v2[e];
-  } else {
-    for (long long e = P * K; e < n; ++e) g[e] *= v2[e];
-  }
-  for (int e = 0; e < P * K; ++e) g[e] /= scale;
   return f;
 }
 
