@@ -52,8 +52,48 @@ __device__ inline void load_tables(float* tabs, const float* __restrict__ gtab) 
   }
 }
 
+// The wave's index in the workgroup as a scalar.  The compiler cannot prove
+// threadIdx.x / 64 uniform, and with a per-lane wave index every condition
+// derived from it (which output unit, how many row blocks) becomes an
+// exec-mask branch around each MFMA, with a wait on all LDS reads before it.
+__device__ inline int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+
 __device__ inline floatx4 mfma4(float a, float b, floatx4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// One output unit: NRB row blocks of 16 rows at arow (lane's row c, offset
+// 4g), contraction chunks kc_lo..kc_hi.  The next chunk's operands are read
+// from LDS while the current chunk's 16 MFMAs issue; no condition inside the
+// loop, so the MFMAs go back to back.
+template <int NRB>
+__device__ inline void gemm_unit(const float* __restrict__ arow, int sin,
+                                 const float* __restrict__ tabs, int abase, int kc_lo, int kc_hi,
+                                 floatx4 (&acc)[4]) {
+  // B[k][j] = Kg(kk + 4g + t, c0 + c) = tab[abase + kk + t], abase = kTabOff + 4g - c0 - c;
+  // A[i][k] = IN[r0 + 16rb + c][kk + 4g + t]
+  auto load = [&](int kc, floatx4 (&a)[NRB], floatx4& b) {
+    const int kk = kc << 4;
+    const int ai = abase + kk;
+    b = *reinterpret_cast<const floatx4*>(tabs + (ai & 3) * kTabLen + (ai & ~3));
+#pragma unroll
+    for (int rb = 0; rb < NRB; ++rb)
+      a[rb] = *reinterpret_cast<const floatx4*>(arow + rb * 16 * sin + kk);
+  };
+  floatx4 av[NRB], bv;
+  load(kc_lo, av, bv);
+  for (int kc = kc_lo; kc <= kc_hi; ++kc) {
+    floatx4 an[NRB], bn;
+    load(min(kc + 1, kc_hi), an, bn);  // the last iteration re-reads its own chunk
+    // t outer, row block inner: consecutive MFMAs write different accumulators.
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int rb = 0; rb < NRB; ++rb) acc[rb] = mfma4(av[rb][t], bv[t], acc[rb]);
+#pragma unroll
+    for (int rb = 0; rb < NRB; ++rb) av[rb] = an[rb];
+    bv = bn;
+  }
 }
 
 // One 1-D pass over an LDS matrix IN (nrows x ncontract, row stride sin):
@@ -66,7 +106,7 @@ __device__ inline floatx4 mfma4(float a, float b, floatx4 c) {
 __device__ inline void rows_gemm(const float* __restrict__ in, int sin, int nrows, int ncontract,
                                  int nout, int off, const float* __restrict__ tabs, int D,
                                  floatx4 (&acc)[kMaxUnits][4]) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = wave_id();
   const int c = lane & 15, g = lane >> 4;
   const int ncb = nout >> 4;
   const int nck = ncontract >> 4;
@@ -84,22 +124,12 @@ __device__ inline void rows_gemm(const float* __restrict__ in, int sin, int nrow
       const int kc_lo = lo > 0 ? (lo >> 4) : 0;
       const int kc_hi = min(nck - 1, (c0 + 15 + D) >> 4);
       const float* arow = in + (r0 + c) * sin + 4 * g;
-      for (int kc = kc_lo; kc <= kc_hi; ++kc) {
-        const int kk = kc << 4;
-        // B[k][j] = Kg(kk + 4g + t, c0 + c) = tab[kTabOff + kk + 4g + t - c0 - c]
-        const int a = kTabOff + kk + 4 * g - c0 - c;
-        const floatx4 bv = *reinterpret_cast<const floatx4*>(tabs + (a & 3) * kTabLen + (a & ~3));
-        // A[i][k] = IN[r0 + 16rb + c][kk + 4g + t]
-        floatx4 av[4];
-#pragma unroll
-        for (int rb = 0; rb < 4; ++rb)
-          if (rb < nrb) av[rb] = *reinterpret_cast<const floatx4*>(arow + rb * 16 * sin + kk);
-        // t outer, row block inner: consecutive MFMAs write different accumulators.
-#pragma unroll
-        for (int t = 0; t < 4; ++t)
-#pragma unroll
-          for (int rb = 0; rb < 4; ++rb)
-            if (rb < nrb) acc[uu][rb] = mfma4(av[rb][t], bv[t], acc[uu][rb]);
+      const int abase = kTabOff + 4 * g - c0 - c;
+      switch (nrb) {  // wave-uniform
+        case 4: gemm_unit<4>(arow, sin, tabs, abase, kc_lo, kc_hi, acc[uu]); break;
+        case 3: gemm_unit<3>(arow, sin, tabs, abase, kc_lo, kc_hi, acc[uu]); break;
+        case 2: gemm_unit<2>(arow, sin, tabs, abase, kc_lo, kc_hi, acc[uu]); break;
+        default: gemm_unit<1>(arow, sin, tabs, abase, kc_lo, kc_hi, acc[uu]); break;
       }
     }
   }
@@ -116,7 +146,7 @@ __device__ inline void rows_gemm(const float* __restrict__ in, int sin, int nrow
 // transposed: dst[j * sdst + r] = OUT[r][j].
 __device__ inline void store_transposed(float* dst, int sdst, int nrows, int ncols,
                                         const floatx4 (&acc)[kMaxUnits][4]) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = wave_id();
   const int c = lane & 15, g = lane >> 4;
   const int ncb = ncols >> 4;
   const int nunits = ncb * ((nrows + 63) >> 6);
@@ -169,7 +199,7 @@ __device__ inline void rows_gemm_bf3(const float* __restrict__ in, int sin, int 
                                      int ncontract, int nout, int off,
                                      const __bf16* __restrict__ tb, int D,
                                      floatx4 (&acc)[kMaxUnits][4]) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = wave_id();
   const int c = lane & 15, g = lane >> 4;
   const int ncb = nout >> 4;
   const int nck = ncontract >> 5;
@@ -269,7 +299,7 @@ template <> struct ConvPrec<true> {
 // accesses of a wave are 64 consecutive floats.  f(lds_index, pixel, valid).
 template <class F>
 __device__ inline void for_each_px(const ConvGeom& G, F&& f) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = wave_id();
   for (int row = wave; row < G.Hp; row += kWaves)
     for (int col = lane; col < G.Wp; col += 64)
       f(row * G.S1 + col, row * G.W + col, row < G.H && col < G.W);

@@ -66,7 +66,7 @@ __global__ __launch_bounds__(kXWaves * 64) void k_xlse_pass(
     const int* __restrict__ Lp, int p0, int ipp, int active_thr) {
   extern __shared__ __align__(16) double xs[];
   const int nch = (len + 15) >> 4;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = wave_id(), lane = threadIdx.x & 63;
   const int o = lane & 15, q4 = lane >> 4;
   const float igf = (float)inv_gamma;
   const float gam = (float)(1.0 / inv_gamma);
