@@ -327,6 +327,23 @@ __device__ inline typename Vec<VW>::T vsplat(float x) {
   return typename Vec<VW>::T(x);
 }
 
+// Visits x[j * stride] for j = 0..K-1 in ascending j as f(j, value), with the
+// loads of kAtomBatch atoms in flight together: K is a runtime value, and a
+// plain loop over it waits for each load before issuing the next.
+constexpr int kAtomBatch = 8;
+template <int VW, class F>
+__device__ inline void for_atoms(const float* __restrict__ x, long long stride, int K, F&& f) {
+  for (int j0 = 0; j0 < K; j0 += kAtomBatch) {
+    typename Vec<VW>::T v[kAtomBatch];
+#pragma unroll
+    for (int t = 0; t < kAtomBatch; ++t)
+      if (j0 + t < K) v[t] = vld<VW>(x + (long long)(j0 + t) * stride);
+#pragma unroll
+    for (int t = 0; t < kAtomBatch; ++t)
+      if (j0 + t < K) f(j0 + t, v[t]);
+  }
+}
+
 template <class F> __device__ inline float vmap(float x, F f) { return f(x); }
 template <class F> __device__ inline floatx4 vmap(floatx4 x, F f) {
   return floatx4{f(x.x), f(x.y), f(x.z), f(x.w)};

@@ -62,7 +62,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_forward(SolveArgs a, int s) {
         [&](int gi) {
           LdF r;
           r.lq = vsplat<VW>(0.f);
-          for (int j = 0; j < K; ++j) r.lq += lam[j] * vld<VW>(Lf + (long long)j * N + gi);
+          for_atoms<VW>(Lf + gi, N, K, [&](int j, V v) { r.lq += lam[j] * v; });
           r.fi = vld<VW>(Fi + gi);
           return r;
         },
@@ -199,11 +199,10 @@ __global__ __launch_bounds__(kThreads, 4) void k_backward(SolveArgs a, int t) {
           } else {
             r.x = vsplat<VW>(0.f);
             r.y = vsplat<VW>(0.f);
-            for (int j = 0; j < K; ++j) {
-              const V v = vld<VW>(Vp + hbase + (long long)j * N + gi);
+            for_atoms<VW>(Vp + hbase + gi, N, K, [&](int j, V v) {
               r.x += v;
               if (j == i) r.y = v;
-            }
+            });
           }
           return r;
         },
@@ -257,11 +256,10 @@ __global__ __launch_bounds__(kThreads, 4) void k_backward(SolveArgs a, int t) {
           } else {
             r.x = vsplat<VW>(0.f);
             r.y = vsplat<VW>(0.f);
-            for (int j = 0; j < K; ++j) {
-              const V v = vld<VW>(Rp + hbase + (long long)j * N + gi);
+            for_atoms<VW>(Rp + hbase + gi, N, K, [&](int j, V v) {
               r.x += v;
               if (j == i) r.y = v;
-            }
+            });
           }
           return r;
         },
