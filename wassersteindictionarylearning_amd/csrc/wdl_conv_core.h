@@ -327,21 +327,26 @@ __device__ inline typename Vec<VW>::T vsplat(float x) {
   return typename Vec<VW>::T(x);
 }
 
-// Visits x[j * stride] for j = 0..K-1 in ascending j as f(j, value), with the
-// loads of kAtomBatch atoms in flight together: K is a runtime value, and a
-// plain loop over it waits for each load before issuing the next.
+// Visits load(j) for j = 0..K-1 in ascending j as f(j, value), with the loads
+// of B atoms in flight together: K is a runtime value, and a plain loop over it
+// waits for each load before issuing the next.
 constexpr int kAtomBatch = 8;
-template <int VW, class F>
-__device__ inline void for_atoms(const float* __restrict__ x, long long stride, int K, F&& f) {
-  for (int j0 = 0; j0 < K; j0 += kAtomBatch) {
-    typename Vec<VW>::T v[kAtomBatch];
+template <int B = kAtomBatch, class Load, class F>
+__device__ inline void for_atoms_l(int K, Load&& load, F&& f) {
+  for (int j0 = 0; j0 < K; j0 += B) {
+    decltype(load(0)) v[B];
 #pragma unroll
-    for (int t = 0; t < kAtomBatch; ++t)
-      if (j0 + t < K) v[t] = vld<VW>(x + (long long)(j0 + t) * stride);
+    for (int t = 0; t < B; ++t)
+      if (j0 + t < K) v[t] = load(j0 + t);
 #pragma unroll
-    for (int t = 0; t < kAtomBatch; ++t)
+    for (int t = 0; t < B; ++t)
       if (j0 + t < K) f(j0 + t, v[t]);
   }
+}
+// The same over x[j * stride] (FP32 groups of VW pixels).
+template <int VW, class F>
+__device__ inline void for_atoms(const float* __restrict__ x, long long stride, int K, F&& f) {
+  for_atoms_l(K, [&](int j) { return vld<VW>(x + (long long)j * stride); }, f);
 }
 
 template <class F> __device__ inline float vmap(float x, F f) { return f(x); }

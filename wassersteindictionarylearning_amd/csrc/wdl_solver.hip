@@ -391,11 +391,12 @@ __global__ __launch_bounds__(kThreads, 4) void k_forward_log(SolveArgs a, int s)
           TwoD r;
           r.x = vd(vsplat<VW>(0.f));
           r.y = vd(vsplat<VW>(0.f));
-          for (int j = 0; j < K; ++j) {
-            const VD f = vldd<VW>(Fp + (long long)j * N + gi);
-            r.x += (double)lam[j] * f;
-            if (j == i) r.y = f;
-          }
+          for_atoms_l<4>(
+              K, [&](int j) { return vldd<VW>(Fp + (long long)j * N + gi); },
+              [&](int j, const VD& f) {
+                r.x += (double)lam[j] * f;
+                if (j == i) r.y = f;
+              });
           return r;
         },
         [&](int, int gi, int, const TwoD& r, bool second) {
@@ -516,11 +517,10 @@ __global__ __launch_bounds__(kThreads, 4) void k_backward_log(SolveArgs a, int t
           } else {
             r.x = vsplat<VW>(0.f);
             r.y = vsplat<VW>(0.f);
-            for (int j = 0; j < K; ++j) {
-              const V v = vld<VW>(Vp + hbase + (long long)j * N + gi);
+            for_atoms<VW>(Vp + hbase + gi, N, K, [&](int j, V v) {
               r.x -= v;  // n = -sum_i v_i (:764)
               if (j == i) r.y = v;
-            }
+            });
           }
           return r;
         },
@@ -570,11 +570,10 @@ __global__ __launch_bounds__(kThreads, 4) void k_backward_log(SolveArgs a, int t
       } else {
         r.x = vsplat<VW>(0.f);
         r.y = vsplat<VW>(0.f);
-        for (int j = 0; j < K; ++j) {
-          const V v = vld<VW>(Rp + hbase + (long long)j * N + gi);
+        for_atoms<VW>(Rp + hbase + gi, N, K, [&](int j, V v) {
           r.x -= v;  // g = -sum_i r_i (:808)
           if (j == i) r.y = v;
-        }
+        });
       }
       return r;
     };

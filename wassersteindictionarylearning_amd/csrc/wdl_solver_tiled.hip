@@ -111,7 +111,7 @@ __global__ __launch_bounds__(256) void k_tq(SolveArgs a, int s) {
   for (long long x = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * VW; x < N;
        x += (long long)gridDim.x * blockDim.x * VW) {
     V lq = vsplat<VW>(0.f);
-    for (int j = 0; j < K; ++j) lq += lam[j] * safe_logf(vld<VW>(F + (long long)j * N + x));
+    for_atoms<VW>(F + x, N, K, [&](int j, V f) { lq += lam[j] * safe_logf(f); });
     vst<VW>(Q + x, vexpf(lq));
   }
 }
@@ -242,11 +242,10 @@ __global__ __launch_bounds__(kThreads, 4) void k_tbwd1(SolveArgs a, int t) {
           } else {
             r.x = vsplat<VW>(0.f);
             r.y = vsplat<VW>(0.f);
-            for (int j = 0; j < K; ++j) {
-              const V v = vld<VW>(Vp + hbase + (long long)j * N + gi);
+            for_atoms<VW>(Vp + hbase + gi, N, K, [&](int j, V v) {
               r.x += v;
               if (j == i) r.y = v;
-            }
+            });
           }
           return r;
         },
@@ -287,11 +286,10 @@ __global__ __launch_bounds__(kThreads, 4) void k_tbwd1(SolveArgs a, int t) {
       } else {
         r.x = vsplat<VW>(0.f);
         r.y = vsplat<VW>(0.f);
-        for (int j = 0; j < K; ++j) {
-          const V v = vld<VW>(Rp + hbase + (long long)j * N + gi);
+        for_atoms<VW>(Rp + hbase + gi, N, K, [&](int j, V v) {
           r.x += v;
           if (j == i) r.y = v;
-        }
+        });
       }
       return r;
     };
@@ -447,11 +445,12 @@ __global__ __launch_bounds__(kThreads, 4) void k_tfwd1_log(SolveArgs a, int s) {
           TwoD r;
           r.x = vd(vsplat<VW>(0.f));
           r.y = vd(vsplat<VW>(0.f));
-          for (int j = 0; j < K; ++j) {
-            const VD f = vldd<VW>(Fp + (long long)j * N + gi);
-            r.x += (double)lam[j] * f;
-            if (j == i) r.y = f;
-          }
+          for_atoms_l<4>(
+              K, [&](int j) { return vldd<VW>(Fp + (long long)j * N + gi); },
+              [&](int j, const VD& f) {
+                r.x += (double)lam[j] * f;
+                if (j == i) r.y = f;
+              });
           return r;
         },
         [&](int li, int gi, const TwoD& r, bool second) {
@@ -559,11 +558,10 @@ __global__ __launch_bounds__(kThreads, 4) void k_tbwd1_log(SolveArgs a, int t) {
           } else {
             r.x = vsplat<VW>(0.f);
             r.y = vsplat<VW>(0.f);
-            for (int j = 0; j < K; ++j) {
-              const V v = vld<VW>(Vp + hbase + (long long)j * N + gi);
+            for_atoms<VW>(Vp + hbase + gi, N, K, [&](int j, V v) {
               r.x -= v;  // n = -sum_i v_i (:764)
               if (j == i) r.y = v;
-            }
+            });
           }
           return r;
         },
@@ -606,11 +604,10 @@ __global__ __launch_bounds__(kThreads, 4) void k_tbwd1_log(SolveArgs a, int t) {
       } else {
         r.x = vsplat<VW>(0.f);
         r.y = vsplat<VW>(0.f);
-        for (int j = 0; j < K; ++j) {
-          const V v = vld<VW>(Rp + hbase + (long long)j * N + gi);
+        for_atoms<VW>(Rp + hbase + gi, N, K, [&](int j, V v) {
           r.x -= v;  // g = -sum_i r_i (:808)
           if (j == i) r.y = v;
-        }
+        });
       }
       return r;
     };

@@ -269,7 +269,10 @@ __global__ void k_xf1(SolveArgs a, int s) {
       const double* Fp = (const double*)a.Fh + (long long)(s - 2) * a.stage +
                          (long long)r.p * a.K * a.N + r.x;
       double lq = 0;
-      for (int j = 0; j < a.K; ++j) lq += (double)a.lam[(long long)r.p * a.K + j] * Fp[j * a.N];
+      const float* lam = a.lam + (long long)r.p * a.K;
+      for_atoms_l<4>(
+          a.K, [&](int j) { return Fp[j * a.N]; },
+          [&](int j, double f) { lq += (double)lam[j] * f; });
       b = lq - Fp[(long long)r.i * a.N];
       ((double*)a.Bh)[(long long)(s - 2) * a.stage + r.img + r.x] = b;
     }
@@ -306,11 +309,12 @@ __global__ void k_xb1(SolveArgs a, int t, int nch) {
       acc = a.G0[(long long)r.p * N + r.x];
       if (chain == 1) acc *= a.Q[(long long)r.p * N + r.x];
     } else {
-      for (int j = 0; j < a.K; ++j) {
-        const float v = Pp[hb + j * N + r.x];
-        acc -= v;  // n = -sum v (:764), g = -sum r (:808)
-        if (j == r.i) own = v;
-      }
+      for_atoms_l(
+          a.K, [&](int j) { return Pp[hb + j * N + r.x]; },
+          [&](int j, float v) {
+            acc -= v;  // n = -sum v (:764), g = -sum r (:808)
+            if (j == r.i) own = v;
+          });
     }
     const float xv = lpi * acc + own;
     const long long h = (long long)(s - 1) * a.stage + r.img + r.x;
@@ -384,7 +388,7 @@ __global__ __launch_bounds__(kThreads) void k_xdot(SolveArgs a, int t) {
     if (last) {
       g = a.G0[(long long)p * N + x] * a.Q[(long long)p * N + x];
     } else {
-      for (int j = 0; j < a.K; ++j) g -= Rp[hb + j * N + x];
+      for_atoms_l(a.K, [&](int j) { return Rp[hb + j * N + x]; }, [&](int, float v) { g -= v; });
     }
     sfg += F[x] * (double)g;
     sf += F[x];
