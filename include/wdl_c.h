@@ -51,6 +51,8 @@ extern "C" {
 
 typedef struct wdl_ctx wdl_ctx;
 typedef struct wdl_gauss2d wdl_gauss2d;
+typedef struct wdl_gauss1d wdl_gauss1d;
+typedef struct wdl_gauss3d wdl_gauss3d;
 typedef struct wdl_solver wdl_solver;
 
 const char* wdl_last_error(void);
@@ -84,6 +86,29 @@ int wdl_gauss2d_log_convolve_signarray(wdl_gauss2d* k, const double* log_u,
                                        unsigned char* signs, double* out, int nvectors);
 /* FP32 device form of convolve (no equivalent in the reference; the batched path's operator). */
 int wdl_gauss2d_convolve_device(wdl_gauss2d* k, const float* d_u, float* d_out, int nvectors);
+
+/* ---- 1-D and 3-D KernelTypes --------------------------------------------------
+ * Replace Gaussian1DKernel (kernels.h:501-556) and Gaussian3DKernel
+ * (kernels.h:712-815), the operators of 1-D and volumetric histograms.  Same
+ * factor table, FP32 MFMA passes.  Lines longer than 128 values (W of the 1-D
+ * kernel, D of the 3-D one) need a band of at most 114 nonzero FP32 factors
+ * (gamma up to ~125); W, D <= 2048.  Larger W x H slices of the 3-D kernel take
+ * the tiled 2-D passes. */
+/* Gaussian1DKernel(gamma, W): kernels.h:503-511. */
+int wdl_gauss1d_create(wdl_ctx* ctx, int W, double gamma, wdl_gauss1d** out);
+int wdl_gauss1d_destroy(wdl_gauss1d* k);
+/* Gaussian1DKernel::convolve / convolveAdjoint (nvectors vectors of W): kernels.h:533-552. */
+int wdl_gauss1d_convolve(wdl_gauss1d* k, const double* u, double* out, int nvectors);
+/* FP32 device form (no equivalent in the reference). */
+int wdl_gauss1d_convolve_device(wdl_gauss1d* k, const float* d_u, float* d_out, int nvectors);
+/* Gaussian3DKernel(gamma, W, H, D): kernels.h:714-724.  Voxel (x, y, z) at
+ * z*W*H + y*W + x. */
+int wdl_gauss3d_create(wdl_ctx* ctx, int W, int H, int D, double gamma, wdl_gauss3d** out);
+int wdl_gauss3d_destroy(wdl_gauss3d* k);
+/* Gaussian3DKernel::convolve / convolveAdjoint: kernels.h:762-810. */
+int wdl_gauss3d_convolve(wdl_gauss3d* k, const double* u, double* out, int nvectors);
+/* FP32 device form (no equivalent in the reference). */
+int wdl_gauss3d_convolve_device(wdl_gauss3d* k, const float* d_u, float* d_out, int nvectors);
 
 /* ---- Sinkhorn barycenter + unrolled gradient, P histograms per call --------
  * Holds the state WassersteinRegression keeps per solve (b, conv_b, phi

@@ -74,6 +74,28 @@ def convolve(u: np.ndarray, W: int, H: int, gamma: float) -> np.ndarray:
 convolve_adjoint = convolve
 
 
+def convolve1d(u: np.ndarray, W: int, gamma: float) -> np.ndarray:
+    """Gaussian1DKernel::convolve, kernels.h:533-549 (table :508-510):
+    out[i*W + j] = sum_k kernel1d[|j-k|] u[i*W + k] for each of the nvectors
+    vectors; the adjoint (:550-552) is the same call."""
+    t = kernel_table(W, 1, gamma)
+    U = np.asarray(u, np.float64).reshape(-1, W)
+    return (U @ toeplitz(W, t)).reshape(np.shape(u))
+
+
+def convolve3d(u: np.ndarray, W: int, H: int, D: int, gamma: float) -> np.ndarray:
+    """Gaussian3DKernel::convolve, kernels.h:762-807 (table :719-723 over
+    max(W,H,D)): voxel (x, y, z) at z*W*H + y*W + x; pass x along rows, pass y
+    along columns of every z-slice (:768-789), then pass z (:791-803)."""
+    n = max(W, H, D)
+    d = np.arange(n, dtype=np.float64)
+    t = np.maximum(EPSILON, np.exp(-(d * d) / gamma))
+    KW, KH, KD = toeplitz(W, t), toeplitz(H, t), toeplitz(D, t)
+    U = np.asarray(u, np.float64).reshape(-1, D, H, W)
+    R = np.einsum("zk,nkhw->nzhw", KD, KH @ (U @ KW))
+    return R.reshape(np.shape(u))
+
+
 # ----------------------------------------------------------------------------
 # a3 / a4 / a5: log-domain convolution and the sign array
 # ----------------------------------------------------------------------------

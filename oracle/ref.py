@@ -31,6 +31,8 @@ def _lib(name: str):
             for fn in ("ref_convolve", "ref_sse_convolve", "ref_log_convolve"):
                 getattr(lib, fn).argtypes = [_i, _i, _f, _d, _d, _i]
             lib.ref_kernel_table.argtypes = [_i, _i, _f, _d]
+            lib.ref_convolve1d.argtypes = [_i, _f, _d, _d, _i]
+            lib.ref_convolve3d.argtypes = [_i, _i, _i, _f, _d, _d, _i]
             lib.ref_log_convolve_signarray.argtypes = [_i, _i, _f, _d, _u8, _d, _i]
             lib.ref_abs_log.argtypes = [_d, _u8, _d, _i]
             lib.ref_abs_exp.argtypes = [_d, _u8, _d, _i]
@@ -75,6 +77,20 @@ def _conv(fn, u, W, H, gamma):
 
 def convolve(u, W, H, gamma):
     return _conv("ref_convolve", u, W, H, gamma)
+
+
+def convolve1d(u, W, gamma):
+    u = _c(u)
+    out = np.zeros_like(u)
+    _lib("libwdlref.so").ref_convolve1d(W, gamma, _p(u), _p(out), u.size // W)
+    return out
+
+
+def convolve3d(u, W, H, D, gamma):
+    u = _c(u)
+    out = np.zeros_like(u)
+    _lib("libwdlref.so").ref_convolve3d(W, H, D, gamma, _p(u), _p(out), u.size // (W * H * D))
+    return out
 
 
 def sse_convolve(u, W, H, gamma):

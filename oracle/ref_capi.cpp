@@ -130,6 +130,18 @@ void ref_convolve(int W, int H, double gamma, const double* u, double* out, int 
   k.convolve(u, out, nvec);
 }
 
+// Gaussian1DKernel::convolve (kernels.h:533-549).
+void ref_convolve1d(int W, double gamma, const double* u, double* out, int nvec) {
+  Gaussian1DKernel k(gamma, W);
+  k.convolve(u, out, nvec);
+}
+
+// Gaussian3DKernel::convolve (kernels.h:762-807).
+void ref_convolve3d(int W, int H, int D, double gamma, const double* u, double* out, int nvec) {
+  Gaussian3DKernel k(gamma, W, H, D);
+  k.convolve(u, out, nvec);
+}
+
 // GaussianSSE2DKernel::convolve (kernels.h:878-897, sse_helpers.h:403-423).
 // Its dot product uses aligned AVX loads (sse_helpers.h:213), so rows are
 // copied into 32-byte aligned storage, as the reference's own aligned_allocator

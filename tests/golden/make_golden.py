@@ -114,6 +114,7 @@ def main():
     for name, d in out.items():
         np.savez_compressed(os.path.join(HERE, name + ".npz"), **d)
     wasserstein_fixtures()
+    line_fixtures()
     print("wrote %d fixtures to %s" % (len(out), HERE))
 
 
@@ -143,6 +144,21 @@ def wasserstein_fixtures():
                         W=W, H=H, K=K, P=P, L=L, gamma=g, loss=3, scale=2.0, exp_weight=True,
                         log_domain=False, x=xx, obs=obs, b_storage=np.zeros(0), f=f, g=gr,
                         b_temp=np.zeros(0))
+
+
+def line_fixtures():
+    # Gaussian1DKernel / Gaussian3DKernel::convolve (kernels.h:533-549,
+    # :762-807), with a generator of their own
+    rl = np.random.default_rng(404)
+    for j, (W, g, nv) in enumerate([(37, 2.0, 3), (200, 9.0, 2), (64, 500.0, 2)]):
+        u = rl.uniform(0, 1, nv * W)
+        np.savez_compressed(os.path.join(HERE, "conv1d_%d.npz" % j), W=W, gamma=g, u=u,
+                            out=ref.convolve1d(u, W, g))
+    for j, (W, H, D, g, nv) in enumerate([(8, 6, 5, 2.0, 2), (20, 16, 12, 1.0, 1),
+                                          (6, 5, 136, 4.0, 1)]):
+        u = rl.uniform(0, 1, nv * W * H * D)
+        np.savez_compressed(os.path.join(HERE, "conv3d_%d.npz" % j), W=W, H=H, D=D, gamma=g, u=u,
+                            out=ref.convolve3d(u, W, H, D, g))
 
 
 if __name__ == "__main__":

@@ -33,6 +33,18 @@ def test_convolve_fixture(name, d):
         assert maxrel(d["out_sse"], d["out"]) < 1e-13
 
 
+@pytest.mark.parametrize("name,d", fixtures("conv1d"))
+def test_convolve1d_fixture(name, d):
+    out = O.convolve1d(d["u"], int(d["W"]), float(d["gamma"]))
+    assert maxrel(out, d["out"]) < 1e-13
+
+
+@pytest.mark.parametrize("name,d", fixtures("conv3d"))
+def test_convolve3d_fixture(name, d):
+    W, H, D, g = int(d["W"]), int(d["H"]), int(d["D"]), float(d["gamma"])
+    assert maxrel(O.convolve3d(d["u"], W, H, D, g), d["out"]) < 1e-13
+
+
 @pytest.mark.parametrize("name,d", fixtures("lconv"))
 def test_log_convolve_fixture(name, d):
     W, H, g = int(d["W"]), int(d["H"]), float(d["gamma"])
@@ -99,6 +111,44 @@ def test_evaluate_both_fixture(name, d):
 def test_convolve_vs_ref(W, H, g):
     u = np.random.default_rng(W * H).uniform(0, 1, 3 * W * H)
     assert maxrel(O.convolve(u, W, H, g), R.convolve(u, W, H, g)) < 1e-13
+
+
+@needs_ref
+@pytest.mark.parametrize("W,g", [(19, 1.0), (150, 30.0)])
+def test_convolve1d_vs_ref(W, g):
+    u = np.random.default_rng(W).uniform(0, 1, 4 * W)
+    assert maxrel(O.convolve1d(u, W, g), R.convolve1d(u, W, g)) < 1e-13
+
+
+@needs_ref
+@pytest.mark.parametrize("W,H,D,g", [(7, 9, 4, 1.0), (5, 6, 21, 8.0)])
+def test_convolve3d_vs_ref(W, H, D, g):
+    u = np.random.default_rng(W * H * D).uniform(0, 1, 2 * W * H * D)
+    assert maxrel(O.convolve3d(u, W, H, D, g), R.convolve3d(u, W, H, D, g)) < 1e-13
+
+
+def test_convolve1d_delta_is_the_table():
+    # the response to a unit impulse at j is the factor row kernel1d[|i - j|]
+    W, g = 23, 3.0
+    t = O.kernel_table(W, 1, g)
+    for j in (0, 7, W - 1):
+        e = np.zeros(W)
+        e[j] = 1.0
+        np.testing.assert_array_equal(O.convolve1d(e, W, g), t[np.abs(np.arange(W) - j)])
+
+
+def test_convolve3d_separable_and_planar():
+    # a separable volume c[z] b[y] a[x] maps to (K c)[z] (K b)[y] (K a)[x]: pins the
+    # axis order and the z pass; with D = 1 the 3-D operator is the 2-D one
+    rng = np.random.default_rng(9)
+    W, H, D, g = 11, 7, 5, 2.0
+    a, b, c = rng.uniform(0, 1, W), rng.uniform(0, 1, H), rng.uniform(0, 1, D)
+    u = np.einsum("z,y,x->zyx", c, b, a).ravel()
+    want = np.einsum("z,y,x->zyx", O.convolve1d(c, D, g), O.convolve1d(b, H, g),
+                     O.convolve1d(a, W, g)).ravel()
+    assert maxrel(O.convolve3d(u, W, H, D, g), want) < 1e-14
+    v = rng.uniform(0, 1, 2 * W * H)
+    assert maxrel(O.convolve3d(v, W, H, 1, g), O.convolve(v, W, H, g)) < 1e-15
 
 
 @needs_ref

@@ -48,6 +48,14 @@ _SIGS = {
     "wdl_gauss2d_log_convolve": (_i, [_vp, _d, _d, _i]),
     "wdl_gauss2d_log_convolve_signarray": (_i, [_vp, _d, _u8, _d, _i]),
     "wdl_gauss2d_convolve_device": (_i, [_vp, _vp, _vp, _i]),
+    "wdl_gauss1d_create": (_i, [_vp, _i, ctypes.c_double, ctypes.POINTER(_vp)]),
+    "wdl_gauss1d_destroy": (_i, [_vp]),
+    "wdl_gauss1d_convolve": (_i, [_vp, _d, _d, _i]),
+    "wdl_gauss1d_convolve_device": (_i, [_vp, _vp, _vp, _i]),
+    "wdl_gauss3d_create": (_i, [_vp, _i, _i, _i, ctypes.c_double, ctypes.POINTER(_vp)]),
+    "wdl_gauss3d_destroy": (_i, [_vp]),
+    "wdl_gauss3d_convolve": (_i, [_vp, _d, _d, _i]),
+    "wdl_gauss3d_convolve_device": (_i, [_vp, _vp, _vp, _i]),
     "wdl_solver_create": (_i, [_vp, _i, _i, _i, _i, _i, ctypes.c_double, _i, _i,
                                ctypes.POINTER(_vp)]),
     "wdl_solver_destroy": (_i, [_vp]),
@@ -149,6 +157,68 @@ class Context:
                 self.close()
             except Exception:
                 pass
+
+
+class _LineKernel:
+    """Shared surface of the 1-D and 3-D KernelTypes (convolve, its adjoint and
+    the FP32 device form)."""
+
+    _prefix = ""
+
+    def _create(self, ctx, fn, *args):
+        h = _vp()
+        _check(getattr(lib(), fn)(ctx.handle, *args, ctypes.byref(h)), fn)
+        self.handle, self.ctx = h, ctx
+        ctx._children.add(self)
+
+    def convolve(self, u, nvectors: int | None = None):
+        u = _f64(u)
+        nv = u.size // self.N if nvectors is None else nvectors
+        out = np.zeros_like(u)
+        fn = self._prefix + "_convolve"
+        _check(getattr(lib(), fn)(self.handle, _dp(u), _dp(out), nv), fn)
+        return out
+
+    convolveAdjoint = convolve
+
+    def convolve_device(self, d_in: int, d_out: int, nvectors: int):
+        fn = self._prefix + "_convolve_device"
+        _check(getattr(lib(), fn)(self.handle, d_in, d_out, nvectors), fn)
+
+    def close(self):
+        if self.handle:
+            getattr(lib(), self._prefix + "_destroy")(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        if not sys.is_finalizing():
+            try:
+                self.close()
+            except Exception:
+                pass
+
+
+class HipGaussian1DKernel(_LineKernel):
+    """Gaussian1DKernel (kernels.h:501-556): nvectors vectors of W values.
+    Constructor argument order follows kernels.h:503 (gamma, W)."""
+
+    _prefix = "wdl_gauss1d"
+
+    def __init__(self, ctx: Context, gamma: float, W: int):
+        self._create(ctx, "wdl_gauss1d_create", W, float(gamma))
+        self.gamma, self.W, self.N = float(gamma), W, W
+
+
+class HipGaussian3DKernel(_LineKernel):
+    """Gaussian3DKernel (kernels.h:712-815): volumes of W x H x D, voxel
+    (x, y, z) at z*W*H + y*W + x.  Constructor argument order follows
+    kernels.h:714 (gamma, W, H, D)."""
+
+    _prefix = "wdl_gauss3d"
+
+    def __init__(self, ctx: Context, gamma: float, W: int, H: int, D: int):
+        self._create(ctx, "wdl_gauss3d_create", W, H, D, float(gamma))
+        self.gamma, self.W, self.H, self.D, self.N = float(gamma), W, H, D, W * H * D
 
 
 class HipGaussian2DKernel:
