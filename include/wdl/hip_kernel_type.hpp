@@ -6,6 +6,8 @@
 // provide the same members, so
 //     typedef wdl::HipGaussian2DKernel KernelType;               // kernels.h:630-708
 //     typedef wdl::HipLogSignArrayGaussian2DKernel KernelType;   // kernels.h:1034-1202
+//     typedef wdl::HipGaussian1DKernel KernelType;               // kernels.h:501-556
+//     typedef wdl::HipGaussian3DKernel KernelType;               // kernels.h:712-815
 // replaces Gaussian2DKernel / GaussianSSE2DKernel / GaussianHalide2DKernel or
 // LogSignArrayGaussian2DKernel without touching the reference's code.
 //
@@ -120,6 +122,82 @@ class HipLogSignArrayGaussian2DKernel : public HipGaussian2DKernel {
                                      int nvectors = 1) const {
     log_convolve_signArray(u, sign_array, result, nvectors);
   }
+};
+
+// Drop-in for Gaussian1DKernel (kernels.h:501-556): nvectors vectors of W
+// values, FP64 in and out, FP32 on the GPU.
+class HipGaussian1DKernel {
+ public:
+  // kernels.h:503-511 (default regularization 25).
+  explicit HipGaussian1DKernel(double regularization = 25., int W = 0)
+      : gamma(regularization), W_(W), N((size_t)W) {
+    if (W > 0) {
+      ctx_ = detail::shared_context();
+      wdl_gauss1d* k = nullptr;
+      detail::check(wdl_gauss1d_create(ctx_.get(), W, regularization, &k), "wdl_gauss1d_create");
+      k_ = std::shared_ptr<wdl_gauss1d>(k, [](wdl_gauss1d* p) { wdl_gauss1d_destroy(p); });
+    }
+  }
+  // kernels.h:530-532
+  double operator()(int i, int j) const {
+    return std::max(1e-250, std::exp(-(double)(i - j) * (i - j) / gamma));
+  }
+  // kernels.h:533-549
+  void convolve(const double* u, double* result, int nvectors = 1) const {
+    detail::check(wdl_gauss1d_convolve(k_.get(), u, result, nvectors), "wdl_gauss1d_convolve");
+  }
+  // kernels.h:550-552
+  void convolveAdjoint(const double* u, double* result, int nvectors = 1) const {
+    convolve(u, result, nvectors);
+  }
+
+  double gamma;
+  size_t W_, N;
+
+ private:
+  std::shared_ptr<wdl_ctx> ctx_;
+  std::shared_ptr<wdl_gauss1d> k_;
+};
+
+// Drop-in for Gaussian3DKernel (kernels.h:712-815): volumes of W x H x D,
+// voxel (x, y, z) at z*W*H + y*W + x.
+class HipGaussian3DKernel {
+ public:
+  // kernels.h:714-724 (default regularization 25).
+  explicit HipGaussian3DKernel(double regularization = 25., int W = 0, int H = 0, int D = 0)
+      : gamma(regularization), W_(W), H_(H), D_(D), N((size_t)W * H * D) {
+    if (W > 0 && H > 0 && D > 0) {
+      ctx_ = detail::shared_context();
+      wdl_gauss3d* k = nullptr;
+      detail::check(wdl_gauss3d_create(ctx_.get(), W, H, D, regularization, &k),
+                    "wdl_gauss3d_create");
+      k_ = std::shared_ptr<wdl_gauss3d>(k, [](wdl_gauss3d* p) { wdl_gauss3d_destroy(p); });
+    }
+  }
+  // kernels.h:750-761: entry (i, j) of the 3-D kernel.
+  double operator()(int i, int j) const {
+    const int WH = (int)(W_ * H_);
+    const int z0 = i / WH, r0 = i - z0 * WH, z1 = j / WH, r1 = j - z1 * WH;
+    const int x0 = r0 % (int)W_, y0 = r0 / (int)W_, x1 = r1 % (int)W_, y1 = r1 / (int)W_;
+    const double d2 = (double)(x0 - x1) * (x0 - x1) + (double)(y0 - y1) * (y0 - y1) +
+                      (double)(z0 - z1) * (z0 - z1);
+    return std::max(1e-250, std::exp(-d2 / gamma));
+  }
+  // kernels.h:762-807
+  void convolve(const double* u, double* result, int nvectors = 1) const {
+    detail::check(wdl_gauss3d_convolve(k_.get(), u, result, nvectors), "wdl_gauss3d_convolve");
+  }
+  // kernels.h:808-810
+  void convolveAdjoint(const double* u, double* result, int nvectors = 1) const {
+    convolve(u, result, nvectors);
+  }
+
+  double gamma;
+  size_t W_, H_, D_, N;
+
+ private:
+  std::shared_ptr<wdl_ctx> ctx_;
+  std::shared_ptr<wdl_gauss3d> k_;
 };
 
 }  // namespace wdl

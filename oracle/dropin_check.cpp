@@ -74,12 +74,12 @@ void solve(WassersteinRegression<KT>& reg, double* gl, double* gd, double* q, st
   reg.compute_gradient_both_log_signArray(gl, gd, q, 0);
 }
 
-// compute_gradient_both (or its log form) for histogram 0.
+// compute_gradient_both (or its log form) for histogram 0, with the given
+// kernel object (its N must be d.W * d.H).
 template <class KT, bool LOGDOM>
-void gradient(const Data& d, std::vector<double>& gl, std::vector<double>& gd,
+void gradient(const Data& d, KT& kernel, std::vector<double>& gl, std::vector<double>& gd,
               std::vector<double>& bary) {
   const int N = d.W * d.H;
-  KT kernel(d.gamma, d.W, d.H);
   std::vector<std::vector<double> > atoms = d.atoms;
   Problem<KT> problem(atoms, d.obs, kernel, const_cast<double*>(&d.lam[0]), d.P);
   QuadraticLoss<KT> loss;
@@ -90,6 +90,13 @@ void gradient(const Data& d, std::vector<double>& gl, std::vector<double>& gd,
   gd.assign((size_t)d.K * N, 0);
   bary.assign(N, 0);
   solve(reg, &gl[0], &gd[0], &bary[0], std::integral_constant<bool, LOGDOM>());
+}
+
+template <class KT, bool LOGDOM>
+void gradient(const Data& d, std::vector<double>& gl, std::vector<double>& gd,
+              std::vector<double>& bary) {
+  KT kernel(d.gamma, d.W, d.H);
+  gradient<KT, LOGDOM>(d, kernel, gl, gd, bary);
 }
 
 std::vector<double> initial_x(const Data& d, double scaleDic) {
@@ -153,6 +160,33 @@ int main() {
     const double e1 = rel(gl2, gl1), e2 = rel(gd2, gd1), e3 = rel(q2, q1);
     ok = ok && e1 < 1e-10 && e2 < 1e-10 && e3 < 1e-10;
     std::snprintf(buf, sizeof buf, "\"gradient_both_log\": [%.3e, %.3e, %.3e], ", e1, e2, e3);
+    out += buf;
+  }
+  // 4. compute_gradient_both on 1-D histograms (150 bins) with the HIP
+  //    Gaussian1DKernel drop-in.
+  {
+    Data d = make(150, 1, 3, 1, 8, 4.0, 13);
+    std::vector<double> gl1, gd1, q1, gl2, gd2, q2;
+    Gaussian1DKernel k1(d.gamma, 150);
+    wdl::HipGaussian1DKernel k2(d.gamma, 150);
+    gradient<Gaussian1DKernel, false>(d, k1, gl1, gd1, q1);
+    gradient<wdl::HipGaussian1DKernel, false>(d, k2, gl2, gd2, q2);
+    const double e1 = rel(gl2, gl1), e2 = rel(gd2, gd1), e3 = rel(q2, q1);
+    ok = ok && e1 < 1e-5 && e2 < 1e-5 && e3 < 1e-5;
+    std::snprintf(buf, sizeof buf, "\"gradient_both_1d\": [%.3e, %.3e, %.3e], ", e1, e2, e3);
+    out += buf;
+  }
+  // 5. ... and on 12 x 10 x 8 volumes with the HIP Gaussian3DKernel drop-in.
+  {
+    Data d = make(12, 10 * 8, 2, 1, 6, 2.0, 14);
+    std::vector<double> gl1, gd1, q1, gl2, gd2, q2;
+    Gaussian3DKernel k1(d.gamma, 12, 10, 8);
+    wdl::HipGaussian3DKernel k2(d.gamma, 12, 10, 8);
+    gradient<Gaussian3DKernel, false>(d, k1, gl1, gd1, q1);
+    gradient<wdl::HipGaussian3DKernel, false>(d, k2, gl2, gd2, q2);
+    const double e1 = rel(gl2, gl1), e2 = rel(gd2, gd1), e3 = rel(q2, q1);
+    ok = ok && e1 < 1e-5 && e2 < 1e-5 && e3 < 1e-5;
+    std::snprintf(buf, sizeof buf, "\"gradient_both_3d\": [%.3e, %.3e, %.3e], ", e1, e2, e3);
     out += buf;
   }
   // 3. The reference's lbfgs() with its own evaluate_both, and with the batched
