@@ -290,10 +290,58 @@ def cpu_baseline(c, gpu, budget_s=15.0):
                          "unit": "Sinkhorn-iters/s",
                          "sample": "compute_gradient_both_log_signArray, 256x256, K=6, gamma=0.5, "
                                    "KL, one curve histogram, 2 iterations (forward + backward)"}
+    # Gaussian3DKernel::convolve (kernels.h:762-807) on one 64^3 volume
+    v = np.random.default_rng(1).uniform(0, 1, LINE_3D[0] ** 3)
+    t0 = time.perf_counter()
+    ref.convolve3d(v, LINE_3D[0], LINE_3D[0], LINE_3D[0], LINE_3D[1])
+    out["conv3d"] = {"value": round(1.0 / (time.perf_counter() - t0), 2), "unit": "volumes/s",
+                     "sample": "Gaussian3DKernel::convolve, one 64^3 volume, gamma=4"}
     err = {k: max(v) for k, v in errs.items()}
     err["histograms_checked"] = done
     err["against"] = "reference compute_gradient_both (FP64, CPU)"
     return out, err, refs
+
+
+LINE_3D = (64, 4.0, 16)     # side, gamma, volumes per call
+LINE_1D = (1024, 25.0, 4096)  # bins, gamma, vectors per call
+
+
+def line_leg(ctx, reps=20):
+    """The 1-D and 3-D KernelTypes (Gaussian1DKernel, Gaussian3DKernel,
+    kernels.h:501-815), FP32 device forms, inputs resident in HBM.  Algorithmic
+    FLOPs per the dense separable count: 2 n len per pass of n values along
+    lines of len (2 W^2 per 1-D vector; 2 N (W + H + D) per volume)."""
+    out = {}
+    rng = np.random.default_rng(2)
+    e0, e1 = _hip.Event(), _hip.Event()
+    n3, g3, v3 = LINE_3D
+    n1, g1, v1 = LINE_1D
+    for name, k, n, nv, flops in (
+            ("conv3d", wdl.HipGaussian3DKernel(ctx, g3, n3, n3, n3), n3 ** 3, v3,
+             2.0 * n3 ** 3 * 3 * n3),
+            ("conv1d", wdl.HipGaussian1DKernel(ctx, g1, n1), n1, v1, 2.0 * n1 * n1)):
+        din = _hip.DeviceArray((nv * n,))
+        dout = _hip.DeviceArray((nv * n,))
+        din.upload(rng.uniform(0, 1, nv * n))
+        k.convolve_device(din.ptr, dout.ptr, nv)
+        e0.record(ctx.stream)
+        for _ in range(reps):
+            k.convolve_device(din.ptr, dout.ptr, nv)
+        e1.record(ctx.stream)
+        e1.synchronize()
+        ms = e0.elapsed_ms(e1) / reps
+        out[name] = {"ms_per_call": round(ms, 4), "vectors_per_call": nv,
+                     "value": round(nv / (ms * 1e-3), 1),
+                     "unit": "volumes/s" if name == "conv3d" else "vectors/s",
+                     "dense_tflops": round(nv * flops / (ms * 1e-3) / 1e12, 2)}
+        din.free()
+        dout.free()
+        k.close()
+    out["conv3d"]["shape"] = "%d^3, gamma=%g" % (n3, g3)
+    out["conv1d"]["shape"] = "%d bins, gamma=%g" % (n1, g1)
+    e0.destroy()
+    e1.destroy()
+    return out
 
 
 def evaluate_leg(s, c, solve_ms, reps=5):
@@ -414,6 +462,7 @@ def main():
     ap.add_argument("--no-log", action="store_true", help="skip the 256x256 log-domain leg")
     ap.add_argument("--no-served", action="store_true", help="skip the served projection leg")
     ap.add_argument("--no-bf16x3", action="store_true", help="skip the split-BF16x3 leg")
+    ap.add_argument("--no-line", action="store_true", help="skip the 1-D / 3-D kernel leg")
     ap.add_argument("--projection-requests", type=int, default=512,
                     help="how many of the 4096 projection requests to run")
     ap.add_argument("--served-seconds", type=float, default=20.0,
@@ -565,6 +614,9 @@ def main():
     }
     line["evaluate_both"] = evaluate_leg(s, c, total_ms / args.steps)
     log("evaluate_both: %s" % json.dumps(line["evaluate_both"]))
+    if not args.no_line:
+        line["line_kernels"] = line_leg(ctx)
+        log("1-D / 3-D kernels: %s" % json.dumps(line["line_kernels"]))
     if not args.no_projection and args.projection_requests > 0:
         line["projection"] = projection_leg(ctx, args.projection_requests)
         log("projection: %s iters/s" % line["projection"]["value"])
