@@ -47,17 +47,19 @@ def band_halfwidth(table64) -> int:
 
 
 def pass_flops(nrows, ncols, D):
-    """MFMA FLOPs one 1-D pass executes: 16-wide contraction chunks whose FP32
-    factor is not all zero, for every 16 x 16 output block (rows_gemm)."""
+    """MFMA FLOPs one 1-D pass executes: the 4-wide contraction steps holding a
+    factor that can be nonzero (indices c0 - D .. c0 + 15 + D), for every 16 x 16
+    output block (rows_gemm); one 16x16x4 MFMA per step and row block."""
     ncb = (ncols + 15) // 16
+    nk4 = 4 * ncb
     rb = (nrows + 15) // 16
-    chunks = 0
+    steps = 0
     for cb in range(ncb):
         c0 = 16 * cb
-        lo = max(0, c0 - D) // 16
-        hi = min(ncb - 1, (c0 + 15 + D) // 16)
-        chunks += hi - lo + 1
-    return chunks * rb * 16 * 16 * 16 * 2
+        lo = max(0, c0 - D) // 4
+        hi = min(nk4 - 1, (c0 + 15 + D) // 4)
+        steps += hi - lo + 1
+    return steps * rb * 16 * 16 * 4 * 2
 
 
 def conv_flops(n, D):

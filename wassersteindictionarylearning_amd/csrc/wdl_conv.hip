@@ -68,7 +68,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_conv_tile(const T* __restrict__
   typedef typename Vec<VW>::T V;
   extern __shared__ __align__(16) float smem[];
   float* tabs = smem;
-  float* buf = smem + 4 * kTabLen;
+  float* buf = smem + kTabFloats;
   load_tables(tabs, gtab);
   const size_t N = (size_t)TG.W * TG.H;
   const int nt = TG.nx * TG.ny;
@@ -131,7 +131,7 @@ static int launch_conv(const wdl_gauss2d* k, const T* in, T* out, int nvec, hipS
   const ConvGeom G = k->geom(!(k->flags & WDL_FLAG_DENSE));
   static bool attr_set = false;
   if (!attr_set) {
-    const int img = conv_lds_bytes(make_geom(kMaxSide, kMaxSide, 0, 32)) - 4 * kTabLen * 4;
+    const int img = conv_lds_bytes(make_geom(kMaxSide, kMaxSide, 0, 32)) - kTabFloats * 4;
     const void* f32[] = {(const void*)k_conv<T, false>, (const void*)k_conv4<false>};
     const void* fbf[] = {(const void*)k_conv<T, true>, (const void*)k_conv4<true>};
     for (const void* f : f32)
@@ -142,7 +142,7 @@ static int launch_conv(const wdl_gauss2d* k, const T* in, T* out, int nvec, hipS
                                   img + ConvPrec<true>::kTabBytes));
     attr_set = true;
   }
-  const int lds = conv_lds_bytes(G) - 4 * kTabLen * 4 +
+  const int lds = conv_lds_bytes(G) - kTabFloats * 4 +
                   (bf3 ? ConvPrec<true>::kTabBytes : ConvPrec<false>::kTabBytes);
   const int grid = nvec;
   const bool v4 = sizeof(T) == sizeof(float) && G.W % 4 == 0;

@@ -9,12 +9,12 @@
 // transposed, so pass 2 is the same routine applied to the transposed
 // intermediate (R = K_H . U . K_W with K symmetric, kernels.h:669-699).
 //
-// The Gaussian factor is never materialised: the B operand is read from a
-// 1-D table of exp(-d^2/gamma) kept in LDS in four copies shifted by 0..3
-// floats, so every lane fetches its four consecutive-k factors with one
-// aligned 16-byte LDS read.  With `band` on, 16-wide contraction chunks whose
-// factors are all exactly 0.0f in FP32 (|k - j| beyond the FP32 underflow of
-// exp(-d^2/gamma)) are skipped; they would add exact zeros.
+// The Gaussian factor is never materialised: the B operand is read from the
+// 1-D table of exp(-d^2/gamma) kept in LDS.  Each MFMA covers four contiguous
+// contraction indices, and with `band` on the 4-wide steps whose factors are
+// all exactly 0.0f in FP32 (|k - j| beyond the FP32 underflow of
+// exp(-d^2/gamma)) for a unit's 16 output columns are skipped; they would add
+// exact zeros.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -26,6 +26,7 @@ constexpr int kWaves = kThreads / 64;
 constexpr int kMaxSide = 128;              // largest padded side held in LDS
 constexpr int kTabOff = kMaxSide + 16;     // table index of d = 0
 constexpr int kTabLen = 2 * kMaxSide + 48; // multiple of 4
+constexpr int kTabFloats = kTabLen;        // LDS floats of the FP32 factor table
 constexpr int kMaxUnits = 2;               // 16x64 output units per wave per pass
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
@@ -41,15 +42,12 @@ struct ConvGeom {
 // Bytes of dynamic LDS for one workgroup.
 __host__ __device__ inline int conv_lds_bytes(const ConvGeom& g) {
   const int img = (g.Hp * g.S1 > g.Wp * g.S2) ? g.Hp * g.S1 : g.Wp * g.S2;
-  return (4 * kTabLen + img) * (int)sizeof(float);
+  return (kTabFloats + img) * (int)sizeof(float);
 }
 
-// copy s of the table holds tab[x + s]; all four copies are 16-byte aligned.
+// The FP32 factor table into LDS: tabs[kTabOff + d] = tab[|d|].
 __device__ inline void load_tables(float* tabs, const float* __restrict__ gtab) {
-  for (int e = threadIdx.x; e < 4 * kTabLen; e += kThreads) {
-    const int s = e / kTabLen, x = e - s * kTabLen;
-    tabs[e] = (x + s < kTabLen) ? gtab[x + s] : 0.f;
-  }
+  for (int e = threadIdx.x; e < kTabFloats; e += kThreads) tabs[e] = gtab[e];
 }
 
 // The wave's index in the workgroup as a scalar.  The compiler cannot prove
@@ -62,34 +60,29 @@ __device__ inline floatx4 mfma4(float a, float b, floatx4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-// One output unit: NRB row blocks of 16 rows at arow (lane's row c, offset
-// 4g), contraction chunks kc_lo..kc_hi.  The next chunk's operands are read
-// from LDS while the current chunk's 16 MFMAs issue; no condition inside the
-// loop, so the MFMAs go back to back.
+// One output unit: NRB row blocks of 16 rows, contraction steps ks_lo..ks_hi.
+// A step is one MFMA per row block over the four contiguous contraction
+// indices 4ks..4ks+3 (lane (c, g) gives A[c][g] = IN[r0 + 16rb + c][4ks + g]
+// and B[g][c] = Kg(4ks + g, c0 + c) = tab[abase + 4ks], abase = kTabOff + g -
+// c0 - c), so the band is skipped to within 4 indices.  The next step's
+// operands are read from LDS while the current step's MFMAs issue; no
+// condition inside the loop.
 template <int NRB>
 __device__ inline void gemm_unit(const float* __restrict__ arow, int sin,
-                                 const float* __restrict__ tabs, int abase, int kc_lo, int kc_hi,
+                                 const float* __restrict__ tab, int abase, int ks_lo, int ks_hi,
                                  floatx4 (&acc)[4]) {
-  // B[k][j] = Kg(kk + 4g + t, c0 + c) = tab[abase + kk + t], abase = kTabOff + 4g - c0 - c;
-  // A[i][k] = IN[r0 + 16rb + c][kk + 4g + t]
-  auto load = [&](int kc, floatx4 (&a)[NRB], floatx4& b) {
-    const int kk = kc << 4;
-    const int ai = abase + kk;
-    b = *reinterpret_cast<const floatx4*>(tabs + (ai & 3) * kTabLen + (ai & ~3));
+  auto load = [&](int ks, float (&a)[NRB], float& b) {
+    b = tab[abase + 4 * ks];
 #pragma unroll
-    for (int rb = 0; rb < NRB; ++rb)
-      a[rb] = *reinterpret_cast<const floatx4*>(arow + rb * 16 * sin + kk);
+    for (int rb = 0; rb < NRB; ++rb) a[rb] = arow[rb * 16 * sin + 4 * ks];
   };
-  floatx4 av[NRB], bv;
-  load(kc_lo, av, bv);
-  for (int kc = kc_lo; kc <= kc_hi; ++kc) {
-    floatx4 an[NRB], bn;
-    load(min(kc + 1, kc_hi), an, bn);  // the last iteration re-reads its own chunk
-    // t outer, row block inner: consecutive MFMAs write different accumulators.
+  float av[NRB], bv;
+  load(ks_lo, av, bv);
+  for (int ks = ks_lo; ks <= ks_hi; ++ks) {
+    float an[NRB], bn;
+    load(min(ks + 1, ks_hi), an, bn);  // the last step re-reads its own operands
 #pragma unroll
-    for (int t = 0; t < 4; ++t)
-#pragma unroll
-      for (int rb = 0; rb < NRB; ++rb) acc[rb] = mfma4(av[rb][t], bv[t], acc[rb]);
+    for (int rb = 0; rb < NRB; ++rb) acc[rb] = mfma4(av[rb], bv, acc[rb]);
 #pragma unroll
     for (int rb = 0; rb < NRB; ++rb) av[rb] = an[rb];
     bv = bn;
@@ -101,7 +94,7 @@ __device__ inline void gemm_unit(const float* __restrict__ arow, int sin,
 // tab[|k - j|].  Output unit u of this wave covers output columns
 // [16*cb, 16*cb+16) and rows [64*rg, 64*rg+64); acc[uu][rb] holds the 16x16
 // block of rows 64*rg+16*rb.. in the MFMA 16x16 layout (lane l, reg r: row
-// 4*(l/16)+r, col l%16).  Contraction chunks whose factors are all beyond D
+// 4*(l/16)+r, col l%16).  Contraction steps whose factors are all beyond D
 // (exactly 0 in FP32) are skipped.
 __device__ inline void rows_gemm(const float* __restrict__ in, int sin, int nrows, int ncontract,
                                  int nout, int off, const float* __restrict__ tabs, int D,
@@ -109,7 +102,7 @@ __device__ inline void rows_gemm(const float* __restrict__ in, int sin, int nrow
   const int lane = threadIdx.x & 63, wave = wave_id();
   const int c = lane & 15, g = lane >> 4;
   const int ncb = nout >> 4;
-  const int nck = ncontract >> 4;
+  const int nk4 = ncontract >> 2;
   const int nunits = ncb * ((nrows + 63) >> 6);
 #pragma unroll
   for (int uu = 0; uu < kMaxUnits; ++uu) {
@@ -120,16 +113,18 @@ __device__ inline void rows_gemm(const float* __restrict__ in, int sin, int nrow
       const int cb = u % ncb, rg = u / ncb;
       const int c0 = (cb << 4) + off, r0 = rg << 6;  // c0: contraction index of output column 0
       const int nrb = min(4, (nrows - r0) >> 4);
+      // contraction steps of 4 holding a factor that can be nonzero for the
+      // unit's 16 columns: indices c0 - D .. c0 + 15 + D
       const int lo = c0 - D;
-      const int kc_lo = lo > 0 ? (lo >> 4) : 0;
-      const int kc_hi = min(nck - 1, (c0 + 15 + D) >> 4);
-      const float* arow = in + (r0 + c) * sin + 4 * g;
-      const int abase = kTabOff + 4 * g - c0 - c;
+      const int ks_lo = lo > 0 ? (lo >> 2) : 0;
+      const int ks_hi = min(nk4 - 1, (c0 + 15 + D) >> 2);
+      const float* arow = in + (r0 + c) * sin + g;
+      const int abase = kTabOff + g - c0 - c;
       switch (nrb) {  // wave-uniform
-        case 4: gemm_unit<4>(arow, sin, tabs, abase, kc_lo, kc_hi, acc[uu]); break;
-        case 3: gemm_unit<3>(arow, sin, tabs, abase, kc_lo, kc_hi, acc[uu]); break;
-        case 2: gemm_unit<2>(arow, sin, tabs, abase, kc_lo, kc_hi, acc[uu]); break;
-        default: gemm_unit<1>(arow, sin, tabs, abase, kc_lo, kc_hi, acc[uu]); break;
+        case 4: gemm_unit<4>(arow, sin, tabs, abase, ks_lo, ks_hi, acc[uu]); break;
+        case 3: gemm_unit<3>(arow, sin, tabs, abase, ks_lo, ks_hi, acc[uu]); break;
+        case 2: gemm_unit<2>(arow, sin, tabs, abase, ks_lo, ks_hi, acc[uu]); break;
+        default: gemm_unit<1>(arow, sin, tabs, abase, ks_lo, ks_hi, acc[uu]); break;
       }
     }
   }
@@ -281,7 +276,7 @@ __device__ inline void conv_lds_bf3(float* buf, const __bf16* tb, const ConvGeom
 // FP32 MFMA (4 shifted FP32 copies), true the split-BF16x3 form.
 template <bool BF3> struct ConvPrec;
 template <> struct ConvPrec<false> {
-  static constexpr int kTabBytes = 4 * kTabLen * (int)sizeof(float);
+  static constexpr int kTabBytes = kTabFloats * (int)sizeof(float);
   __device__ static void load(void* t, const float* g) { load_tables((float*)t, g); }
   __device__ static void conv(float* buf, const void* t, const ConvGeom& G) {
     conv_lds(buf, (const float*)t, G);

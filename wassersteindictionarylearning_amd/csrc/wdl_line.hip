@@ -39,7 +39,7 @@ static LineGeom make_line_geom(int len, int D) {
   return g;
 }
 
-static int line_lds_bytes(const LineGeom& g) { return (4 * kTabLen + g.lpw * g.S) * 4; }
+static int line_lds_bytes(const LineGeom& g) { return (kTabFloats + g.lpw * g.S) * 4; }
 
 template <class T>
 __global__ __launch_bounds__(kThreads) void k_line(const T* __restrict__ in, T* __restrict__ out,
@@ -47,7 +47,7 @@ __global__ __launch_bounds__(kThreads) void k_line(const T* __restrict__ in, T* 
                                                    const float* __restrict__ gtab) {
   extern __shared__ __align__(16) float smem[];
   float* tabs = smem;
-  float* buf = smem + 4 * kTabLen;
+  float* buf = smem + kTabFloats;
   load_tables(tabs, gtab);
   const long long nlines = outer * inner;
   const long long ntiles = (nlines + g.lpw - 1) / g.lpw;

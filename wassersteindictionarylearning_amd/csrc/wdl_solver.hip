@@ -350,7 +350,7 @@ struct LogSmem {
 __device__ inline LogSmem log_smem(float* smem, const ConvGeom& G) {
   LogSmem m;
   m.tabs = smem;
-  m.buf = smem + 4 * kTabLen;
+  m.buf = smem + kTabFloats;
   const int img = (G.Hp * G.S1 > G.Wp * G.S2) ? G.Hp * G.S1 : G.Wp * G.S2;
   int* keys = reinterpret_cast<int*>(m.buf + img);
   m.L.mrow = keys;

@@ -87,7 +87,7 @@ struct TileSmem {
 __device__ inline TileSmem tile_smem(float* smem, const TileGeom& T) {
   TileSmem m;
   m.tabs = smem;
-  m.buf = smem + 4 * kTabLen;
+  m.buf = smem + kTabFloats;
   int* keys = reinterpret_cast<int*>(m.buf + T.R * T.S);
   m.L.mrow = keys;
   m.L.mcol = keys + kMaxSide;

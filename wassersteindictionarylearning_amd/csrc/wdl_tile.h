@@ -28,7 +28,7 @@ struct TileGeom {
 
 __host__ __device__ inline int tile_lds_bytes(const TileGeom& t) {
   const int img = t.R * t.S;
-  return (4 * kTabLen + img) * (int)sizeof(float);
+  return (kTabFloats + img) * (int)sizeof(float);
 }
 
 // Visits the haloed region of tile (tx, ty): region position (r, k) <-> image
