@@ -115,13 +115,30 @@ __global__ __launch_bounds__(kXWaves * 64) void k_xlse_pass(
     if (active)
       for (int oc = 0; oc < nch; ++oc) {
         const int j = oc * 16 + o;
+        // The lane's quarter of chunk c is k = kb..kb+3, kb = 16c + 4q4; a
+        // whole quarter reads its four values together and needs no guard.
+        auto qmax = [&](int c, float r) {
+          const int kb = c * 16 + q4 * 4;
+          if (kb + 4 <= len) {
+            float x[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) x[t] = XF[kb + t];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+              const int dk = j - kb - t;
+              r = fmaxf(r, x[t] - (float)__mul24(dk, dk) * igf);
+            }
+          } else {
+            for (int t = 0; t < 4; ++t) {
+              const int k = kb + t;
+              if (k < len) r = fmaxf(r, XF[k] - (float)((j - k) * (j - k)) * igf);
+            }
+          }
+          return r;
+        };
         // maximum of x_k - (j-k)^2/gamma: own chunk, then every chunk whose
         // bound can raise it (the four lanes of j decide alike)
-        float m = -INFINITY;
-        for (int t = 0; t < 4; ++t) {
-          const int k = oc * 16 + q4 * 4 + t;
-          if (k < len) m = fmaxf(m, XF[k] - (float)((j - k) * (j - k)) * igf);
-        }
+        float m = qmax(oc, -INFINITY);
         m = fmaxf(m, __shfl_xor(m, 16));
         m = fmaxf(m, __shfl_xor(m, 32));
         // chunks more than R away cannot raise the maximum of any of the 16
@@ -131,11 +148,7 @@ __global__ __launch_bounds__(kXWaves * 64) void k_xlse_pass(
           if (c == oc) continue;
           const int d = c < oc ? j - (c * 16 + 15) : c * 16 - j;
           if (CMF[c] - (float)(d * d) * igf <= m) continue;
-          float mc = -INFINITY;
-          for (int t = 0; t < 4; ++t) {
-            const int k = c * 16 + q4 * 4 + t;
-            if (k < len) mc = fmaxf(mc, XF[k] - (float)((j - k) * (j - k)) * igf);
-          }
+          float mc = qmax(c, -INFINITY);
           mc = fmaxf(mc, __shfl_xor(mc, 16));
           mc = fmaxf(mc, __shfl_xor(mc, 32));
           m = fmaxf(m, mc);
@@ -147,17 +160,36 @@ __global__ __launch_bounds__(kXWaves * 64) void k_xlse_pass(
         for (int c = max(0, oc - Rs); c <= min(nch - 1, oc + Rs); ++c) {
           const int d = c == oc ? 0 : (c < oc ? j - (c * 16 + 15) : c * 16 - j);
           if (CM[c] - (double)(d * d) * inv_gamma - m64 < -kXPrune) continue;
-          for (int t = 0; t < 4; ++t) {
-            const int k = c * 16 + q4 * 4 + t;
-            if (k < len) {
-              // the term's exponent is formed in FP64 (exact however large X and
-              // m are), then exp(x) = 2^(x log2 e) on the hardware exp2: terms
-              // that matter have x in [-40, 0], where the FP32 product rounds by
-              // at most ~3.5e-6 relative (~3e-7 on the terms that carry the sum)
-              const int dk = j - k;
-              const float x = (float)(X[k] - (double)__mul24(dk, dk) * inv_gamma - m64);
-              const float e = __builtin_amdgcn_exp2f(x * 1.4426950408889634f);
-              sum += SIGNED ? SG[k] * e : e;
+          // the term's exponent is formed in FP64 (exact however large X and m
+          // are), then exp(x) = 2^(x log2 e) on the hardware exp2: terms that
+          // matter have x in [-40, 0], where the FP32 product rounds by at most
+          // ~3.5e-6 relative (~3e-7 on the terms that carry the sum)
+          auto term = [&](double xk, int k) {
+            const int dk = j - k;
+            const float x = (float)(xk - (double)__mul24(dk, dk) * inv_gamma - m64);
+            return __builtin_amdgcn_exp2f(x * 1.4426950408889634f);
+          };
+          const int kb = c * 16 + q4 * 4;
+          if (kb + 4 <= len) {
+            double xk[4];
+            float sg[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+              xk[t] = X[kb + t];
+              if (SIGNED) sg[t] = SG[kb + t];
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+              const float e = term(xk[t], kb + t);
+              sum += SIGNED ? sg[t] * e : e;
+            }
+          } else {
+            for (int t = 0; t < 4; ++t) {
+              const int k = kb + t;
+              if (k < len) {
+                const float e = term(X[k], k);
+                sum += SIGNED ? SG[k] * e : e;
+              }
             }
           }
         }
