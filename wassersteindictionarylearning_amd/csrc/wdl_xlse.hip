@@ -90,26 +90,30 @@ __global__ __launch_bounds__(kXWaves * 64) void k_xlse_pass(
       const long long img = line / rows;
       active = Lp[p0 + (int)(img / ipp)] >= active_thr;
     }
-    if (active) {
-      const double* src = in + line * len;
-      for (int k = lane; k < len; k += 64) {
-        const double x = src[k];
-        X[k] = x;
-        XF[k] = (float)x;
-        if (SIGNED) SG[k] = sin[line * len + k] ? 1.f : -1.f;  // getSign (signArray.h:19-22)
-      }
-    }
-    __syncthreads();
     float gmax = -INFINITY;  // the line's maximum, in every lane
     if (active) {
-      for (int c = lane; c < nch; c += 64) {
-        double m = -INFINITY;
-        for (int k = c * 16; k < min(len, c * 16 + 16); ++k) m = fmax(m, X[k]);
-        CM[c] = m;
-        CMF[c] = (float)m;
+      // 64 values per step; each 16-lane group holds one chunk and reduces
+      // its maximum with shuffles (all lanes take part in every step)
+      const double* src = in + line * len;
+      for (int base = 0; base < len; base += 64) {
+        const int k = base + lane;
+        const bool in_line = k < len;
+        const double x = in_line ? src[k] : -INFINITY;
+        if (in_line) {
+          X[k] = x;
+          XF[k] = (float)x;
+          if (SIGNED) SG[k] = sin[line * len + k] ? 1.f : -1.f;  // getSign (signArray.h:19-22)
+        }
+        double m = x;
+        for (int sh = 1; sh < 16; sh <<= 1) m = fmax(m, __shfl_xor(m, sh));
+        if ((lane & 15) == 0 && k < len) {
+          CM[k >> 4] = m;
+          CMF[k >> 4] = (float)m;
+        }
         gmax = fmaxf(gmax, (float)m);
       }
-      for (int sh = 1; sh < 64; sh <<= 1) gmax = fmaxf(gmax, __shfl_xor(gmax, sh));
+      gmax = fmaxf(gmax, __shfl_xor(gmax, 16));
+      gmax = fmaxf(gmax, __shfl_xor(gmax, 32));
     }
     __syncthreads();
     if (active)
