@@ -60,6 +60,18 @@ __device__ inline void tile_region_pass(const TileGeom& T, int tx, int ty, Load&
   }
 }
 
+// tile_region_pass with the group index e = r * (R / VW) + k / VW handed to
+// apply(e, li, gi, valid, value) as well.
+template <int VW, class Load, class Apply>
+__device__ inline void tile_region_pass_e(const TileGeom& T, int tx, int ty, Load&& load,
+                                          Apply&& apply) {
+  tile_region_pass<VW>(T, tx, ty, load, [&](int li, int gi, bool ok, const auto& v) {
+    const int q = T.R / VW;
+    const int r = li / T.S;
+    apply(r * q + (li - r * T.S) / VW, li, gi, ok, v);
+  });
+}
+
 // Visits the 64 x 64 inner tile as laid out by tile_conv's result (stride S).
 template <int VW, class Load, class Apply>
 __device__ inline void tile_inner_pass(const TileGeom& T, int tx, int ty, Load&& load,
@@ -125,15 +137,44 @@ __device__ inline void tile_log_conv(float* buf, const float* tabs, const LogLds
     L.rmin[r] = fkey(INFINITY);
   }
   __syncthreads();
-  tile_region_pass<VW>(T, tx, ty, load, [&](int li, int gi, bool ok, const auto& v) {
-    if (ok) {
-      const SLog<VW> z = make(li, gi, v, false);
-      const V xf = vf(z.x);
-      const int r = li / T.S;
-      atomicMax(L.mrow + r, fkey(vmax_all(xf)));
-      atomicMin(L.rmin + r, fkey(vmin_fin(xf)));
+  if constexpr (VW == 4) {
+    // Row maxima (and finite minima) without contended LDS atomics: each
+    // group's keys go to buf (free until the shifted values are written),
+    // then one thread per region row reduces its R / 4 groups.
+    const int q = T.R / 4, ng = T.R * q;
+    int* gmax = reinterpret_cast<int*>(buf);
+    int* gmin = gmax + ng;
+    tile_region_pass_e<VW>(T, tx, ty, load, [&](int e, int li, int gi, bool ok, const auto& v) {
+      int kx = fkey(-INFINITY), kn = fkey(INFINITY);
+      if (ok) {
+        const V xf = vf(make(li, gi, v, false).x);
+        kx = fkey(vmax_all(xf));
+        kn = fkey(vmin_fin(xf));
+      }
+      gmax[e] = kx;
+      gmin[e] = kn;
+    });
+    __syncthreads();
+    for (int r = threadIdx.x; r < T.R; r += kThreads) {
+      int kx = fkey(-INFINITY), kn = fkey(INFINITY);
+      for (int g = 0; g < q; ++g) {
+        kx = max(kx, gmax[r * q + g]);
+        kn = min(kn, gmin[r * q + g]);
+      }
+      L.mrow[r] = kx;
+      L.rmin[r] = kn;
     }
-  });
+  } else {
+    tile_region_pass<VW>(T, tx, ty, load, [&](int li, int gi, bool ok, const auto& v) {
+      if (ok) {
+        const SLog<VW> z = make(li, gi, v, false);
+        const V xf = vf(z.x);
+        const int r = li / T.S;
+        atomicMax(L.mrow + r, fkey(vmax_all(xf)));
+        atomicMin(L.rmin + r, fkey(vmin_fin(xf)));
+      }
+    });
+  }
   __syncthreads();
   check_range(L.mrow, L.rmin, T.R, events);  // region rows outside the image stay at -inf
   tile_region_pass<VW>(T, tx, ty, load, [&](int li, int gi, bool ok, const auto& v) {
@@ -150,36 +191,47 @@ __device__ inline void tile_log_conv(float* buf, const float* tabs, const LogLds
   rows_gemm(buf, T.S, T.R, T.R, kTile, T.HB, tabs, T.D, acc);
   __syncthreads();
   store_transposed(buf, T.S, T.R, kTile, acc);  // line c (inner column), position r (region row)
-  for (int c = threadIdx.x; c < kMaxSide; c += kThreads) {
-    L.mcol[c] = fkey(-INFINITY);
-    L.rmin[c] = fkey(INFINITY);
-  }
   __syncthreads();
   const int ncol = min(kTile, T.W - tx * kTile);  // inner columns inside the image
-  const int tot = kTile * T.R;
-  for (int e = threadIdx.x; e < tot; e += kThreads) {
-    const int c = e / T.R, r = e - c * T.R;
-    const float m = funkey(L.mrow[r]);
-    if (c < ncol && m > -INFINITY) {
-      const float t = m + logf(fabsf(buf[c * T.S + r]));
-      atomicMax(L.mcol + c, fkey(t));
-      atomicMin(L.rmin + c, fkey(vmin_fin(t)));
+  // Column maxima (and finite minima) of T = M_r + log|S1|: one wave per
+  // inner column, reduced with shuffles (no LDS atomics, no index division).
+  const int lane = threadIdx.x & 63, wave = wave_id();
+  for (int c = wave; c < kTile; c += kWaves) {
+    float mx = -INFINITY, mn = INFINITY;
+    if (c < ncol)
+      for (int r = lane; r < T.R; r += 64) {
+        const float m = funkey(L.mrow[r]);
+        if (m > -INFINITY) {
+          const float t = m + logf(fabsf(buf[c * T.S + r]));
+          mx = fmaxf(mx, t);
+          mn = fminf(mn, vmin_fin(t));
+        }
+      }
+    for (int sh = 1; sh < 64; sh <<= 1) {
+      mx = fmaxf(mx, __shfl_xor(mx, sh));
+      mn = fminf(mn, __shfl_xor(mn, sh));
+    }
+    if (lane == 0) {
+      L.mcol[c] = fkey(mx);
+      L.rmin[c] = fkey(mn);
     }
   }
   __syncthreads();
   check_range(L.mcol, L.rmin, kTile, events);
-  for (int e = threadIdx.x; e < tot; e += kThreads) {
-    const int c = e / T.R, r = e - c * T.R;
-    float* p = buf + c * T.S + r;
-    float v = 0.f;
-    const float m = funkey(L.mcol[c]), mr = funkey(L.mrow[r]);
-    if (c < ncol && m > -INFINITY && mr > -INFINITY) {
-      // E2 = S1 exp(M_r - M'_c), the shift difference in FP64; a factor
-      // beyond FP32 range only occurs for |S1| < e^-80 and goes through logs.
-      const float d = (float)((double)mr - (double)m);
-      v = d < 80.f ? *p * expf(d) : vsign(*p) * expf(logf(fabsf(*p)) + d);
+  for (int c = wave; c < kTile; c += kWaves) {
+    const float m = funkey(L.mcol[c]);
+    for (int r = lane; r < T.R; r += 64) {
+      float* p = buf + c * T.S + r;
+      float v = 0.f;
+      const float mr = funkey(L.mrow[r]);
+      if (c < ncol && m > -INFINITY && mr > -INFINITY) {
+        // E2 = S1 exp(M_r - M'_c), the shift difference in FP64; a factor
+        // beyond FP32 range only occurs for |S1| < e^-80 and goes through logs.
+        const float d = (float)((double)mr - (double)m);
+        v = d < 80.f ? *p * expf(d) : vsign(*p) * expf(logf(fabsf(*p)) + d);
+      }
+      *p = v;
     }
-    *p = v;
   }
   __syncthreads();
   rows_gemm(buf, T.S, kTile, T.R, kTile, T.HB, tabs, T.D, acc);
