@@ -151,15 +151,41 @@ __device__ inline void log_conv_lds(float* buf, const float* tabs, const LogLds&
     L.rmin[r] = fkey(INFINITY);
   }
   __syncthreads();
+  const int lane = threadIdx.x & 63, wave = wave_id();
   // row maxima (and finite minima for the range check)
-  px_pass_r<VW>(G, load, [&](int li, int gi, int row, bool ok, const auto& v) {
-    if (ok) {
-      const SLog<VW> z = make(li, gi, row, v, false);
-      const V xf = vf(z.x);
-      atomicMax(L.mrow + row, fkey(vmax_all(xf)));
-      atomicMin(L.rmin + row, fkey(vmin_fin(xf)));
+  if constexpr (VW == 4) {
+    // Two rows per wave step, 32 lanes each (a row has at most 32 float4
+    // groups), reduced with shuffles: no contended LDS atomics.  make only
+    // reads LDS here.
+    const int q = G.Wp / 4, half = lane >> 5, gcol = (lane & 31) * 4;
+    for (int r0 = 2 * wave; r0 < G.Hp; r0 += 2 * kWaves) {
+      const int row = r0 + half;
+      float mx = -INFINITY, mn = INFINITY;
+      if (row < G.H && (lane & 31) < q && gcol < G.W) {
+        const int li = row * G.S1 + gcol, gi = row * G.W + gcol;
+        const V xf = vf(make(li, gi, row, load(gi), false).x);
+        mx = vmax_all(xf);
+        mn = vmin_fin(xf);
+      }
+      for (int sh = 1; sh < 32; sh <<= 1) {
+        mx = fmaxf(mx, __shfl_xor(mx, sh));
+        mn = fminf(mn, __shfl_xor(mn, sh));
+      }
+      if ((lane & 31) == 0 && row < G.H) {
+        L.mrow[row] = fkey(mx);
+        L.rmin[row] = fkey(mn);
+      }
     }
-  });
+  } else {
+    px_pass_r<VW>(G, load, [&](int li, int gi, int row, bool ok, const auto& v) {
+      if (ok) {
+        const SLog<VW> z = make(li, gi, row, v, false);
+        const V xf = vf(z.x);
+        atomicMax(L.mrow + row, fkey(vmax_all(xf)));
+        atomicMin(L.rmin + row, fkey(vmin_fin(xf)));
+      }
+    });
+  }
   __syncthreads();
   check_range(L.mrow, L.rmin, G.H, events);
   // shifted, signed values
@@ -177,36 +203,41 @@ __device__ inline void log_conv_lds(float* buf, const float* tabs, const LogLds&
   rows_gemm(buf, G.S1, G.Hp, G.Wp, tabs, G.D, acc);  // S1 = E . K_W
   __syncthreads();
   store_transposed(buf, G.S2, G.Hp, G.Wp, acc);       // buf = S1^T (Wp lines of Hp)
-  for (int c = threadIdx.x; c < kMaxSide; c += kThreads) {
-    L.mcol[c] = fkey(-INFINITY);
-    L.rmin[c] = fkey(INFINITY);
-  }
   __syncthreads();
-  const int tot = G.Wp * G.Hp;
-  for (int e = threadIdx.x; e < tot; e += kThreads) {
-    const int c = e / G.Hp, r = e - c * G.Hp;  // line c (image column), position r (image row)
-    if (c < G.W && r < G.H) {
-      const float t = funkey(L.mrow[r]) + logf(fabsf(buf[c * G.S2 + r]));
-      atomicMax(L.mcol + c, fkey(t));
-      atomicMin(L.rmin + c, fkey(vmin_fin(t)));
+  // Column maxima (and finite minima) of T = M_r + log|S1|: one wave per
+  // line c (image column), reduced with shuffles (no LDS atomics).
+  for (int c = wave; c < kMaxSide; c += kWaves) {
+    float mx = -INFINITY, mn = INFINITY;
+    if (c < G.W)
+      for (int r = lane; r < G.H; r += 64) {
+        const float t = funkey(L.mrow[r]) + logf(fabsf(buf[c * G.S2 + r]));
+        mx = fmaxf(mx, t);
+        mn = fminf(mn, vmin_fin(t));
+      }
+    for (int sh = 1; sh < 64; sh <<= 1) {
+      mx = fmaxf(mx, __shfl_xor(mx, sh));
+      mn = fminf(mn, __shfl_xor(mn, sh));
+    }
+    if (lane == 0) {
+      L.mcol[c] = fkey(mx);
+      L.rmin[c] = fkey(mn);
     }
   }
   __syncthreads();
   check_range(L.mcol, L.rmin, G.W, events);
-  for (int e = threadIdx.x; e < tot; e += kThreads) {
-    const int c = e / G.Hp, r = e - c * G.Hp;
-    float* p = buf + c * G.S2 + r;
-    float v = 0.f;
-    if (c < G.W && r < G.H) {
-      const float m = funkey(L.mcol[c]);
-      if (m > -INFINITY) {
+  for (int c = wave; c < G.Wp; c += kWaves) {
+    const float m = c < G.W ? funkey(L.mcol[c]) : -INFINITY;
+    for (int r = lane; r < G.Hp; r += 64) {
+      float* p = buf + c * G.S2 + r;
+      float v = 0.f;
+      if (r < G.H && m > -INFINITY) {
         // E2 = S1 exp(M_r - M'_c), the shift difference in FP64; a factor
         // beyond FP32 range only occurs for |S1| < e^-80 and goes through logs.
         const float d = (float)((double)funkey(L.mrow[r]) - (double)m);
         v = d < 80.f ? *p * expf(d) : vsign(*p) * expf(logf(fabsf(*p)) + d);
       }
+      *p = v;
     }
-    *p = v;
   }
   __syncthreads();
   rows_gemm(buf, G.S2, G.Wp, G.Hp, tabs, G.D, acc);  // S2^T = E2^T . K_H
