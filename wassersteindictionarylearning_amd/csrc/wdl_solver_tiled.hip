@@ -414,7 +414,7 @@ __device__ inline double tdot(doublex4 a, floatx4 b) {
 // log c^{s-1} = K log b^{s-1}, log b^{s-1} = sum_j lam_j log phi_j^{s-1} - log phi_i^{s-1}
 // (:711-724; b^0: 0 or the warm-start store), also stored as the history Bh.
 template <int VW>
-__global__ __launch_bounds__(kThreads, 4) void k_tfwd1_log(SolveArgs a, int s) {
+__global__ __launch_bounds__(kThreads, 6) void k_tfwd1_log(SolveArgs a, int s) {
   typedef typename Vec<VW>::T V;
   typedef typename VecD<VW>::T VD;
   int p, i, tile;
@@ -469,7 +469,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_tfwd1_log(SolveArgs a, int s) {
 
 // log phi^s = K (log p - log c^{s-1}) (:704-707).
 template <int VW>
-__global__ __launch_bounds__(kThreads, 4) void k_tfwd2_log(SolveArgs a, int s) {
+__global__ __launch_bounds__(kThreads, 6) void k_tfwd2_log(SolveArgs a, int s) {
   typedef typename Vec<VW>::T V;
   typedef typename VecD<VW>::T VD;
   int p, i, tile;
@@ -512,7 +512,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_tfwd2_log(SolveArgs a, int s) {
 // chain 1 (:778-796): the dot <log phi^s, g>, c = K [(lam_i g + r_i) / phi^s].
 // Both store the second convolution's input log p + c - 2 log c^{s-1}, sign(c).
 template <int VW>
-__global__ __launch_bounds__(kThreads, 4) void k_tbwd1_log(SolveArgs a, int t) {
+__global__ __launch_bounds__(kThreads, 6) void k_tbwd1_log(SolveArgs a, int t) {
   typedef typename Vec<VW>::T V;
   typedef typename VecD<VW>::T VD;
   int p, i, tile, chain;
@@ -649,7 +649,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_tbwd1_log(SolveArgs a, int t) {
 // Backward step s >= 2, second convolution, log domain: -v = sign exp(K X0 - log
 // phi^{s-1}) (:757-762), -r = sign exp(K X1 + log b^{s-1}) (:797-805).
 template <int VW>
-__global__ __launch_bounds__(kThreads, 4) void k_tbwd2_log(SolveArgs a, int t) {
+__global__ __launch_bounds__(kThreads, 6) void k_tbwd2_log(SolveArgs a, int t) {
   typedef typename Vec<VW>::T V;
   typedef typename VecD<VW>::T VD;
   int p, i, tile, chain;

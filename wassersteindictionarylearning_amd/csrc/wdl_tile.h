@@ -33,6 +33,9 @@ __host__ __device__ inline int tile_lds_bytes(const TileGeom& t) {
 
 // Visits the haloed region of tile (tx, ty): region position (r, k) <-> image
 // pixel (ty*64 - HB + r, tx*64 - HB + k); outside the image `ok` is false.
+// One group per thread at a time: the log-domain loads carry FP64 values, and
+// fewer of them in flight keep the kernels at three workgroups per CU.
+constexpr int kUnrollTile = 1;
 template <int VW, class Load, class Apply>
 __device__ inline void tile_region_pass(const TileGeom& T, int tx, int ty, Load&& load,
                                         Apply&& apply) {
@@ -40,12 +43,12 @@ __device__ inline void tile_region_pass(const TileGeom& T, int tx, int ty, Load&
   const int total = T.R * q;
   const int y0 = ty * kTile - T.HB, x0 = tx * kTile - T.HB;
   typedef decltype(load(0)) V;
-  for (int base = threadIdx.x; base < total; base += kThreads * kUnroll) {
-    V v[kUnroll];
-    int li[kUnroll], gi[kUnroll];
-    bool ok[kUnroll];
+  for (int base = threadIdx.x; base < total; base += kThreads * kUnrollTile) {
+    V v[kUnrollTile];
+    int li[kUnrollTile], gi[kUnrollTile];
+    bool ok[kUnrollTile];
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
+    for (int u = 0; u < kUnrollTile; ++u) {
       const int e = base + u * kThreads;
       const int r = e / q, k = (e - r * q) * VW;
       const int y = y0 + r, x = x0 + k;
@@ -55,7 +58,7 @@ __device__ inline void tile_region_pass(const TileGeom& T, int tx, int ty, Load&
       if (ok[u]) v[u] = load(gi[u]);
     }
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u)
+    for (int u = 0; u < kUnrollTile; ++u)
       if (base + u * kThreads < total) apply(li[u], gi[u], ok[u], v[u]);
   }
 }
