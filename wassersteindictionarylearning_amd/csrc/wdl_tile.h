@@ -128,7 +128,8 @@ __device__ inline bool tile_inner(const TileGeom& T, int li) {
 // tile's shapes).  The input of region pixel group li / gi is
 // make(li, gi, v, second) -> SLog<VW> with v = load(gi); make is evaluated
 // twice, for the row maxima (second = false) and for the shifted values
-// (second = true), so side outputs belong in the second call.  On return buf
+// (second = true), so side outputs belong in the second call.  make must not
+// read buf: the first evaluation stages its row keys there.  On return buf
 // holds S2 (64 x 64 at stride S) and L.mcol the shifts of the inner columns.
 template <int VW, class Load, class Make>
 __device__ inline void tile_log_conv(float* buf, const float* tabs, const LogLds& L,
