@@ -1,7 +1,8 @@
 """Parity of the exact-shift log-domain path (WDL_FLAG_EXACT_LSE: the
-reference's exact per-output maximum, FP32 sums pruned at e^-40) with the
+reference's exact per-output maximum, FP32 sums over windows outside which every term is below e^-41) with the
 FP64 oracle, in the drop-in convolution and in the batched solver, including
-inputs whose log values spread far beyond the shifted MFMA form's range.
+inputs whose log values spread far beyond the shifted MFMA form's range, lines
+of one and two positions, lines of 1024, and lines that are -inf throughout.
 
 Tolerances: unsigned log-convolution outputs within 2e-6 absolute (relative
 in linear terms); signed outputs within 1e-6 of the magnitude scale, i.e.
@@ -57,7 +58,9 @@ def spread_input(rng, W, H, nv, slope):
 
 
 @pytest.mark.parametrize("W,H,g,nv,slope", [(64, 64, 1.0, 2, 0.5), (37, 50, 0.5, 3, 30.0),
-                                            (128, 96, 2.0, 1, 80.0), (300, 20, 1.0, 1, 10.0)])
+                                            (128, 96, 2.0, 1, 80.0), (300, 20, 1.0, 1, 10.0),
+                                            (1, 7, 1.0, 2, 5.0), (2, 3, 0.5, 2, 5.0),
+                                            (1024, 3, 1.0, 1, 2.0), (5, 1024, 4.0, 1, 0.2)])
 def test_exact_log_convolve_matches_oracle(ctx, W, H, g, nv, slope):
     print("exact log_convolve W=%d H=%d slope=%g" % (W, H, slope), flush=True)
     rng = np.random.default_rng(W * 7 + H)
@@ -69,6 +72,28 @@ def test_exact_log_convolve_matches_oracle(ctx, W, H, g, nv, slope):
     out, s = k.log_convolve_signArray(x, signs)
     o2, s2 = O.log_convolve_signarray(x, signs, W, H, g)
     close_signed(out, s, o2, s2, ref, x.size)
+    k.close()
+
+
+def test_exact_log_convolve_infinite_lines(ctx):
+    # whole rows and columns of -inf (zero histogram bins) next to finite ones
+    print("exact log_convolve -inf lines", flush=True)
+    rng = np.random.default_rng(5)
+    W, H, g = 70, 40, 1.0
+    x = rng.normal(0, 20, (H, W))
+    x[3] = -np.inf
+    x[:, 10:13] = -np.inf
+    x[20:, 40:] = -np.inf
+    x = x.ravel()
+    k = wdl.HipGaussian2DKernel(ctx, g, W, H, wdl.FLAG_EXACT_LSE)
+    ref = O.log_convolve(x, W, H, g)
+    close_log(k.log_convolve(x), ref)
+    signs = rng.integers(0, 256, (x.size + 7) // 8).astype(np.uint8)
+    out, s = k.log_convolve_signArray(x, signs)
+    o2, s2 = O.log_convolve_signarray(x, signs, W, H, g)
+    close_signed(out, s, o2, s2, ref, x.size)
+    xi = np.full(W * H, -np.inf)
+    np.testing.assert_array_equal(k.log_convolve(xi), xi)
     k.close()
 
 

@@ -6,14 +6,13 @@
 //     out_j = m_j + log |sum_k s_k exp(x_k - (j-k)^2/gamma - m_j)|
 // with the reference's exact per-output maximum, so any spread of log values
 // along a line is handled (the shifted MFMA form of wdl_logconv.h needs every
-// line within ~80 nats).  The line is split into 16-pixel chunks with their
-// maxima; a chunk whose bound max_c - d_min^2/gamma cannot raise m_j is not
-// visited for the maximum, and one that lies more than 40 nats below m_j is
-// not visited for the sum (its terms are below e^-40 each: at most len * 4e-18
-// relative).  Terms are formed in FP64, exponentiated and summed in FP32 (each
-// term <= ~1).  Cost per output: one bound per chunk plus the visited chunks'
-// terms -- O(len) per line in the worst case, a few chunks when the log values
-// are smooth.
+// line within ~80 nats).  Prefix and suffix maxima of the line bound every
+// term by distance, which fixes for each output a window outside which all
+// terms lie more than 41 nats below m_j (each below e^-41 relative: at most
+// len * 2e-18 of the sum); the window gives m_j exactly and holds every term
+// of the sum that counts.  Terms are formed in FP64, exponentiated and summed
+// in FP32 (each term <= ~1).  Cost per output: the window's terms, ~2 sqrt(41
+// gamma) + 1 when the log values are smooth, up to len when they are not.
 //
 // The solver stages (exact_forward / exact_backward) run the log-domain
 // recursions of compute_gradient_both_log_signArray (inverseWasserstein.h:
@@ -27,196 +26,296 @@
 namespace wdl {
 
 constexpr int kXWaves = 4;  // lines per workgroup, one wave each
-constexpr double kXPrune = 40.0;
+constexpr float kXMargin = 41.f;  // terms outside a window lie this far below m_j
+constexpr int kXPad = 32;         // -inf positions on both sides of a staged line
 
+// Dynamic LDS of k_xlse_pass: per line its values (FP64 scaled by log2 e, FP32),
+// input signs, prefix and suffix maxima, outputs and output signs; per
+// workgroup the penalty tables t^2 / gamma (FP32) and t^2 log2(e) / gamma (FP64)
+// for t = 0..len+2 (distances up to len - 1, and 2 for lines shorter than 2)
+// and, centred, for t = -kXPad..kXPad.
+constexpr int kXSym = 2 * kXPad + 1;
+__host__ __device__ inline int xlse_tab_len(int len) { return len + 3; }
 __host__ __device__ inline int xlse_lds_bytes(int len) {
-  const int nch = (len + 15) / 16;
-  return kXWaves * (len * (8 + 8 + 4 + 4 + 1) + nch * (8 + 4)) + 64;
+  const int lp = len + 2 * kXPad;
+  return kXWaves * (lp * 8 + len * 8 + lp * 4 * 2 + len * 4 * 2 + len) +
+         (xlse_tab_len(len) + kXSym) * 12 + 64;
 }
 
-// Reach, in chunks, of the 16 outputs of a chunk: with m the smallest output
-// maximum among them (lanes with `valid` false excluded), a chunk c with
-// |c - oc| > R has bound <= gmax - (16 (|c - oc| - 1))^2 / gamma < m - margin.
-__device__ inline int chunk_reach(float gmax, float m, bool valid, float margin, float gam,
-                                  int nch) {
-  float mm = valid ? m : INFINITY;
-  for (int sh = 1; sh < 16; sh <<= 1) mm = fminf(mm, __shfl_xor(mm, sh));
-  const float span = gmax - mm + margin;
-  if (!(span < 1e30f)) return nch;
-  if (span <= 0.f) return 1;
-  return min(nch, 1 + (int)ceilf(sqrtf(span * gam) * (1.f / 16.f)));
+// Maximum of XF_k - t^2/gamma over k = jc + t, t = -DL..DR.  Without CLAMP
+// every k lies in the padded line and |t| <= kXPad, and the penalty comes from
+// the table centred on t = 0 (CFs[t], t = -kXPad..kXPad: one address for the
+// whole window, stepped by immediate offsets); with it, positions outside the
+// line count as -inf and the penalty is CF[|t|].
+template <bool CLAMP>
+__device__ inline float xlse_window_max(const float* XF, const float* CF, const float* CFs, int jc,
+                                        int len, int DL, int DR) {
+  auto ld = [&](int t) {
+    const int k = jc + t;
+    if (CLAMP) return (unsigned)k < (unsigned)len ? XF[k] : -INFINITY;
+    return XF[k];
+  };
+  auto pen = [&](int t) { return CLAMP ? CF[abs(t)] : CFs[t]; };
+  float m0 = -INFINITY, m1 = -INFINITY;
+  int t = -DL;
+  for (; t + 3 <= DR; t += 4) {
+    float x[4], c[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      x[u] = ld(t + u);
+      c[u] = pen(t + u);
+    }
+    m0 = fmaxf(m0, x[0] - c[0]);
+    m1 = fmaxf(m1, x[1] - c[1]);
+    m0 = fmaxf(m0, x[2] - c[2]);
+    m1 = fmaxf(m1, x[3] - c[3]);
+  }
+  for (; t <= DR; ++t) m0 = fmaxf(m0, ld(t) - pen(t));
+  return fmaxf(m0, m1);
 }
+
+// Sum of s_k 2^((X2_k - t^2 log2(e)/gamma) - m2) over the same window (C2s:
+// the centred FP64 table, C2: by |t|).
+template <bool CLAMP, bool SIGNED>
+__device__ inline float xlse_window_sum(const double* X2, const float* SG, const double* C2,
+                                        const double* C2s, int jc, int len, int DL, int DR,
+                                        double m2) {
+  auto pen = [&](int t) { return CLAMP ? C2[abs(t)] : C2s[t]; };
+  auto term = [&](double x2, double c, float sg) {
+    const float e = __builtin_amdgcn_exp2f((float)((x2 - c) - m2));
+    return SIGNED ? sg * e : e;
+  };
+  auto ldx = [&](int t) {
+    const int k = jc + t;
+    if (CLAMP) return (unsigned)k < (unsigned)len ? X2[k] : -(double)INFINITY;
+    return X2[k];
+  };
+  auto lds = [&](int t) {
+    if (!SIGNED) return 1.f;
+    const int k = jc + t;
+    if (CLAMP) return (unsigned)k < (unsigned)len ? SG[k] : 0.f;
+    return SG[k];
+  };
+  float s0 = 0.f, s1 = 0.f;
+  int t = -DL;
+  for (; t + 3 <= DR; t += 4) {
+    double x[4], c[4];
+    float sg[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      x[u] = ldx(t + u);
+      c[u] = pen(t + u);
+      sg[u] = lds(t + u);
+    }
+    s0 += term(x[0], c[0], sg[0]);
+    s1 += term(x[1], c[1], sg[1]);
+    s0 += term(x[2], c[2], sg[2]);
+    s1 += term(x[3], c[3], sg[3]);
+  }
+  for (; t <= DR; ++t) s0 += term(ldx(t), pen(t), lds(t));
+  return s0 + s1;
+}
+
+// Radius of output jc's window on one side (B: suffix maxima stepping right,
+// dir = 1, or prefix maxima stepping left, dir = -1; lim: positions available
+// on that side).  The bound B[jc + dir d] - d^2/gamma falls strictly with d,
+// so the distances 4, 8, 12, 16 that pass form a prefix: their count locates
+// the first failing multiple of 4, and one more test at 2 below it gives the
+// first failing even distance d; the radius is d - 1.  Windows wider than 16
+// continue in steps of 2.
+__device__ inline int xlse_radius(const float* B, const float* CF, int jc, int dir, int lim,
+                                  float thr, float igf) {
+  // the four reads are independent (distances clamped to lim stay in the line
+  // and fail anyway), so they are issued together
+  float b[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) b[i] = B[jc + dir * min(4 * i + 4, lim)];
+  int cnt = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int d = 4 * i + 4;
+    cnt += (int)((d <= lim) & (b[i] - (float)(d * d) * igf > thr));
+  }
+  int d;
+  if (cnt < 4) {
+    d = 4 * cnt + 4;  // first failing multiple of 4
+    const int h = d - 2, hh = min(h, lim);
+    if (!((h <= lim) & (B[jc + dir * hh] - CF[hh] > thr))) d = h;
+  } else {
+    d = 18;
+    while (d <= lim && B[jc + dir * d] - CF[d] > thr) d += 2;
+  }
+  return min(d - 1, lim);
+}
+
+// Waves per line of k_xlse_pass: long lines share their workgroup's LDS over
+// more waves (the LDS per line is fixed by len), for latency hiding.
+__host__ __device__ inline int xlse_waves_per_line(int len) {
+  return len <= 128 ? 1 : (len <= 512 ? 2 : 4);
+}
+constexpr int kXMaxThreads = kXWaves * 4 * 64;
 
 // One pass over `nlines` lines of length `len` (line-major input), writing the
 // transposed layout (line index fastest) of each image of `rows` lines.  A line
 // is skipped (output untouched) when active_thr > 0 and Lp[p0 + image / ipp] <
 // active_thr.
 //
-// One wave per line.  Outputs are taken 16 at a time (one output chunk): lane l
-// serves output j = 16 oc + (l % 16) over the quarter l / 16 of every visited
-// input chunk, and the four lanes of an output combine their maxima and sums
-// with cross-lane shuffles, so the 16 outputs of a chunk -- which need nearly
-// the same input chunks -- move through them together.  The maximum is found
-// in FP32 (any shift near the maximum serves: the output adds back exactly the
-// shift the terms were formed with); terms are formed in FP64.
+// kXWaves lines per workgroup, xlse_waves_per_line(len) waves each; one output
+// j per lane, 64 consecutive outputs at a time.  Every term of output j is
+// x_k - (j-k)^2/gamma <= PM[k] - (j-k)^2/gamma for k <= j (PM: prefix maxima)
+// and <= SM[k] - (k-j)^2/gamma for k >= j (SM: suffix maxima), and both bounds
+// fall as k moves away from j.  With m0 the maximum of the five terms
+// |j-k| <= 2, the lane's window ends on each side at the first even distance d
+// where the bound at j +- d is at most m0 - 41: every term beyond lies more
+// than 41 nats below m0 <= m_j.  The 64 lanes share one window [j-DL, j+DR]
+// (the widest of theirs), so the loops over it run in step with the penalty
+// t^2/gamma read from one LDS table.  The window gives m_j exactly (terms
+// outside it are below m0); the sum adds every term within it, so the terms
+// left out are each below e^-41 relative to the largest.  m_j is found in FP32
+// (any shift near the maximum serves: the output adds back the shift the terms
+// were formed with); exponents are formed in FP64 in log2 units and
+// exponentiated on the hardware exp2.
 template <bool SIGNED>
-__global__ __launch_bounds__(kXWaves * 64) void k_xlse_pass(
+__global__ __launch_bounds__(kXMaxThreads) void k_xlse_pass(
     const double* __restrict__ in, const unsigned char* __restrict__ sin, double* __restrict__ out,
     unsigned char* __restrict__ sout, int rows, int len, long long nlines, double inv_gamma,
     const int* __restrict__ Lp, int p0, int ipp, int active_thr) {
   extern __shared__ __align__(16) double xs[];
-  const int nch = (len + 15) >> 4;
+  const int lp = len + 2 * kXPad;
+  const int wpl = xlse_waves_per_line(len);
   const int wave = wave_id(), lane = threadIdx.x & 63;
-  const int o = lane & 15, q4 = lane >> 4;
+  const int w = wave % kXWaves, sub = wave / kXWaves;  // line slot, wave within the line
   const float igf = (float)inv_gamma;
-  const float gam = (float)(1.0 / inv_gamma);
-  // per wave: its line X (FP64) and XF (FP32), chunk maxima CM / CMF, input
-  // signs SG, outputs O and their signs SO; O0 / SO0 address all waves' outputs
-  double* X = xs + wave * len;
-  double* O0 = xs + kXWaves * len;
-  double* O = O0 + wave * len;
-  double* CM = xs + 2 * kXWaves * len + wave * nch;
-  float* F0 = reinterpret_cast<float*>(xs + 2 * kXWaves * len + kXWaves * nch);
-  float* SG = F0 + wave * len;
-  float* XF = F0 + kXWaves * len + wave * len;
-  float* CMF = F0 + 2 * kXWaves * len + wave * nch;
-  unsigned char* SO0 = reinterpret_cast<unsigned char*>(F0 + 2 * kXWaves * len + kXWaves * nch);
-  unsigned char* SO = SO0 + wave * len;
+  const double kLog2e = 1.4426950408889634;
+  // layout: X2 [kXWaves][lp], O0 [kXWaves][len], C2 [nt], C2s [kXSym] (doubles);
+  // XF, SG [kXWaves][lp], PM, SM [kXWaves][len], CF [nt], CFs [kXSym] (floats);
+  // SO0 [kXWaves][len]
+  const int nt = xlse_tab_len(len);
+  double* X2 = xs + w * lp + kXPad;
+  double* O0 = xs + kXWaves * lp;
+  double* O = O0 + w * len;
+  double* C2 = O0 + kXWaves * len;
+  double* C2s = C2 + nt + kXPad;  // C2s[t], t = -kXPad..kXPad
+  float* F0 = reinterpret_cast<float*>(C2 + nt + kXSym);
+  float* XF = F0 + w * lp + kXPad;
+  float* SG = F0 + kXWaves * lp + w * lp + kXPad;
+  float* PM = F0 + 2 * kXWaves * lp + w * len;
+  float* SM = F0 + 2 * kXWaves * lp + kXWaves * len + w * len;
+  float* CF = F0 + 2 * kXWaves * lp + 2 * kXWaves * len;
+  float* CFs = CF + nt + kXPad;
+  unsigned char* SO0 = reinterpret_cast<unsigned char*>(CF + nt + kXSym);
+  unsigned char* SO = SO0 + w * len;
+  for (int t = threadIdx.x; t < nt; t += blockDim.x) {
+    C2[t] = (double)t * t * (inv_gamma * kLog2e);
+    CF[t] = (float)(t * t) * igf;
+  }
+  for (int t = threadIdx.x; t < kXSym; t += blockDim.x) {
+    const int d = t - kXPad;
+    C2s[d] = (double)d * d * (inv_gamma * kLog2e);
+    CFs[d] = (float)(d * d) * igf;
+  }
+  // padding: the window loops read up to kXPad positions past either end
+  if (sub == 0 && lane < kXPad) {
+    X2[-1 - lane] = X2[len + lane] = -(double)INFINITY;
+    XF[-1 - lane] = XF[len + lane] = -INFINITY;
+    SG[-1 - lane] = SG[len + lane] = 0.f;
+  }
   for (long long l0 = (long long)blockIdx.x * kXWaves; l0 < nlines;
        l0 += (long long)gridDim.x * kXWaves) {
-    const long long line = l0 + wave;
+    const long long line = l0 + w;
     bool active = line < nlines;
     if (active && active_thr > 0) {
       const long long img = line / rows;
       active = Lp[p0 + (int)(img / ipp)] >= active_thr;
     }
-    float gmax = -INFINITY;  // the line's maximum, in every lane
-    if (active) {
-      // 64 values per step; each 16-lane group holds one chunk and reduces
-      // its maximum with shuffles (all lanes take part in every step)
-      const double* src = in + line * len;
+    if (active)  // stage the line: 64 values per wave and step
+      for (int k = sub * 64 + lane; k < len; k += wpl * 64) {
+        const double x = in[line * len + k];
+        X2[k] = x * kLog2e;
+        XF[k] = (float)x;
+        if (SIGNED) SG[k] = sin[line * len + k] ? 1.f : -1.f;  // getSign (signArray.h:19-22)
+      }
+    __syncthreads();
+    // prefix maxima (wave 0 of the line) and suffix maxima (its last wave):
+    // inclusive max-scans across the wave, 64 positions per step, with a
+    // running carry; __shfl_up hands lanes below sh their own value
+    if (active && sub == 0) {
+      float carry = -INFINITY;
       for (int base = 0; base < len; base += 64) {
         const int k = base + lane;
-        const bool in_line = k < len;
-        const double x = in_line ? src[k] : -INFINITY;
-        if (in_line) {
-          X[k] = x;
-          XF[k] = (float)x;
-          if (SIGNED) SG[k] = sin[line * len + k] ? 1.f : -1.f;  // getSign (signArray.h:19-22)
-        }
-        double m = x;
-        for (int sh = 1; sh < 16; sh <<= 1) m = fmax(m, __shfl_xor(m, sh));
-        if ((lane & 15) == 0 && k < len) {
-          CM[k >> 4] = m;
-          CMF[k >> 4] = (float)m;
-        }
-        gmax = fmaxf(gmax, (float)m);
+        float v = k < len ? XF[k] : -INFINITY;
+        for (int sh = 1; sh < 64; sh <<= 1) v = fmaxf(v, __shfl_up(v, sh));
+        v = fmaxf(v, carry);
+        if (k < len) PM[k] = v;
+        carry = __shfl(v, 63);
       }
-      gmax = fmaxf(gmax, __shfl_xor(gmax, 16));
-      gmax = fmaxf(gmax, __shfl_xor(gmax, 32));
+    }
+    if (active && sub == wpl - 1) {
+      float carry = -INFINITY;
+      for (int top = len - 1; top >= 0; top -= 64) {
+        const int k = top - lane;
+        float v = k >= 0 ? XF[k] : -INFINITY;
+        for (int sh = 1; sh < 64; sh <<= 1) v = fmaxf(v, __shfl_up(v, sh));
+        v = fmaxf(v, carry);
+        if (k >= 0) SM[k] = v;
+        carry = __shfl(v, 63);
+      }
     }
     __syncthreads();
     if (active)
-      for (int oc = 0; oc < nch; ++oc) {
-        const int j = oc * 16 + o;
-        // The lane's quarter of chunk c is k = kb..kb+3, kb = 16c + 4q4; a
-        // whole quarter reads its four values together and needs no guard.
-        auto qmax = [&](int c, float r) {
-          const int kb = c * 16 + q4 * 4;
-          if (kb + 4 <= len) {
-            float x[4];
+      for (int j0 = sub * 64; j0 < len; j0 += wpl * 64) {
+        const int j = j0 + lane;
+        const bool valid = j < len;
+        const int jc = valid ? j : len - 1;
+        int dl = 0, dr = 0;
+        if (valid) {
+          float m0 = XF[jc];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) x[t] = XF[kb + t];
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-              const int dk = j - kb - t;
-              r = fmaxf(r, x[t] - (float)__mul24(dk, dk) * igf);
-            }
-          } else {
-            for (int t = 0; t < 4; ++t) {
-              const int k = kb + t;
-              if (k < len) r = fmaxf(r, XF[k] - (float)((j - k) * (j - k)) * igf);
-            }
-          }
-          return r;
-        };
-        // maximum of x_k - (j-k)^2/gamma: own chunk, then every chunk whose
-        // bound can raise it (the four lanes of j decide alike)
-        float m = qmax(oc, -INFINITY);
-        m = fmaxf(m, __shfl_xor(m, 16));
-        m = fmaxf(m, __shfl_xor(m, 32));
-        // chunks more than R away cannot raise the maximum of any of the 16
-        // outputs: their bound is at most gmax - (16 (R-1))^2 / gamma <= m
-        const int R = chunk_reach(gmax, m, j < len, 0.f, gam, nch);
-        for (int c = max(0, oc - R); c <= min(nch - 1, oc + R); ++c) {
-          if (c == oc) continue;
-          const int d = c < oc ? j - (c * 16 + 15) : c * 16 - j;
-          if (CMF[c] - (float)(d * d) * igf <= m) continue;
-          float mc = qmax(c, -INFINITY);
-          mc = fmaxf(mc, __shfl_xor(mc, 16));
-          mc = fmaxf(mc, __shfl_xor(mc, 32));
-          m = fmaxf(m, mc);
+          for (int t = 1; t <= 2; ++t) m0 = fmaxf(m0, fmaxf(XF[jc - t], XF[jc + t]) - CF[t]);
+          const float thr = m0 - kXMargin;
+          dr = xlse_radius(SM, CF, jc, 1, len - 1 - jc, thr, igf);
+          dl = xlse_radius(PM, CF, jc, -1, jc, thr, igf);
         }
+        for (int sh = 1; sh < 64; sh <<= 1) {
+          dl = max(dl, __shfl_xor(dl, sh));
+          dr = max(dr, __shfl_xor(dr, sh));
+        }
+        const int DL = __builtin_amdgcn_readfirstlane(dl);
+        const int DR = __builtin_amdgcn_readfirstlane(dr);
+        const bool clamp = DL > kXPad || DR > kXPad;
+        const float m = clamp ? xlse_window_max<true>(XF, CF, CFs, jc, len, DL, DR)
+                              : xlse_window_max<false>(XF, CF, CFs, jc, len, DL, DR);
         const double m64 = fmax((double)m, -1e99);
-        float sum = 0.f;
-        // chunks beyond Rs lie more than 40 nats below every output's maximum
-        const int Rs = chunk_reach(gmax, m, j < len, (float)kXPrune + 1.f, gam, nch);
-        for (int c = max(0, oc - Rs); c <= min(nch - 1, oc + Rs); ++c) {
-          const int d = c == oc ? 0 : (c < oc ? j - (c * 16 + 15) : c * 16 - j);
-          if (CM[c] - (double)(d * d) * inv_gamma - m64 < -kXPrune) continue;
-          // the term's exponent is formed in FP64 (exact however large X and m
-          // are), then exp(x) = 2^(x log2 e) on the hardware exp2: terms that
-          // matter have x in [-40, 0], where the FP32 product rounds by at most
-          // ~3.5e-6 relative (~3e-7 on the terms that carry the sum)
-          auto term = [&](double xk, int k) {
-            const int dk = j - k;
-            const float x = (float)(xk - (double)__mul24(dk, dk) * inv_gamma - m64);
-            return __builtin_amdgcn_exp2f(x * 1.4426950408889634f);
-          };
-          const int kb = c * 16 + q4 * 4;
-          if (kb + 4 <= len) {
-            double xk[4];
-            float sg[4];
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-              xk[t] = X[kb + t];
-              if (SIGNED) sg[t] = SG[kb + t];
-            }
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-              const float e = term(xk[t], kb + t);
-              sum += SIGNED ? sg[t] * e : e;
-            }
-          } else {
-            for (int t = 0; t < 4; ++t) {
-              const int k = kb + t;
-              if (k < len) {
-                const float e = term(X[k], k);
-                sum += SIGNED ? SG[k] * e : e;
-              }
-            }
-          }
-        }
-        sum += __shfl_xor(sum, 16);
-        sum += __shfl_xor(sum, 32);
-        if (q4 == 0 && j < len) {
+        const double m2 = m64 * kLog2e;
+        const float sum =
+            clamp ? xlse_window_sum<true, SIGNED>(X2, SG, C2, C2s, jc, len, DL, DR, m2)
+                  : xlse_window_sum<false, SIGNED>(X2, SG, C2, C2s, jc, len, DL, DR, m2);
+        if (valid) {
           O[j] = m64 + hlog(fabsf(sum));
           if (SIGNED) SO[j] = sum >= 0.f ? 1 : 0;
         }
       }
     __syncthreads();
     // transposed store: the lines of one workgroup are consecutive, so each
-    // output position j gets up to kXWaves adjacent values
-    for (int e = threadIdx.x; e < kXWaves * len; e += blockDim.x) {
-      const int w = e % kXWaves, j = e / kXWaves;
-      const long long ln = l0 + w;
-      if (ln >= nlines) continue;
-      const long long img = ln / rows;
-      if (active_thr > 0 && Lp[p0 + (int)(img / ipp)] < active_thr) continue;
-      const int r = (int)(ln - img * rows);
-      const long long oi = img * (long long)rows * len + (long long)j * rows + r;
-      out[oi] = O0[w * len + j];
-      if (SIGNED) sout[oi] = SO0[w * len + j];
+    // output position j gets up to kXWaves adjacent values.  Thread i serves
+    // line slot i % kXWaves throughout (blockDim is a multiple of kXWaves), so
+    // the line's offset and activity are worked out once per line group.
+    {
+      const int ws = threadIdx.x % kXWaves;
+      const long long ln = l0 + ws;
+      bool act = ln < nlines;
+      long long ob = 0;
+      if (act) {
+        const long long img = ln / rows;
+        act = active_thr <= 0 || Lp[p0 + (int)(img / ipp)] >= active_thr;
+        ob = img * (long long)rows * len + (ln - img * rows);
+      }
+      if (act)
+        for (int j = threadIdx.x / kXWaves; j < len; j += blockDim.x / kXWaves) {
+          const long long oi = ob + (long long)j * rows;
+          out[oi] = O0[ws * len + j];
+          if (SIGNED) sout[oi] = SO0[ws * len + j];
+        }
     }
     __syncthreads();
   }
@@ -234,11 +333,12 @@ static int xlse_conv(const double* in, const unsigned char* sin, double* out, un
     const long long nl = nimg * rows;
     const int grid = (int)std::min<long long>((nl + kXWaves - 1) / kXWaves, 1 << 20);
     const int lds = xlse_lds_bytes(len);
+    const int block = kXWaves * 64 * xlse_waves_per_line(len);
     if (si)
-      hipLaunchKernelGGL(k_xlse_pass<true>, dim3(grid), dim3(kXWaves * 64), lds, st, i, si, o, so,
+      hipLaunchKernelGGL(k_xlse_pass<true>, dim3(grid), dim3(block), lds, st, i, si, o, so,
                          rows, len, nl, ig, Lp, p0, ipp, thr_);
     else
-      hipLaunchKernelGGL(k_xlse_pass<false>, dim3(grid), dim3(kXWaves * 64), lds, st, i, si, o,
+      hipLaunchKernelGGL(k_xlse_pass<false>, dim3(grid), dim3(block), lds, st, i, si, o,
                          so, rows, len, nl, ig, Lp, p0, ipp, thr_);
   };
   launch(in, sin, tmp, sin ? stmp : nullptr, H, W);   // along x: lines are rows
