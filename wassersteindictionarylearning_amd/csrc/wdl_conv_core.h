@@ -45,9 +45,16 @@ __host__ __device__ inline int conv_lds_bytes(const ConvGeom& g) {
   return (kTabFloats + img) * (int)sizeof(float);
 }
 
-// The FP32 factor table into LDS: tabs[kTabOff + d] = tab[|d|].
+// Output units (16 x 64 blocks) per wave when the workgroup has NW waves:
+// enough for a 128 x 128 pass.
+template <int NW> struct Units {
+  static constexpr int n = (kMaxUnits * kWaves + NW - 1) / NW;
+};
+
+// The FP32 factor table into LDS: tabs[kTabOff + d] = tab[|d|] (NT threads).
+template <int NT = kThreads>
 __device__ inline void load_tables(float* tabs, const float* __restrict__ gtab) {
-  for (int e = threadIdx.x; e < kTabFloats; e += kThreads) tabs[e] = gtab[e];
+  for (int e = threadIdx.x; e < kTabFloats; e += NT) tabs[e] = gtab[e];
 }
 
 // The wave's index in the workgroup as a scalar.  The compiler cannot prove
@@ -95,20 +102,21 @@ __device__ inline void gemm_unit(const float* __restrict__ arow, int sin,
 // [16*cb, 16*cb+16) and rows [64*rg, 64*rg+64); acc[uu][rb] holds the 16x16
 // block of rows 64*rg+16*rb.. in the MFMA 16x16 layout (lane l, reg r: row
 // 4*(l/16)+r, col l%16).  Contraction steps whose factors are all beyond D
-// (exactly 0 in FP32) are skipped.
+// (exactly 0 in FP32) are skipped.  NW: waves in the workgroup.
+template <int NW = kWaves>
 __device__ inline void rows_gemm(const float* __restrict__ in, int sin, int nrows, int ncontract,
                                  int nout, int off, const float* __restrict__ tabs, int D,
-                                 floatx4 (&acc)[kMaxUnits][4]) {
+                                 floatx4 (&acc)[Units<NW>::n][4]) {
   const int lane = threadIdx.x & 63, wave = wave_id();
   const int c = lane & 15, g = lane >> 4;
   const int ncb = nout >> 4;
   const int nk4 = ncontract >> 2;
   const int nunits = ncb * ((nrows + 63) >> 6);
 #pragma unroll
-  for (int uu = 0; uu < kMaxUnits; ++uu) {
+  for (int uu = 0; uu < Units<NW>::n; ++uu) {
 #pragma unroll
     for (int rb = 0; rb < 4; ++rb) acc[uu][rb] = floatx4{0.f, 0.f, 0.f, 0.f};
-    const int u = wave + uu * kWaves;
+    const int u = wave + uu * NW;
     if (u < nunits) {
       const int cb = u % ncb, rg = u / ncb;
       const int c0 = (cb << 4) + off, r0 = rg << 6;  // c0: contraction index of output column 0
@@ -131,23 +139,25 @@ __device__ inline void rows_gemm(const float* __restrict__ in, int sin, int nrow
 }
 
 // The square pass of the image-resident convolution (nout = ncols, no offset).
+template <int NW = kWaves>
 __device__ inline void rows_gemm(const float* __restrict__ in, int sin, int nrows, int ncols,
                                  const float* __restrict__ tabs, int D,
-                                 floatx4 (&acc)[kMaxUnits][4]) {
-  rows_gemm(in, sin, nrows, ncols, ncols, 0, tabs, D, acc);
+                                 floatx4 (&acc)[Units<NW>::n][4]) {
+  rows_gemm<NW>(in, sin, nrows, ncols, ncols, 0, tabs, D, acc);
 }
 
 // Writes OUT (nrows x ncols output columns, held in acc by rows_gemm) to dst
 // transposed: dst[j * sdst + r] = OUT[r][j].
+template <int NW = kWaves>
 __device__ inline void store_transposed(float* dst, int sdst, int nrows, int ncols,
-                                        const floatx4 (&acc)[kMaxUnits][4]) {
+                                        const floatx4 (&acc)[Units<NW>::n][4]) {
   const int lane = threadIdx.x & 63, wave = wave_id();
   const int c = lane & 15, g = lane >> 4;
   const int ncb = ncols >> 4;
   const int nunits = ncb * ((nrows + 63) >> 6);
 #pragma unroll
-  for (int uu = 0; uu < kMaxUnits; ++uu) {
-    const int u = wave + uu * kWaves;
+  for (int uu = 0; uu < Units<NW>::n; ++uu) {
+    const int u = wave + uu * NW;
     if (u < nunits) {
       const int cb = u % ncb, rg = u / ncb;
       const int c0 = cb << 4, r0 = rg << 6;
@@ -175,8 +185,9 @@ constexpr int kTabBfBytes = 2 * kTabBfCopies * kTabLen * 2;  // hi and lo copies
 
 // copy s of hi (lo) holds the split of tab[x + s] at tb[s*kTabLen + x]
 // (tb[(8 + s)*kTabLen + x]).
+template <int NT = kThreads>
 __device__ inline void load_tables_bf3(__bf16* tb, const float* __restrict__ gtab) {
-  for (int e = threadIdx.x; e < kTabBfCopies * kTabLen; e += kThreads) {
+  for (int e = threadIdx.x; e < kTabBfCopies * kTabLen; e += NT) {
     const int s = e / kTabLen, x = e - s * kTabLen;
     const float v = (x + s < kTabLen) ? gtab[x + s] : 0.f;
     const __bf16 h = (__bf16)v;
@@ -190,20 +201,21 @@ __device__ inline floatx4 mfma_bf16(bf16x8 a, bf16x8 b, floatx4 c) {
 }
 
 // rows_gemm with the split-BF16x3 products; same output layout in acc.
+template <int NW = kWaves>
 __device__ inline void rows_gemm_bf3(const float* __restrict__ in, int sin, int nrows,
                                      int ncontract, int nout, int off,
                                      const __bf16* __restrict__ tb, int D,
-                                     floatx4 (&acc)[kMaxUnits][4]) {
+                                     floatx4 (&acc)[Units<NW>::n][4]) {
   const int lane = threadIdx.x & 63, wave = wave_id();
   const int c = lane & 15, g = lane >> 4;
   const int ncb = nout >> 4;
   const int nck = ncontract >> 5;
   const int nunits = ncb * ((nrows + 63) >> 6);
 #pragma unroll
-  for (int uu = 0; uu < kMaxUnits; ++uu) {
+  for (int uu = 0; uu < Units<NW>::n; ++uu) {
 #pragma unroll
     for (int rb = 0; rb < 4; ++rb) acc[uu][rb] = floatx4{0.f, 0.f, 0.f, 0.f};
-    const int u = wave + uu * kWaves;
+    const int u = wave + uu * NW;
     if (u < nunits) {
       const int cb = u % ncb, rg = u / ncb;
       const int c0 = (cb << 4) + off, r0 = rg << 6;
@@ -244,31 +256,34 @@ __device__ inline void rows_gemm_bf3(const float* __restrict__ in, int sin, int 
 
 // buf holds U (Hp x Wp, stride S1, zero outside H x W) on entry and
 // R = K_H . U . K_W (stride S1; values outside H x W are don't-care) on exit.
-// All threads of the workgroup must call it; it synchronises internally.
+// All threads of the workgroup (NW waves) must call it; it synchronises
+// internally.
+template <int NW = kWaves>
 __device__ inline void conv_lds(float* buf, const float* tabs, const ConvGeom& G) {
-  floatx4 acc[kMaxUnits][4];
+  floatx4 acc[Units<NW>::n][4];
   __syncthreads();
-  rows_gemm(buf, G.S1, G.Hp, G.Wp, tabs, G.D, acc);   // T = U . K_W
+  rows_gemm<NW>(buf, G.S1, G.Hp, G.Wp, tabs, G.D, acc);   // T = U . K_W
   __syncthreads();
-  store_transposed(buf, G.S2, G.Hp, G.Wp, acc);        // buf = T^T (Wp x Hp)
+  store_transposed<NW>(buf, G.S2, G.Hp, G.Wp, acc);        // buf = T^T (Wp x Hp)
   __syncthreads();
-  rows_gemm(buf, G.S2, G.Wp, G.Hp, tabs, G.D, acc);   // R^T = T^T . K_H
+  rows_gemm<NW>(buf, G.S2, G.Wp, G.Hp, tabs, G.D, acc);   // R^T = T^T . K_H
   __syncthreads();
-  store_transposed(buf, G.S1, G.Wp, G.Hp, acc);        // buf = R (Hp x Wp)
+  store_transposed<NW>(buf, G.S1, G.Wp, G.Hp, acc);        // buf = R (Hp x Wp)
   __syncthreads();
 }
 
 // conv_lds with the split-BF16x3 passes (Wp, Hp multiples of 32).
+template <int NW = kWaves>
 __device__ inline void conv_lds_bf3(float* buf, const __bf16* tb, const ConvGeom& G) {
-  floatx4 acc[kMaxUnits][4];
+  floatx4 acc[Units<NW>::n][4];
   __syncthreads();
-  rows_gemm_bf3(buf, G.S1, G.Hp, G.Wp, G.Wp, 0, tb, G.D, acc);
+  rows_gemm_bf3<NW>(buf, G.S1, G.Hp, G.Wp, G.Wp, 0, tb, G.D, acc);
   __syncthreads();
-  store_transposed(buf, G.S2, G.Hp, G.Wp, acc);
+  store_transposed<NW>(buf, G.S2, G.Hp, G.Wp, acc);
   __syncthreads();
-  rows_gemm_bf3(buf, G.S2, G.Wp, G.Hp, G.Hp, 0, tb, G.D, acc);
+  rows_gemm_bf3<NW>(buf, G.S2, G.Wp, G.Hp, G.Hp, 0, tb, G.D, acc);
   __syncthreads();
-  store_transposed(buf, G.S1, G.Wp, G.Hp, acc);
+  store_transposed<NW>(buf, G.S1, G.Wp, G.Hp, acc);
   __syncthreads();
 }
 
@@ -277,16 +292,20 @@ __device__ inline void conv_lds_bf3(float* buf, const __bf16* tb, const ConvGeom
 template <bool BF3> struct ConvPrec;
 template <> struct ConvPrec<false> {
   static constexpr int kTabBytes = kTabFloats * (int)sizeof(float);
-  __device__ static void load(void* t, const float* g) { load_tables((float*)t, g); }
+  template <int NT = kThreads>
+  __device__ static void load(void* t, const float* g) { load_tables<NT>((float*)t, g); }
+  template <int NW = kWaves>
   __device__ static void conv(float* buf, const void* t, const ConvGeom& G) {
-    conv_lds(buf, (const float*)t, G);
+    conv_lds<NW>(buf, (const float*)t, G);
   }
 };
 template <> struct ConvPrec<true> {
   static constexpr int kTabBytes = kTabBfBytes;
-  __device__ static void load(void* t, const float* g) { load_tables_bf3((__bf16*)t, g); }
+  template <int NT = kThreads>
+  __device__ static void load(void* t, const float* g) { load_tables_bf3<NT>((__bf16*)t, g); }
+  template <int NW = kWaves>
   __device__ static void conv(float* buf, const void* t, const ConvGeom& G) {
-    conv_lds_bf3(buf, (const __bf16*)t, G);
+    conv_lds_bf3<NW>(buf, (const __bf16*)t, G);
   }
 };
 
@@ -361,7 +380,8 @@ constexpr int kUnrollPx = 2;
 // load(gi) -> any value type; apply(li, gi, valid, value).  li / gi are the
 // LDS and pixel indices of the group's first pixel.  U groups per thread have
 // their loads in flight together (passes that load few arrays take more).
-template <int VW, int U = kUnrollPx, class Load, class Apply>
+// NT: threads in the workgroup.
+template <int VW, int U = kUnrollPx, int NT = kThreads, class Load, class Apply>
 __device__ inline void px_pass(const ConvGeom& G, Load&& load, Apply&& apply) {
   const int q = G.Wp / VW;
   const int total = G.Hp * q;
@@ -369,10 +389,10 @@ __device__ inline void px_pass(const ConvGeom& G, Load&& load, Apply&& apply) {
   // Group e = threadIdx.x + k kThreads sits at row e / q, column VW (e % q);
   // a thread's successive groups advance by (drow, dcol) rows and groups, plus
   // one row when the column wraps -- no per-group integer division.
-  const int drow = kThreads / q, dcol = kThreads - drow * q;
+  const int drow = NT / q, dcol = NT - drow * q;
   int row = threadIdx.x / q, col = threadIdx.x - row * q;
   int lrow = row * G.S1 + col * VW, grow = row * G.W + col * VW;
-  for (int base = threadIdx.x; base < total; base += kThreads * U) {
+  for (int base = threadIdx.x; base < total; base += NT * U) {
     V v[U];
     int li[U], gi[U];
     bool ok[U];
@@ -395,7 +415,7 @@ __device__ inline void px_pass(const ConvGeom& G, Load&& load, Apply&& apply) {
     }
 #pragma unroll
     for (int u = 0; u < U; ++u)
-      if (base + u * kThreads < total) apply(li[u], gi[u], ok[u], v[u]);
+      if (base + u * NT < total) apply(li[u], gi[u], ok[u], v[u]);
   }
 }
 

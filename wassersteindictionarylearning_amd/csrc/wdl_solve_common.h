@@ -184,15 +184,16 @@ int evaluate_epilogue(const double* GL, const double* GDsum, const double* lossP
                       const double* w64, const double* a64, double* g, double* f, int P, int K,
                       long long N, double scale, int expw, int weightsOnly, hipStream_t st);
 
-// Sum over the workgroup, result valid in thread 0.
+// Sum over the workgroup (NW waves), result valid in thread 0.
+template <int NW = kWaves>
 __device__ inline double block_sum(double v) {
-  __shared__ double red[kWaves];
+  __shared__ double red[NW];
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
   double t = 0;
   if (threadIdx.x == 0)
-    for (int w = 0; w < kWaves; ++w) t += red[w];
+    for (int w = 0; w < NW; ++w) t += red[w];
   __syncthreads();
   return t;
 }

@@ -30,15 +30,27 @@ namespace wdl {
 
 // Forward iteration s (1..L): b^{s-1} (geometric-mean step of iteration s-1,
 // :577-592), c^{s-1} = K b^{s-1} (:568), a = p / c (:569-574), phi^s = K a (:575).
+//
+// FP32 passes: sixteen waves per workgroup (two workgroups per CU).  The
+// forward launch has one workgroup per (histogram, atom), fewer than the CUs
+// hold, so each image is worked on by twice the waves of the other stages.
+// The split-BF16x3 passes need more registers and keep eight.
+template <bool BF3> struct FwdCfg {
+  static constexpr int threads = BF3 ? kThreads : 1024;
+  static constexpr int waves = threads / 64;
+  static constexpr int waves_per_simd = BF3 ? 4 : 8;
+};
 template <int VW, bool BF3>
-__global__ __launch_bounds__(kThreads, 4) void k_forward(SolveArgs a, int s) {
+__global__ __launch_bounds__(FwdCfg<BF3>::threads, FwdCfg<BF3>::waves_per_simd) void k_forward(
+    SolveArgs a, int s) {
+  constexpr int kFwdThreads = FwdCfg<BF3>::threads, kFwdWaves = FwdCfg<BF3>::waves;
   typedef typename Vec<VW>::T V;
   int p, i;
   if (!map_block(a, p, i) || s > steps_of(a, p)) return;
   extern __shared__ __align__(16) float smem[];
   void* tabs = smem;
   float* buf = smem + ConvPrec<BF3>::kTabBytes / 4;
-  ConvPrec<BF3>::load(tabs, a.tab);
+  ConvPrec<BF3>::template load<kFwdThreads>(tabs, a.tab);
   const ConvGeom G = a.G;
   const long long N = a.N;
   const int K = a.K;
@@ -46,7 +58,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_forward(SolveArgs a, int s) {
   const long long img = hbase + (long long)i * N;
   if (s == 1) {
     const float* bi = a.bInit ? a.bInit + img : nullptr;
-    px_pass<VW>(
+    px_pass<VW, kUnrollPx, kFwdThreads>(
         G, [&](int gi) { return bi ? vld<VW>(bi + gi) : vsplat<VW>(1.f); },
         [&](int li, int gi, bool ok, V b) { vst<VW>(buf + li, ok ? b : vsplat<VW>(0.f)); });
   } else {
@@ -57,7 +69,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_forward(SolveArgs a, int s) {
     float* Qd = i == 0 ? a.Qh + (long long)(s - 2) * a.qstage + (long long)p * N : nullptr;
     const float* lam = a.lam + (long long)p * K;
     struct LdF { V lq, fi; };
-    px_pass<VW>(
+    px_pass<VW, kUnrollPx, kFwdThreads>(
         G,
         [&](int gi) {
           LdF r;
@@ -76,11 +88,11 @@ __global__ __launch_bounds__(kThreads, 4) void k_forward(SolveArgs a, int s) {
           vst<VW>(buf + li, b);
         });
   }
-  ConvPrec<BF3>::conv(buf, tabs, G);
+  ConvPrec<BF3>::template conv<kFwdWaves>(buf, tabs, G);
   {
     float* Cd = a.Ch + (long long)(s - 1) * a.stage + img;
     const float* Ai = a.A + (long long)i * N;
-    px_pass<VW>(
+    px_pass<VW, kUnrollPx, kFwdThreads>(
         G, [&](int gi) { return vld<VW>(Ai + gi); },
         [&](int li, int gi, bool ok, V av) {
           V u = vsplat<VW>(0.f);
@@ -92,12 +104,12 @@ __global__ __launch_bounds__(kThreads, 4) void k_forward(SolveArgs a, int s) {
           vst<VW>(buf + li, u);
         });
   }
-  ConvPrec<BF3>::conv(buf, tabs, G);
+  ConvPrec<BF3>::template conv<kFwdWaves>(buf, tabs, G);
   float* Fd = a.Fh + (long long)(s - 1) * a.stage + img;
   // the next stage's geometric mean reads safe_log phi^s (V0 / V1 are free
   // until the backward writes them; k_final logs phi^L itself)
   float* Ld = s < steps_of(a, p) ? ((s & 1) ? a.V1 : a.V0) + img : nullptr;
-  px_pass<VW>(
+  px_pass<VW, kUnrollPx, kFwdThreads>(
       G, [&](int gi) { return 0; },
       [&](int li, int gi, bool ok, int) {
         if (!ok) return;
@@ -662,13 +674,17 @@ static void launch_forward(const SolveArgs& a, int grid, int lds, hipStream_t st
   const bool v4 = a.G.W % 4 == 0;
   if (a.bf3) {
     if (v4)
-      hipLaunchKernelGGL((k_forward<4, true>), dim3(grid), dim3(kThreads), lds, st, a, s);
+      hipLaunchKernelGGL((k_forward<4, true>), dim3(grid), dim3(FwdCfg<true>::threads), lds, st,
+                         a, s);
     else
-      hipLaunchKernelGGL((k_forward<1, true>), dim3(grid), dim3(kThreads), lds, st, a, s);
+      hipLaunchKernelGGL((k_forward<1, true>), dim3(grid), dim3(FwdCfg<true>::threads), lds, st,
+                         a, s);
   } else if (v4) {
-    hipLaunchKernelGGL((k_forward<4, false>), dim3(grid), dim3(kThreads), lds, st, a, s);
+    hipLaunchKernelGGL((k_forward<4, false>), dim3(grid), dim3(FwdCfg<false>::threads), lds, st,
+                       a, s);
   } else {
-    hipLaunchKernelGGL((k_forward<1, false>), dim3(grid), dim3(kThreads), lds, st, a, s);
+    hipLaunchKernelGGL((k_forward<1, false>), dim3(grid), dim3(FwdCfg<false>::threads), lds, st,
+                       a, s);
   }
 }
 
