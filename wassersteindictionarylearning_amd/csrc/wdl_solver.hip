@@ -172,8 +172,18 @@ __global__ __launch_bounds__(kThreads) void k_final(SolveArgs a) {
 
 // Backward step s (L..1).  chain 0: dictionary gradient (:600-629);
 // chain 1: weight gradient (:635-673).
+//
+// FP32 passes: twelve waves per workgroup, two workgroups per CU (six waves
+// per SIMD within the register budget); the split-BF16x3 passes keep eight.
+template <bool BF3> struct BwdCfg {
+  static constexpr int threads = BF3 ? kThreads : 768;
+  static constexpr int waves = threads / 64;
+  static constexpr int waves_per_simd = BF3 ? 4 : 6;
+};
 template <int VW, bool BF3>
-__global__ __launch_bounds__(kThreads, 4) void k_backward(SolveArgs a, int t) {
+__global__ __launch_bounds__(BwdCfg<BF3>::threads, BwdCfg<BF3>::waves_per_simd) void k_backward(
+    SolveArgs a, int t) {
+  constexpr int kBwdThreads = BwdCfg<BF3>::threads, kBwdWaves = BwdCfg<BF3>::waves;
   typedef typename Vec<VW>::T V;
   int p, i, chain;
   if (!map_block_bwd(a, p, i, chain)) return;
@@ -182,7 +192,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_backward(SolveArgs a, int t) {
   extern __shared__ __align__(16) float smem[];
   void* tabs = smem;
   float* buf = smem + ConvPrec<BF3>::kTabBytes / 4;
-  ConvPrec<BF3>::load(tabs, a.tab);
+  ConvPrec<BF3>::template load<kBwdThreads>(tabs, a.tab);
   const ConvGeom G = a.G;
   const long long N = a.N;
   const int K = a.K;
@@ -200,7 +210,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_backward(SolveArgs a, int t) {
     const float* Qs = a.Qh + (long long)(s - 1) * a.qstage + (long long)p * N;  // q^s
     const float* Fs = a.Fh + (long long)(s - 1) * a.stage + img;                // phi^s
     const float* G0 = a.G0 + (long long)p * N;
-    px_pass<VW>(
+    px_pass<VW, kUnrollPx, kBwdThreads>(
         G,
         [&](int gi) {
           Three r;  // n, v_i, b^s = q^s / phi^s
@@ -221,9 +231,9 @@ __global__ __launch_bounds__(kThreads, 4) void k_backward(SolveArgs a, int t) {
         [&](int li, int gi, bool ok, const Three& r) {
           vst<VW>(buf + li, ok ? (lpi * r.x - r.y) * r.z : vsplat<VW>(0.f));
         });
-    ConvPrec<BF3>::conv(buf, tabs, G);
+    ConvPrec<BF3>::template conv<kBwdWaves>(buf, tabs, G);
     float* GDp = a.GD + img;
-    px_pass<VW>(
+    px_pass<VW, kUnrollPx, kBwdThreads>(
         G,
         [&](int gi) {
           Three r;  // conv_b, resultDic so far, atom
@@ -243,9 +253,9 @@ __global__ __launch_bounds__(kThreads, 4) void k_backward(SolveArgs a, int t) {
           vst<VW>(buf + li, t);
         });
     if (s == 1) return;
-    ConvPrec<BF3>::conv(buf, tabs, G);
+    ConvPrec<BF3>::template conv<kBwdWaves>(buf, tabs, G);
     const float* Fm = a.Fh + (long long)(s - 2) * a.stage + img;  // phi^{s-1}
-    px_pass<VW>(
+    px_pass<VW, kUnrollPx, kBwdThreads>(
         G, [&](int gi) { return vld<VW>(Fm + gi); },
         [&](int li, int gi, bool ok, V f) {
           if (ok) vst<VW>(Vc + img + gi, fdiv(vld<VW>(buf + li), f));
@@ -257,7 +267,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_backward(SolveArgs a, int t) {
     const float* G0 = a.G0 + (long long)p * N;
     const float* Qp = a.Q + (long long)p * N;
     double dot = 0, sf = 0, sg = 0;
-    px_pass<VW>(
+    px_pass<VW, kUnrollPx, kBwdThreads>(
         G,
         [&](int gi) {
           Three r;  // g, r_i, phi^s
@@ -286,14 +296,14 @@ __global__ __launch_bounds__(kThreads, 4) void k_backward(SolveArgs a, int t) {
           }
           vst<VW>(buf + li, u);
         });
-    const double t = block_sum(dot), tf = block_sum(sf), tg = block_sum(sg);
+    const double t = block_sum<kBwdWaves>(dot), tf = block_sum<kBwdWaves>(sf), tg = block_sum<kBwdWaves>(sg);
     if (threadIdx.x == 0) {
       double* gl = a.GL + (long long)p * K + i;
       *gl = last ? t : *gl + (t - tf / (double)N * (tg - exact_gsum(a, p, s)));
     }
     if (s == 1) return;
-    ConvPrec<BF3>::conv(buf, tabs, G);
-    px_pass<VW>(
+    ConvPrec<BF3>::template conv<kBwdWaves>(buf, tabs, G);
+    px_pass<VW, kUnrollPx, kBwdThreads>(
         G,
         [&](int gi) {
           Two r;
@@ -304,10 +314,10 @@ __global__ __launch_bounds__(kThreads, 4) void k_backward(SolveArgs a, int t) {
         [&](int li, int gi, bool ok, const Two& r) {
           vst<VW>(buf + li, ok ? vld<VW>(buf + li) * fdiv(fdiv(-r.y, r.x), r.x) : vsplat<VW>(0.f));
         });
-    ConvPrec<BF3>::conv(buf, tabs, G);
+    ConvPrec<BF3>::template conv<kBwdWaves>(buf, tabs, G);
     const float* Qm = a.Qh + (long long)(s - 2) * a.qstage + (long long)p * N;  // q^{s-1}
     const float* Fm = a.Fh + (long long)(s - 2) * a.stage + img;                // phi^{s-1}
-    px_pass<VW>(
+    px_pass<VW, kUnrollPx, kBwdThreads>(
         G, [&](int gi) { return fdiv(vld<VW>(Qm + gi), vld<VW>(Fm + gi)); },  // b^{s-1}
         [&](int li, int gi, bool ok, V b) {
           if (ok) vst<VW>(Rc + img + gi, vld<VW>(buf + li) * b);
@@ -700,13 +710,17 @@ static void launch_backward(const SolveArgs& a, int grid1, int lds, hipStream_t 
   const bool v4 = a.G.W % 4 == 0;
   if (a.bf3) {
     if (v4)
-      hipLaunchKernelGGL((k_backward<4, true>), dim3(blocks), dim3(kThreads), lds, st, a, s);
+      hipLaunchKernelGGL((k_backward<4, true>), dim3(blocks), dim3(BwdCfg<true>::threads), lds,
+                         st, a, s);
     else
-      hipLaunchKernelGGL((k_backward<1, true>), dim3(blocks), dim3(kThreads), lds, st, a, s);
+      hipLaunchKernelGGL((k_backward<1, true>), dim3(blocks), dim3(BwdCfg<true>::threads), lds,
+                         st, a, s);
   } else if (v4) {
-    hipLaunchKernelGGL((k_backward<4, false>), dim3(blocks), dim3(kThreads), lds, st, a, s);
+    hipLaunchKernelGGL((k_backward<4, false>), dim3(blocks), dim3(BwdCfg<false>::threads), lds,
+                         st, a, s);
   } else {
-    hipLaunchKernelGGL((k_backward<1, false>), dim3(blocks), dim3(kThreads), lds, st, a, s);
+    hipLaunchKernelGGL((k_backward<1, false>), dim3(blocks), dim3(BwdCfg<false>::threads), lds,
+                         st, a, s);
   }
 }
 
