@@ -289,3 +289,27 @@ def test_autotune_keeps_results(ctx):
     s.close()
     for k in ("grad_lambda", "grad_dic_sum", "bary", "loss"):
         np.testing.assert_array_equal(r[k], ref[k], err_msg=k)
+
+
+def test_full_batched_config_against_reference(ctx):
+    # The batched configuration at full size (128x128, P=64, K=5, L=100,
+    # gamma=1, quadratic loss; the benchmark's own inputs): all 64 histograms in
+    # one batched run, three of them against the compiled reference's FP64
+    # compute_gradient_both (oracle/_ref, GaussianSSE2DKernel), and the plain
+    # form (one solve per launch sequence) equal to the batched one to the bit.
+    from oracle import ref
+    if not ref.available():
+        pytest.skip("oracle/_ref not built")
+    c = synthetic.make_problem("batched_std")
+    n, P, K, L, g, loss = c["n"], c["P"], c["K"], c["L"], c["gamma"], c["loss"]
+    print("full batched configuration %dx%d P=%d K=%d L=%d" % (n, n, P, K, L), flush=True)
+    r = solve_gpu(ctx, c["atoms"], c["weights"], c["obs"], n, n, g, L, loss)
+    plain = solve_gpu(ctx, c["atoms"], c["weights"], c["obs"], n, n, g, L, loss,
+                      mode=wdl.RUN_PLAIN)
+    for k in ("grad_lambda", "grad_dic", "bary", "loss"):
+        np.testing.assert_array_equal(plain[k], r[k], err_msg=k)
+    for p in (0, 31, 63):
+        o = ref.gradient_both(c["atoms"], c["weights"][p], c["obs"][p], n, n, g, L, loss,
+                              variant="sse")
+        for k in ("grad_lambda", "grad_dic", "bary"):
+            assert rel(r[k][p], o[k]) < TOL, (p, k)
