@@ -187,3 +187,30 @@ def test_exact_uneven_weights_only_warm(ctx):
         assert rel(full["grad_lambda"][p], o["grad_lambda"]) < TOL, p
         assert rel(full["grad_dic"][p], o["grad_dic"]) < TOL, p
         assert rel(full["b_temp"][p], o["b_last"]) < TOL, p
+
+
+def test_exact_projection_config_against_reference(ctx):
+    # The projection configuration at full size: 128x128, a K=8 blob dictionary,
+    # gamma=1, weights gradient only, warm-restart store at its initial ones,
+    # exact shifts, each request with its own L_p.  The shortest and the longest
+    # of 8 requests against the compiled reference's FP64
+    # compute_gradient_both_log_signArray (oracle/_ref), whose weight gradient the
+    # weights-only chain computes.
+    from oracle import ref
+    if not ref.available():
+        pytest.skip("oracle/_ref not built")
+    c = synthetic.make_problem("projection", P=8)
+    n, K, L, g, loss = c["n"], c["K"], c["L"], c["gamma"], c["loss"]
+    atoms = synthetic.blob_histograms(np.random.default_rng(7), n, K, c["nblobs"], c["sigma"])
+    iters = c["iterations"]
+    print("exact projection configuration %dx%d K=%d L_p=%s" % (n, n, K, list(iters)), flush=True)
+    f = EXACT | wdl.FLAG_WEIGHTS_ONLY | wdl.FLAG_WARM_RESTART
+    r = run(ctx, atoms, c["weights"], c["obs"], n, n, g, L, loss, f, iters)
+    assert r["events"] == 0
+    for p in (int(np.argmin(iters)), int(np.argmax(iters))):
+        o = ref.gradient_both(atoms, c["weights"][p], c["obs"][p], n, n, g, int(iters[p]), loss,
+                              variant="log")
+        e = rel(r["grad_lambda"][p], o["grad_lambda"])
+        print("request %d, L_p=%d: weight-gradient error %.2e" % (p, iters[p], e), flush=True)
+        assert e < TOL, p
+        assert rel(r["b_temp"][p], o["b_last"]) < TOL, p
