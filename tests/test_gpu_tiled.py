@@ -184,3 +184,26 @@ def test_tiled_errors(ctx):
         wdl.SinkhornSolver(ctx, 256, 256, 2, 1, 3, 1.0, 2, wdl.FLAG_DENSE)
     with pytest.raises(wdl.WdlError):  # log domain at gamma 8 needs a halo above 32
         wdl.SinkhornSolver(ctx, 256, 256, 2, 1, 3, 8.0, 2, wdl.FLAG_LOG_DOMAIN)
+
+
+def test_log_config_full_size_against_reference(ctx):
+    # The small-epsilon log-domain configuration at its full spatial size (the
+    # benchmark's own inputs: 256x256 curve histograms, K=6, gamma=0.5, KL loss,
+    # tiled log-domain kernels), with 8 Sinkhorn iterations instead of 200 so the
+    # compiled reference's FP64 compute_gradient_both_log_signArray (oracle/_ref)
+    # finishes in seconds; one histogram of two.
+    from oracle import ref
+    if not ref.available():
+        pytest.skip("oracle/_ref not built")
+    c = synthetic.make_problem("log_small_eps", P=2)
+    n, K, g, loss = c["n"], c["K"], c["gamma"], c["loss"]
+    L = 8
+    print("log configuration %dx%d K=%d L=%d" % (n, n, K, L), flush=True)
+    r = run(ctx, c["atoms"], c["weights"], c["obs"], n, n, g, L, loss, wdl.FLAG_LOG_DOMAIN)
+    assert r["events"] == 0
+    o = ref.gradient_both(c["atoms"], c["weights"][0], c["obs"][0], n, n, g, L, loss,
+                          variant="log")
+    for k in ("bary", "grad_lambda", "grad_dic"):
+        e = rel(r[k][0], o[k])
+        print("%s: %.2e" % (k, e), flush=True)
+        assert e < TOL, k
