@@ -543,6 +543,47 @@ __global__ __launch_bounds__(kThreads) void k_xdot(SolveArgs a, int t) {
   }
 }
 
+// Weights-only backward step (one chain): the first convolution's input, as
+// k_xb1 forms it for the weight chain (:800-808), and the weight-gradient dot
+// of k_xdot, from the same loads of the K atoms' r and of log phi_i^s; one
+// workgroup per (histogram, atom), the dot's sums in the same order as k_xdot.
+__global__ __launch_bounds__(kThreads) void k_xb1_dot(SolveArgs a, int t) {
+  const int p = a.p0 + (int)(blockIdx.x / a.K), i = (int)(blockIdx.x % a.K);
+  const int s = bwd_step(a, p, t);
+  if (s < 1) return;
+  const bool last = s == steps_of(a, p);
+  const long long N = a.N, hb = (long long)p * a.K * N, img = hb + (long long)i * N;
+  const float lpi = a.lam[(long long)p * a.K + i];
+  const float* Rp = (s & 1) ? a.R0 : a.R1;
+  const double* F = (const double*)a.Fh + (long long)(s - 1) * a.stage + img;
+  double sfg = 0, sf = 0, sg = 0;
+  for (long long x = threadIdx.x; x < N; x += blockDim.x) {
+    float g = 0.f, own = 0.f;
+    if (last) {
+      g = a.G0[(long long)p * N + x] * a.Q[(long long)p * N + x];
+    } else {
+      for_atoms_l(
+          a.K, [&](int j) { return Rp[hb + j * N + x]; },
+          [&](int j, float v) {
+            g -= v;
+            if (j == i) own = v;
+          });
+    }
+    const float xv = lpi * g + own;
+    const double f = F[x];
+    a.xa[img + x] = hlog(fabsf(xv)) - f;  // image (p, i) of the one-chain layout
+    a.sga[img + x] = xv >= 0.f ? 1 : 0;
+    sfg += f * (double)g;
+    sf += f;
+    sg += (double)g;
+  }
+  const double tfg = block_sum(sfg), tf = block_sum(sf), tg = block_sum(sg);
+  if (threadIdx.x == 0) {
+    double* gl = a.GL + (long long)p * a.K + i;
+    *gl = last ? tfg : *gl + (tfg - tf / (double)N * (tg - exact_gsum(a, p, s)));
+  }
+}
+
 static dim3 ew_grid(long long n) {
   return dim3((unsigned)std::min<long long>((n + 255) / 256, 8192));
 }
@@ -572,8 +613,12 @@ int exact_backward(const SolveArgs& a, hipStream_t st, int t) {
   const int W = a.G.W, H = a.G.H;
   const double g = a.gamma;
   int e;
-  hipLaunchKernelGGL(k_xdot, dim3((unsigned)(a.np * a.K)), dim3(kThreads), 0, st, a, t);
-  hipLaunchKernelGGL(k_xb1, ew_grid(n), dim3(256), 0, st, a, t, nch);
+  if (a.weightsOnly) {
+    hipLaunchKernelGGL(k_xb1_dot, dim3((unsigned)(a.np * a.K)), dim3(kThreads), 0, st, a, t);
+  } else {
+    hipLaunchKernelGGL(k_xdot, dim3((unsigned)(a.np * a.K)), dim3(kThreads), 0, st, a, t);
+    hipLaunchKernelGGL(k_xb1, ew_grid(n), dim3(256), 0, st, a, t, nch);
+  }
   // histogram p runs this step when L_p >= L - t + 1, the second convolution when >= L - t + 2
   if ((e = xlse_conv(a.xa + off, a.sga + off, a.xb + off, a.sgb + off, a.xt + off, a.sgt + off, W, H,
                      nimg, g, a.Lp, a.p0, nch * a.K, a.L - t + 1, st)))
