@@ -400,25 +400,31 @@ __device__ inline long long xoff(const SolveArgs& a, int nch, const XEl& r) {
 
 // forward, input of the first convolution: log b^{s-1} (:711-724; b^0: 0 or the store)
 __global__ void k_xf1(SolveArgs a, int s) {
-  const long long tot = (long long)a.np * a.K * a.N;
+  // one thread per (histogram, pixel): the geometric mean of the K atoms'
+  // phi^{s-1} is formed once and gives all K images' b
+  const long long N = a.N, tot = (long long)a.np * N;
   XLOOP(tot) {
-    const XEl r = xel(a, 1, e);
-    double b;
-    if (s > steps_of(a, r.p)) continue;
+    const int p = a.p0 + (int)(e / N);
+    const long long x = e % N;
+    if (s > steps_of(a, p)) continue;
+    const long long hb = (long long)p * a.K * N;
     if (s == 1) {
-      b = a.bInit ? (double)a.bInit[r.img + r.x] : 0.0;
+      for (int i = 0; i < a.K; ++i)
+        a.xa[hb + i * N + x] = a.bInit ? (double)a.bInit[hb + i * N + x] : 0.0;
     } else {
-      const double* Fp = (const double*)a.Fh + (long long)(s - 2) * a.stage +
-                         (long long)r.p * a.K * a.N + r.x;
+      const double* Fp = (const double*)a.Fh + (long long)(s - 2) * a.stage + hb + x;
+      double* Bp = (double*)a.Bh + (long long)(s - 2) * a.stage + hb + x;
       double lq = 0;
-      const float* lam = a.lam + (long long)r.p * a.K;
+      const float* lam = a.lam + (long long)p * a.K;
       for_atoms_l<4>(
-          a.K, [&](int j) { return Fp[j * a.N]; },
+          a.K, [&](int j) { return Fp[j * N]; },
           [&](int j, double f) { lq += (double)lam[j] * f; });
-      b = lq - Fp[(long long)r.i * a.N];
-      ((double*)a.Bh)[(long long)(s - 2) * a.stage + r.img + r.x] = b;
+      for (int i = 0; i < a.K; ++i) {
+        const double b = lq - Fp[i * N];
+        Bp[i * N] = b;
+        a.xa[hb + i * N + x] = b;  // image (p, i) of the one-chain layout
+      }
     }
-    a.xa[xoff(a, 1, r)] = b;
   }
 }
 
@@ -594,7 +600,7 @@ int exact_forward(const SolveArgs& a, hipStream_t st, int s) {
   const int W = a.G.W, H = a.G.H;
   const double g = a.gamma;
   int e;
-  hipLaunchKernelGGL(k_xf1, ew_grid(n), dim3(256), 0, st, a, s);
+  hipLaunchKernelGGL(k_xf1, ew_grid((long long)a.np * a.N), dim3(256), 0, st, a, s);
   if ((e = xlse_conv(a.xa + off, nullptr, (double*)a.Ch + (long long)(s - 1) * a.stage + off,
                      nullptr, a.xt + off, nullptr, W, H, nimg, g, a.Lp, a.p0, a.K, s, st)))
     return e;
