@@ -169,11 +169,14 @@ def test_exact_solver_far_transport(ctx):
         assert rel(ex["grad_dic"][p], o["grad_dic"]) < TOL, p
 
 
-def test_exact_uneven_weights_only_warm(ctx):
-    W = H = 20
+@pytest.mark.parametrize("W,H", [(20, 20), (48, 48)])
+def test_exact_uneven_weights_only_warm(ctx, W, H):
+    # the weights-only backward (one fused kernel per step) against the full
+    # one (separate dot and input kernels) bit for bit, at images of fewer and
+    # of more pixels than a workgroup has threads
     K, P, L, g = 3, 4, 9, 1.0
     iters = np.array([5, 9, 7, 6], np.int32)
-    print("exact uneven iterations", flush=True)
+    print("exact uneven iterations W=%d H=%d" % (W, H), flush=True)
     rng = np.random.default_rng(77)
     obs = synthetic.blob_histograms(rng, W, P, (1, 3), (2, 5))
     atoms, lam = synthetic.init_atoms_weights(rng, K, P, W * H)
