@@ -63,6 +63,7 @@ struct SolveArgs {
   double* xa;
   double* xb;
   double* xt;
+  double* xdp;  // per-chunk partial sums of the weight-gradient dots
   unsigned char* sga;
   unsigned char* sgb;
   unsigned char* sgt;
@@ -173,6 +174,11 @@ int exact_backward(const SolveArgs& a, hipStream_t st, int t);
 // WassersteinLoss at the barycenters: fills G0, lossPart, gsumPart (wdl_xlse.hip).
 int wasserstein_loss(const SolveArgs& a, hipStream_t st);
 constexpr int kXMaxLen = 1024;  // longest line of the exact-shift passes
+// Pixels per workgroup of the exact-shift weight-gradient dot, and chunks per image.
+constexpr int kXDotChunk = 1024;
+__host__ __device__ inline long long xdot_chunks(long long N) {
+  return (N + kXDotChunk - 1) / kXDotChunk;
+}
 
 // evaluate_both around the batched solve, on the device (wdl_evaluate.hip):
 // x -> normalised weights / atoms (FP64 w64, a64 and the solve's FP32 copies),

@@ -520,50 +520,30 @@ __global__ void k_xb3(SolveArgs a, int t, int nch) {
   }
 }
 
-// weight-gradient dot of backward step s, <log phi_i^s, g_s> (:786), centred as
-// in the other solvers (exact_gsum); one workgroup per (histogram, atom)
-__global__ __launch_bounds__(kThreads) void k_xdot(SolveArgs a, int t) {
-  const int p = a.p0 + (int)(blockIdx.x / a.K), i = (int)(blockIdx.x % a.K);
+// Weight-gradient dot of backward step s, <log phi_i^s, g_s> (:786), centred as
+// in the other solvers (exact_gsum), in two kernels.  k_xdot_chunk: one
+// workgroup per (histogram, atom, chunk of kXDotChunk pixels) writes the
+// chunk's sums of log phi * g, log phi and g to xdp; with FUSED (the
+// weights-only backward, one chain) it also forms the first convolution's
+// input as k_xb1 does for the weight chain (:800-808), from the same loads of
+// the K atoms' r and of log phi_i^s.  k_xdot_fin adds the chunks in order, so
+// both forms give the same dots to the bit.
+template <bool FUSED>
+__global__ __launch_bounds__(kThreads) void k_xdot_chunk(SolveArgs a, int t) {
+  const long long N = a.N, nc = xdot_chunks(N);
+  const long long ic = blockIdx.x / nc;
+  const int c = (int)(blockIdx.x - ic * nc);
+  const int p = a.p0 + (int)(ic / a.K), i = (int)(ic % a.K);
   const int s = bwd_step(a, p, t);
   if (s < 1) return;
   const bool last = s == steps_of(a, p);
-  const long long N = a.N, hb = (long long)p * a.K * N;
-  const double* F = (const double*)a.Fh + (long long)(s - 1) * a.stage + hb + (long long)i * N;
-  const float* Rp = (s & 1) ? a.R0 : a.R1;
-  double sfg = 0, sf = 0, sg = 0;
-  for (long long x = threadIdx.x; x < N; x += blockDim.x) {
-    float g = 0.f;
-    if (last) {
-      g = a.G0[(long long)p * N + x] * a.Q[(long long)p * N + x];
-    } else {
-      for_atoms_l(a.K, [&](int j) { return Rp[hb + j * N + x]; }, [&](int, float v) { g -= v; });
-    }
-    sfg += F[x] * (double)g;
-    sf += F[x];
-    sg += (double)g;
-  }
-  const double tfg = block_sum(sfg), tf = block_sum(sf), tg = block_sum(sg);
-  if (threadIdx.x == 0) {
-    double* gl = a.GL + (long long)p * a.K + i;
-    *gl = last ? tfg : *gl + (tfg - tf / (double)N * (tg - exact_gsum(a, p, s)));
-  }
-}
-
-// Weights-only backward step (one chain): the first convolution's input, as
-// k_xb1 forms it for the weight chain (:800-808), and the weight-gradient dot
-// of k_xdot, from the same loads of the K atoms' r and of log phi_i^s; one
-// workgroup per (histogram, atom), the dot's sums in the same order as k_xdot.
-__global__ __launch_bounds__(kThreads) void k_xb1_dot(SolveArgs a, int t) {
-  const int p = a.p0 + (int)(blockIdx.x / a.K), i = (int)(blockIdx.x % a.K);
-  const int s = bwd_step(a, p, t);
-  if (s < 1) return;
-  const bool last = s == steps_of(a, p);
-  const long long N = a.N, hb = (long long)p * a.K * N, img = hb + (long long)i * N;
+  const long long hb = (long long)p * a.K * N, img = hb + (long long)i * N;
   const float lpi = a.lam[(long long)p * a.K + i];
   const float* Rp = (s & 1) ? a.R0 : a.R1;
   const double* F = (const double*)a.Fh + (long long)(s - 1) * a.stage + img;
+  const long long xe = (long long)(c + 1) * kXDotChunk, x1 = xe < N ? xe : N;
   double sfg = 0, sf = 0, sg = 0;
-  for (long long x = threadIdx.x; x < N; x += blockDim.x) {
+  for (long long x = (long long)c * kXDotChunk + threadIdx.x; x < x1; x += blockDim.x) {
     float g = 0.f, own = 0.f;
     if (last) {
       g = a.G0[(long long)p * N + x] * a.Q[(long long)p * N + x];
@@ -575,18 +555,41 @@ __global__ __launch_bounds__(kThreads) void k_xb1_dot(SolveArgs a, int t) {
             if (j == i) own = v;
           });
     }
-    const float xv = lpi * g + own;
     const double f = F[x];
-    a.xa[img + x] = hlog(fabsf(xv)) - f;  // image (p, i) of the one-chain layout
-    a.sga[img + x] = xv >= 0.f ? 1 : 0;
+    if (FUSED) {
+      const float xv = lpi * g + own;
+      a.xa[img + x] = hlog(fabsf(xv)) - f;  // image (p, i) of the one-chain layout
+      a.sga[img + x] = xv >= 0.f ? 1 : 0;
+    }
     sfg += f * (double)g;
     sf += f;
     sg += (double)g;
   }
   const double tfg = block_sum(sfg), tf = block_sum(sf), tg = block_sum(sg);
   if (threadIdx.x == 0) {
+    double* o = a.xdp + (((long long)p * a.K + i) * nc + c) * 3;
+    o[0] = tfg;
+    o[1] = tf;
+    o[2] = tg;
+  }
+}
+
+__global__ void k_xdot_fin(SolveArgs a, int t) {
+  const long long N = a.N, nc = xdot_chunks(N);
+  const long long tot = (long long)a.np * a.K;
+  XLOOP(tot) {
+    const int p = a.p0 + (int)(e / a.K), i = (int)(e % a.K);
+    const int s = bwd_step(a, p, t);
+    if (s < 1) continue;
+    const double* q = a.xdp + ((long long)p * a.K + i) * nc * 3;
+    double tfg = 0, tf = 0, tg = 0;
+    for (long long c = 0; c < nc; ++c) {
+      tfg += q[3 * c];
+      tf += q[3 * c + 1];
+      tg += q[3 * c + 2];
+    }
     double* gl = a.GL + (long long)p * a.K + i;
-    *gl = last ? tfg : *gl + (tfg - tf / (double)N * (tg - exact_gsum(a, p, s)));
+    *gl = s == steps_of(a, p) ? tfg : *gl + (tfg - tf / (double)N * (tg - exact_gsum(a, p, s)));
   }
 }
 
@@ -619,12 +622,14 @@ int exact_backward(const SolveArgs& a, hipStream_t st, int t) {
   const int W = a.G.W, H = a.G.H;
   const double g = a.gamma;
   int e;
+  const dim3 gdot((unsigned)((long long)a.np * a.K * xdot_chunks(a.N)));
   if (a.weightsOnly) {
-    hipLaunchKernelGGL(k_xb1_dot, dim3((unsigned)(a.np * a.K)), dim3(kThreads), 0, st, a, t);
+    hipLaunchKernelGGL(k_xdot_chunk<true>, gdot, dim3(kThreads), 0, st, a, t);
   } else {
-    hipLaunchKernelGGL(k_xdot, dim3((unsigned)(a.np * a.K)), dim3(kThreads), 0, st, a, t);
+    hipLaunchKernelGGL(k_xdot_chunk<false>, gdot, dim3(kThreads), 0, st, a, t);
     hipLaunchKernelGGL(k_xb1, ew_grid(n), dim3(256), 0, st, a, t, nch);
   }
+  hipLaunchKernelGGL(k_xdot_fin, ew_grid((long long)a.np * a.K), dim3(256), 0, st, a, t);
   // histogram p runs this step when L_p >= L - t + 1, the second convolution when >= L - t + 2
   if ((e = xlse_conv(a.xa + off, a.sga + off, a.xb + off, a.sgb + off, a.xt + off, a.sgt + off, W, H,
                      nimg, g, a.Lp, a.p0, nch * a.K, a.L - t + 1, st)))
