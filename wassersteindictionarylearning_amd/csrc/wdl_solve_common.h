@@ -175,7 +175,7 @@ int exact_backward(const SolveArgs& a, hipStream_t st, int t);
 int wasserstein_loss(const SolveArgs& a, hipStream_t st);
 constexpr int kXMaxLen = 1024;  // longest line of the exact-shift passes
 // Pixels per workgroup of the exact-shift weight-gradient dot, and chunks per image.
-constexpr int kXDotChunk = 1024;
+constexpr int kXDotChunk = 4096;
 __host__ __device__ inline long long xdot_chunks(long long N) {
   return (N + kXDotChunk - 1) / kXDotChunk;
 }
