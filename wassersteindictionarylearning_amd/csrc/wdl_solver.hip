@@ -736,7 +736,7 @@ struct wdl_solver {
   int D = 0;
   DevBuf tab, A, lam, obs, Bh, Qh, Ch, Fh, V0, V1, R0, R1, GD, GL, Q, G0, lossPart, bStore,
       bTemp, GDsum, logA, events, Lp, X0, X1, XS0, XS1, GLt, gsumPart, Mt, eta, xa, xb, xt, xdp,
-      sga, sgb, sgt, wla, wlb, wlc, wlt, evX, evG, evW, evA, evF;
+      nj, gj, sga, sgb, sgt, wla, wlb, wlc, wlt, evX, evG, evW, evA, evF;
   bool tiled = false;  // 64 x 64 tiles with a halo (a side above kMaxSide)
   bool exact = false;  // log domain with the exact-shift line passes (WDL_FLAG_EXACT_LSE)
   TileGeom tg{};
@@ -803,6 +803,8 @@ struct wdl_solver {
     a.xb = xb.as<double>();
     a.xt = xt.as<double>();
     a.xdp = xdp.as<double>();
+    a.nj = nj.as<float>();
+    a.gj = gj.as<float>();
     a.sga = sga.as<unsigned char>();
     a.sgb = sgb.as<unsigned char>();
     a.sgt = sgt.as<unsigned char>();
@@ -970,9 +972,10 @@ static long long footprint(int W, int H, int K, int P, int L, int flags) {
     f += 3LL * 2 * PKN * (8 + 1);
     f += (long long)P * K * xdot_chunks((long long)W * H) * 3 * 8;
   }
-  if (use_tiles(W, H, flags)) {  // X0, X1 (+ sign bytes), per-tile dots
+  if (use_tiles(W, H, flags)) {  // X0, X1 (+ sign bytes), per-tile dots, log: cross-atom sums
     const long long nt = (long long)((W + kTile - 1) / kTile) * ((H + kTile - 1) / kTile);
     f += 2 * PKN * (logd ? 9 : 4) + (long long)P * K * nt * 8 * (1 + L);
+    if (logd) f += 2LL * P * N * 4;
   }
   return f;
 }
@@ -1071,6 +1074,8 @@ int wdl_solver_create(wdl_ctx* ctx, int W, int H, int K, int P, int L, double ga
       {&s->X1, tiled ? (size_t)PKN * hsz : 0},
       {&s->XS0, tiled && (flags & WDL_FLAG_LOG_DOMAIN) ? (size_t)PKN : 0},
       {&s->XS1, tiled && (flags & WDL_FLAG_LOG_DOMAIN) ? (size_t)PKN : 0},
+      {&s->nj, tiled && (flags & WDL_FLAG_LOG_DOMAIN) ? (size_t)P * N * f4 : 0},
+      {&s->gj, tiled && (flags & WDL_FLAG_LOG_DOMAIN) ? (size_t)P * N * f4 : 0},
       {&s->GLt, tiled ? (size_t)P * K * ntiles * sizeof(double) : 0},
       {&s->Mt, tiled ? (size_t)L * P * K * ntiles * sizeof(double) : 0},
       {&s->gsumPart, (size_t)P * kLossBlocks * sizeof(double)},
@@ -1142,7 +1147,7 @@ int wdl_solver_destroy(wdl_solver* s) {
   DevBuf* bufs[] = {&s->tab, &s->A,  &s->lam, &s->obs, &s->Bh, &s->Qh, &s->Ch,  &s->Fh,
                     &s->V0,  &s->V1, &s->R0,  &s->R1,  &s->GD, &s->GL,       &s->Q,
                     &s->G0,  &s->lossPart,    &s->bStore,      &s->bTemp,    &s->GDsum,
-                    &s->logA, &s->events, &s->Lp, &s->X0, &s->X1, &s->XS0, &s->XS1, &s->GLt, &s->Mt, &s->gsumPart, &s->eta, &s->xa, &s->xb, &s->xt, &s->xdp, &s->sga, &s->sgb,
+                    &s->logA, &s->events, &s->Lp, &s->X0, &s->X1, &s->XS0, &s->XS1, &s->GLt, &s->Mt, &s->gsumPart, &s->eta, &s->xa, &s->xb, &s->xt, &s->xdp, &s->nj, &s->gj, &s->sga, &s->sgb,
                     &s->sgt, &s->wla, &s->wlb, &s->wlc,
                     &s->wlt, &s->evX, &s->evG, &s->evW, &s->evA, &s->evF};
   for (DevBuf* b : bufs) b->release();

@@ -411,6 +411,55 @@ __device__ inline double tdot(doublex4 a, floatx4 b) {
   return a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w;
 }
 
+// Log domain, forward step s >= 2: log b^{s-1} = lq - log phi_i^{s-1} of every
+// atom, lq = sum_j lam_j log phi_j^{s-1} (:711-724), into the history Bh[s-2].
+// One thread per (histogram, pixel) forms lq once for the K images; the tiles
+// of k_tfwd1_log then load one image over their region instead of K.
+__global__ __launch_bounds__(256) void k_tlogb(SolveArgs a, int s) {
+  const long long N = a.N, tot = (long long)a.np * N;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < tot;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int p = a.p0 + (int)(e / N);
+    const long long x = e % N;
+    if (s > steps_of(a, p)) continue;
+    const long long hb = (long long)p * a.K * N;
+    const double* Fp = (const double*)a.Fh + (long long)(s - 2) * a.stage + hb + x;
+    double* Bp = (double*)a.Bh + (long long)(s - 2) * a.stage + hb + x;
+    const float* lam = a.lam + (long long)p * a.K;
+    double lq = 0;
+    for_atoms_l<4>(
+        a.K, [&](int j) { return Fp[j * N]; },
+        [&](int j, double f) { lq += (double)lam[j] * f; });
+    for (int i = 0; i < a.K; ++i) Bp[i * N] = lq - Fp[i * N];
+  }
+}
+
+// Log domain, backward launch t: the cross-atom sums of step s (not the last)
+// that every atom's workgroup of k_tbwd1_log needs, n = -sum_j v_j (:764) and
+// g = -sum_j r_j (:808), once per (histogram, pixel), summed in atom order as
+// the tiles did.
+__global__ __launch_bounds__(256) void k_tjoin(SolveArgs a, int t) {
+  const long long N = a.N, tot = (long long)a.np * N;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < tot;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int p = a.p0 + (int)(e / N);
+    const long long x = e % N;
+    const int s = bwd_step(a, p, t);
+    if (s < 1 || s == steps_of(a, p)) continue;
+    const long long hb = (long long)p * a.K * N;
+    if (!a.weightsOnly) {
+      const float* Vp = ((s & 1) ? a.V0 : a.V1) + hb + x;
+      float n = 0.f;
+      for_atoms_l(a.K, [&](int j) { return Vp[j * N]; }, [&](int, float v) { n -= v; });
+      a.nj[(long long)p * N + x] = n;
+    }
+    const float* Rp = ((s & 1) ? a.R0 : a.R1) + hb + x;
+    float g = 0.f;
+    for_atoms_l(a.K, [&](int j) { return Rp[j * N]; }, [&](int, float v) { g -= v; });
+    a.gj[(long long)p * N + x] = g;
+  }
+}
+
 // log c^{s-1} = K log b^{s-1}, log b^{s-1} = sum_j lam_j log phi_j^{s-1} - log phi_i^{s-1}
 // (:711-724; b^0: 0 or the warm-start store), also stored as the history Bh.
 template <int VW>
@@ -435,29 +484,11 @@ __global__ __launch_bounds__(kThreads, 6) void k_tfwd1_log(SolveArgs a, int s) {
         [&](int gi) { return bi ? vld<VW>(bi + gi) : vsplat<VW>(0.f); },
         [&](int, int, V b, bool) { return SLog<VW>{vd(b), vsplat<VW>(1.f)}; });
   } else {
-    const double* Fp = (const double*)a.Fh + (long long)(s - 2) * a.stage + hbase;
-    double* Bd = (double*)a.Bh + (long long)(s - 2) * a.stage + img;
-    const float* lam = a.lam + (long long)p * K;
-    struct TwoD { VD x, y; };
+    // log b^{s-1}, written to the history by k_tlogb
+    const double* Bd = (const double*)a.Bh + (long long)(s - 2) * a.stage + img;
     tile_log_conv<VW>(
-        sm.buf, sm.tabs, sm.L, T, tx, ty, a.events,
-        [&](int gi) {
-          TwoD r;
-          r.x = vd(vsplat<VW>(0.f));
-          r.y = vd(vsplat<VW>(0.f));
-          for_atoms_l<4>(
-              K, [&](int j) { return vldd<VW>(Fp + (long long)j * N + gi); },
-              [&](int j, const VD& f) {
-                r.x += (double)lam[j] * f;
-                if (j == i) r.y = f;
-              });
-          return r;
-        },
-        [&](int li, int gi, const TwoD& r, bool second) {
-          const VD b = r.x - r.y;
-          if (second && tile_inner(T, li)) vstd<VW>(Bd + gi, b);
-          return SLog<VW>{b, vsplat<VW>(1.f)};
-        });
+        sm.buf, sm.tabs, sm.L, T, tx, ty, a.events, [&](int gi) { return vldd<VW>(Bd + gi); },
+        [&](int, int, const VD& b, bool) { return SLog<VW>{b, vsplat<VW>(1.f)}; });
   }
   double* Cd = (double*)a.Ch + (long long)(s - 1) * a.stage + img;
   tile_inner_pass<VW>(
@@ -532,6 +563,8 @@ __global__ __launch_bounds__(kThreads, 6) void k_tbwd1_log(SolveArgs a, int t) {
   const float* LA = a.logA + (long long)i * N;
   const double* Cs = (const double*)a.Ch + (long long)(s - 1) * a.stage + img;  // log c^{s-1}
   const float* G0 = a.G0 + (long long)p * N;
+  const float* NJ = a.nj + (long long)p * N;
+  const float* GJ = a.gj + (long long)p * N;
   const bool last = (s == steps_of(a, p));
   struct Three { V x, y; VD z; };
   struct Mid { VD c; V la, gd; };
@@ -556,12 +589,8 @@ __global__ __launch_bounds__(kThreads, 6) void k_tbwd1_log(SolveArgs a, int t) {
             r.x = vld<VW>(G0 + gi);
             r.y = vsplat<VW>(0.f);
           } else {
-            r.x = vsplat<VW>(0.f);
-            r.y = vsplat<VW>(0.f);
-            for_atoms<VW>(Vp + hbase + gi, N, K, [&](int j, V v) {
-              r.x -= v;  // n = -sum_i v_i (:764)
-              if (j == i) r.y = v;
-            });
+            r.x = vld<VW>(NJ + gi);  // n = -sum_i v_i (:764), from k_tjoin
+            r.y = vld<VW>(Vp + img + gi);
           }
           return r;
         },
@@ -602,12 +631,8 @@ __global__ __launch_bounds__(kThreads, 6) void k_tbwd1_log(SolveArgs a, int t) {
         r.x = vld<VW>(G0 + gi) * vld<VW>(Qp + gi);
         r.y = vsplat<VW>(0.f);
       } else {
-        r.x = vsplat<VW>(0.f);
-        r.y = vsplat<VW>(0.f);
-        for_atoms<VW>(Rp + hbase + gi, N, K, [&](int j, V v) {
-          r.x -= v;  // g = -sum_i r_i (:808)
-          if (j == i) r.y = v;
-        });
+        r.x = vld<VW>(GJ + gi);  // g = -sum_i r_i (:808), from k_tjoin
+        r.y = vld<VW>(Rp + img + gi);
       }
       return r;
     };
@@ -738,6 +763,11 @@ int tiled_forward(const SolveArgs& a, hipStream_t st, int s) {
   const int grid = tile_grid(a);
   const bool v4 = a.T.W % 4 == 0;
   if (a.logA) {
+    if (s >= 2) {
+      const long long n = (long long)a.np * a.N;
+      hipLaunchKernelGGL(k_tlogb, dim3((unsigned)std::min<long long>((n + 255) / 256, 8192)),
+                         dim3(256), 0, st, a, s);
+    }
     WDL_TLAUNCH(k_tfwd1_log, grid, a, s);
     WDL_TLAUNCH(k_tfwd2_log, grid, a, s);
   } else {
@@ -762,6 +792,9 @@ int tiled_backward(const SolveArgs& a, hipStream_t st, int t) {
   const int grid = (a.weightsOnly ? 1 : 2) * tile_grid(a);
   const bool v4 = a.T.W % 4 == 0;
   if (a.logA) {
+    const long long n = (long long)a.np * a.N;
+    hipLaunchKernelGGL(k_tjoin, dim3((unsigned)std::min<long long>((n + 255) / 256, 8192)),
+                       dim3(256), 0, st, a, t);
     WDL_TLAUNCH(k_tbwd1_log, grid, a, t);
     if (t >= 2) WDL_TLAUNCH(k_tbwd2_log, grid, a, t);
   } else {
