@@ -16,8 +16,8 @@
 //
 // The solver stages (exact_forward / exact_backward) run the log-domain
 // recursions of compute_gradient_both_log_signArray (inverseWasserstein.h:
-// 678-814) as elementwise kernels around these passes, with FP64 scratch
-// images between them.
+// 678-814) as elementwise kernels before the passes and as output stages of
+// their second pass, with FP64 scratch images between them.
 #include <algorithm>
 #include <cmath>
 
@@ -175,11 +175,11 @@ constexpr int kXMaxThreads = kXWaves * 4 * 64;
 // (any shift near the maximum serves: the output adds back the shift the terms
 // were formed with); exponents are formed in FP64 in log2 units and
 // exponentiated on the hardware exp2.
-template <bool SIGNED>
+template <bool SIGNED, class Epi>
 __global__ __launch_bounds__(kXMaxThreads) void k_xlse_pass(
-    const double* __restrict__ in, const unsigned char* __restrict__ sin, double* __restrict__ out,
-    unsigned char* __restrict__ sout, int rows, int len, long long nlines, double inv_gamma,
-    const int* __restrict__ Lp, int p0, int ipp, int active_thr) {
+    const double* __restrict__ in, const unsigned char* __restrict__ sin, Epi epi, int rows,
+    int len, long long nlines, double inv_gamma, const int* __restrict__ Lp, int p0, int ipp,
+    int active_thr) {
   extern __shared__ __align__(16) double xs[];
   const int lp = len + 2 * kXPad;
   const int wpl = xlse_waves_per_line(len);
@@ -311,51 +311,151 @@ __global__ __launch_bounds__(kXMaxThreads) void k_xlse_pass(
         ob = img * (long long)rows * len + (ln - img * rows);
       }
       if (act)
-        for (int j = threadIdx.x / kXWaves; j < len; j += blockDim.x / kXWaves) {
-          const long long oi = ob + (long long)j * rows;
-          out[oi] = O0[ws * len + j];
-          if (SIGNED) sout[oi] = SO0[ws * len + j];
-        }
+        for (int j = threadIdx.x / kXWaves; j < len; j += blockDim.x / kXWaves)
+          epi(ob + (long long)j * rows, O0[ws * len + j], SIGNED ? SO0[ws * len + j] : 1);
     }
     __syncthreads();
   }
 }
 
+// Output stage of k_xlse_pass, called once per output (index oi into the
+// image layout of the pass, value, sign byte): the plain stage stores them.
+struct XStore {
+  double* out;
+  unsigned char* sout;  // nullptr without signs
+  __device__ void operator()(long long oi, double v, unsigned char sg) const {
+    out[oi] = v;
+    if (sout) sout[oi] = sg;
+  }
+};
+
+template <bool SIGNED, class Epi>
+static int xlse_set_attr(int lds) {
+  static int done = 0;
+  if (lds <= done) return WDL_OK;
+  WDL_HIP(hipFuncSetAttribute((const void*)k_xlse_pass<SIGNED, Epi>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  done = lds;
+  return WDL_OK;
+}
+
+// One pass over nimg images of rows lines of length len (see k_xlse_pass).
+template <bool SIGNED, class Epi>
+static void xlse_launch(const double* in, const unsigned char* sin, Epi epi, int rows, int len,
+                        long long nimg, double ig, const int* Lp, int p0, int ipp, int thr,
+                        hipStream_t st) {
+  const long long nl = nimg * rows;
+  const int grid = (int)std::min<long long>((nl + kXWaves - 1) / kXWaves, 1 << 20);
+  const int lds = xlse_lds_bytes(len);
+  const int block = kXWaves * 64 * xlse_waves_per_line(len);
+  hipLaunchKernelGGL((k_xlse_pass<SIGNED, Epi>), dim3(grid), dim3(block), lds, st, in, sin, epi,
+                     rows, len, nl, ig, Lp, p0, ipp, thr);
+}
+
 // Two passes (along x, then along y) over nimg W x H images; tmp holds the
 // transposed intermediate.  Signs: none (sin == nullptr) or one byte per value.
-static int xlse_conv(const double* in, const unsigned char* sin, double* out, unsigned char* sout,
-                     double* tmp, unsigned char* stmp, int W, int H, long long nimg, double gamma,
-                     const int* Lp, int p0, int ipp, int thr, hipStream_t st) {
+// The second pass hands its outputs to epi (XStore: into out / sout).
+template <class Epi>
+static int xlse_conv_e(const double* in, const unsigned char* sin, Epi epi, double* tmp,
+                       unsigned char* stmp, int W, int H, long long nimg, double gamma,
+                       const int* Lp, int p0, int ipp, int thr, hipStream_t st) {
   const double ig = 1.0 / gamma;
   const int thr_ = Lp ? thr : 0;
-  auto launch = [&](const double* i, const unsigned char* si, double* o, unsigned char* so,
-                    int rows, int len) {
-    const long long nl = nimg * rows;
-    const int grid = (int)std::min<long long>((nl + kXWaves - 1) / kXWaves, 1 << 20);
-    const int lds = xlse_lds_bytes(len);
-    const int block = kXWaves * 64 * xlse_waves_per_line(len);
-    if (si)
-      hipLaunchKernelGGL(k_xlse_pass<true>, dim3(grid), dim3(block), lds, st, i, si, o, so,
-                         rows, len, nl, ig, Lp, p0, ipp, thr_);
-    else
-      hipLaunchKernelGGL(k_xlse_pass<false>, dim3(grid), dim3(block), lds, st, i, si, o,
-                         so, rows, len, nl, ig, Lp, p0, ipp, thr_);
-  };
-  launch(in, sin, tmp, sin ? stmp : nullptr, H, W);   // along x: lines are rows
-  launch(tmp, sin ? stmp : nullptr, out, sout, W, H);  // along y: lines are columns
+  if (sin) {
+    xlse_launch<true>(in, sin, XStore{tmp, stmp}, H, W, nimg, ig, Lp, p0, ipp, thr_, st);  // rows
+    xlse_launch<true>(tmp, stmp, epi, W, H, nimg, ig, Lp, p0, ipp, thr_, st);  // columns
+  } else {
+    xlse_launch<false>(in, nullptr, XStore{tmp, nullptr}, H, W, nimg, ig, Lp, p0, ipp, thr_, st);
+    xlse_launch<false>(tmp, nullptr, epi, W, H, nimg, ig, Lp, p0, ipp, thr_, st);
+  }
   WDL_HIP(hipGetLastError());
   return WDL_OK;
 }
 
+static int xlse_conv(const double* in, const unsigned char* sin, double* out, unsigned char* sout,
+                     double* tmp, unsigned char* stmp, int W, int H, long long nimg, double gamma,
+                     const int* Lp, int p0, int ipp, int thr, hipStream_t st) {
+  return xlse_conv_e(in, sin, XStore{out, sin ? sout : nullptr}, tmp, stmp, W, H, nimg, gamma, Lp,
+                     p0, ipp, thr, st);
+}
+
+// Solver output stages fused into the second pass (the image layout of the
+// scratch arrays: image img = (histogram - p0, chain, atom), pixel x).
+
+// forward, first convolution: c^{s-1} into the history and the second
+// convolution's input log p - c^{s-1} (:704-707)
+struct XEpiFwd1 {
+  double* ch;  // Ch[s-1] at the first image
+  double* xa;  // the solver's xa at the first image
+  const float* logA;
+  int K;
+  long long N;
+  __device__ void operator()(long long oi, double c, unsigned char) const {
+    const long long img = oi / N, x = oi - img * N;
+    ch[oi] = c;
+    xa[oi] = (double)logA[(img % K) * N + x] - c;
+  }
+};
+
+// backward step s, after the first convolution: resultDic (chain 0, :750-752)
+// and, for s >= 2, the second convolution's input log p + c - 2 log c^{s-1},
+// sign of c (:754-756, :796)
+struct XEpiBwd1 {
+  SolveArgs a;
+  int t, nch;
+  long long off;  // scratch offset of the first image
+  __device__ void operator()(long long oi, double c, unsigned char sgb) const {
+    const long long N = a.N, img = oi / N, x = oi - img * N;
+    const int i = (int)(img % a.K);
+    const long long pc = img / a.K;
+    const int chain = nch == 1 ? 1 : (int)(pc % nch);
+    const int p = a.p0 + (int)(pc / nch);
+    const int s = bwd_step(a, p, t);
+    const long long pimg = ((long long)p * a.K + i) * N;
+    const double cs = ((const double*)a.Ch)[(long long)(s - 1) * a.stage + pimg + x];
+    if (chain == 0) {
+      float* gd = a.GD + pimg + x;
+      const float prev = s == steps_of(a, p) ? 0.f : *gd;
+      *gd = prev + (sgb ? 1.f : -1.f) * expf((float)(c - cs));
+    }
+    if (s > 1) {
+      a.xa[off + oi] = (double)a.logA[(long long)i * N + x] + c - 2.0 * cs;
+      a.sga[off + oi] = sgb;
+    }
+  }
+};
+
+// backward step s >= 2, after the second convolution: -v = sign exp(res - log
+// phi^{s-1}) (:757-762), -r = sign exp(res + log b^{s-1}) (:797-805)
+struct XEpiBwd2 {
+  SolveArgs a;
+  int t, nch;
+  __device__ void operator()(long long oi, double v, unsigned char sgb) const {
+    const long long N = a.N, img = oi / N, x = oi - img * N;
+    const int i = (int)(img % a.K);
+    const long long pc = img / a.K;
+    const int chain = nch == 1 ? 1 : (int)(pc % nch);
+    const int p = a.p0 + (int)(pc / nch);
+    const int s = bwd_step(a, p, t);
+    const long long pimg = ((long long)p * a.K + i) * N;
+    const float sg = sgb ? 1.f : -1.f;
+    if (chain == 0) {
+      const double f = ((const double*)a.Fh)[(long long)(s - 2) * a.stage + pimg + x];
+      ((s & 1) ? a.V1 : a.V0)[pimg + x] = sg * expf((float)(v - f));
+    } else {
+      const double b = ((const double*)a.Bh)[(long long)(s - 2) * a.stage + pimg + x];
+      ((s & 1) ? a.R1 : a.R0)[pimg + x] = sg * expf((float)(v + b));
+    }
+  }
+};
+
 int xlse_set_lds_attr(int maxlen) {
-  static int done = 0;
-  if (done >= maxlen) return WDL_OK;
   const int b = xlse_lds_bytes(maxlen);
-  WDL_HIP(hipFuncSetAttribute((const void*)k_xlse_pass<true>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, b));
-  WDL_HIP(hipFuncSetAttribute((const void*)k_xlse_pass<false>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, b));
-  done = maxlen;
+  int e;
+  if ((e = xlse_set_attr<true, XStore>(b)) || (e = xlse_set_attr<false, XStore>(b)) ||
+      (e = xlse_set_attr<false, XEpiFwd1>(b)) || (e = xlse_set_attr<true, XEpiBwd1>(b)) ||
+      (e = xlse_set_attr<true, XEpiBwd2>(b)))
+    return e;
   return WDL_OK;
 }
 
@@ -428,17 +528,6 @@ __global__ void k_xf1(SolveArgs a, int s) {
   }
 }
 
-// forward, input of the second convolution: log p - log c^{s-1} (:704-707)
-__global__ void k_xf2(SolveArgs a, int s) {
-  const long long tot = (long long)a.np * a.K * a.N;
-  XLOOP(tot) {
-    const XEl r = xel(a, 1, e);
-    if (s > steps_of(a, r.p)) continue;
-    const double c = ((const double*)a.Ch)[(long long)(s - 1) * a.stage + r.img + r.x];
-    a.xa[xoff(a, 1, r)] = (double)a.logA[(long long)r.i * a.N + r.x] - c;
-  }
-}
-
 // backward step, first convolution's input: chain 0 (lam_i n + v_i) b^s, chain 1
 // (lam_i g + r_i) / phi^s, as log magnitude and sign (:740-749, :778-793)
 __global__ void k_xb1(SolveArgs a, int t, int nch) {
@@ -470,53 +559,6 @@ __global__ void k_xb1(SolveArgs a, int t, int nch) {
     const long long o = xoff(a, nch, r);
     a.xa[o] = hlog(fabsf(xv)) + lh;
     a.sga[o] = xv >= 0.f ? 1 : 0;
-  }
-}
-
-// backward step, after the first convolution: resultDic (chain 0, :750-752) and
-// the second convolution's input log p + c - 2 log c^{s-1}, sign of c (:754-756, :796)
-__global__ void k_xb2(SolveArgs a, int t, int nch) {
-  const long long tot = (long long)a.np * nch * a.K * a.N;
-  XLOOP(tot) {
-    XEl r = xel(a, nch, e);
-    const int s = bwd_step(a, r.p, t);
-    if (s < 1) continue;
-    const int chain = nch == 1 ? 1 : r.cs;
-    const long long o = xoff(a, nch, r);
-    const double c = a.xb[o];
-    const float sg = a.sgb[o] ? 1.f : -1.f;
-    const double cs = ((const double*)a.Ch)[(long long)(s - 1) * a.stage + r.img + r.x];
-    if (chain == 0) {
-      float* gd = a.GD + r.img + r.x;
-      const float prev = s == steps_of(a, r.p) ? 0.f : *gd;
-      *gd = prev + sg * expf((float)(c - cs));
-    }
-    if (s > 1) {
-      a.xa[o] = (double)a.logA[(long long)r.i * a.N + r.x] + c - 2.0 * cs;
-      a.sga[o] = a.sgb[o];
-    }
-  }
-}
-
-// backward step s >= 2, after the second convolution: -v = sign exp(res - log
-// phi^{s-1}) (:757-762), -r = sign exp(res + log b^{s-1}) (:797-805)
-__global__ void k_xb3(SolveArgs a, int t, int nch) {
-  const long long tot = (long long)a.np * nch * a.K * a.N;
-  XLOOP(tot) {
-    XEl r = xel(a, nch, e);
-    const int s = bwd_step(a, r.p, t);
-    if (s < 2) continue;
-    const int chain = nch == 1 ? 1 : r.cs;
-    const long long o = xoff(a, nch, r);
-    const double v = a.xb[o];
-    const float sg = a.sgb[o] ? 1.f : -1.f;
-    if (chain == 0) {
-      const double f = ((const double*)a.Fh)[(long long)(s - 2) * a.stage + r.img + r.x];
-      ((s & 1) ? a.V1 : a.V0)[r.img + r.x] = sg * expf((float)(v - f));
-    } else {
-      const double b = ((const double*)a.Bh)[(long long)(s - 2) * a.stage + r.img + r.x];
-      ((s & 1) ? a.R1 : a.R0)[r.img + r.x] = sg * expf((float)(v + b));
-    }
   }
 }
 
@@ -598,16 +640,18 @@ static dim3 ew_grid(long long n) {
 }
 
 int exact_forward(const SolveArgs& a, hipStream_t st, int s) {
-  const long long nimg = (long long)a.np * a.K, n = nimg * a.N;
+  const long long nimg = (long long)a.np * a.K;
   const long long off = (long long)a.p0 * a.K * a.N;
   const int W = a.G.W, H = a.G.H;
   const double g = a.gamma;
   int e;
   hipLaunchKernelGGL(k_xf1, ew_grid((long long)a.np * a.N), dim3(256), 0, st, a, s);
-  if ((e = xlse_conv(a.xa + off, nullptr, (double*)a.Ch + (long long)(s - 1) * a.stage + off,
-                     nullptr, a.xt + off, nullptr, W, H, nimg, g, a.Lp, a.p0, a.K, s, st)))
+  // c^{s-1}, and log p - c^{s-1} as the second convolution's input
+  const XEpiFwd1 epi{(double*)a.Ch + (long long)(s - 1) * a.stage + off, a.xa + off, a.logA, a.K,
+                     a.N};
+  if ((e = xlse_conv_e(a.xa + off, nullptr, epi, a.xt + off, nullptr, W, H, nimg, g, a.Lp, a.p0,
+                       a.K, s, st)))
     return e;
-  hipLaunchKernelGGL(k_xf2, ew_grid(n), dim3(256), 0, st, a, s);
   if ((e = xlse_conv(a.xa + off, nullptr, (double*)a.Fh + (long long)(s - 1) * a.stage + off,
                      nullptr, a.xt + off, nullptr, W, H, nimg, g, a.Lp, a.p0, a.K, s, st)))
     return e;
@@ -631,16 +675,13 @@ int exact_backward(const SolveArgs& a, hipStream_t st, int t) {
   }
   hipLaunchKernelGGL(k_xdot_fin, ew_grid((long long)a.np * a.K), dim3(256), 0, st, a, t);
   // histogram p runs this step when L_p >= L - t + 1, the second convolution when >= L - t + 2
-  if ((e = xlse_conv(a.xa + off, a.sga + off, a.xb + off, a.sgb + off, a.xt + off, a.sgt + off, W, H,
-                     nimg, g, a.Lp, a.p0, nch * a.K, a.L - t + 1, st)))
+  if ((e = xlse_conv_e(a.xa + off, a.sga + off, XEpiBwd1{a, t, nch, off}, a.xt + off, a.sgt + off,
+                       W, H, nimg, g, a.Lp, a.p0, nch * a.K, a.L - t + 1, st)))
     return e;
-  hipLaunchKernelGGL(k_xb2, ew_grid(n), dim3(256), 0, st, a, t, nch);
-  if (t >= 2) {
-    if ((e = xlse_conv(a.xa + off, a.sga + off, a.xb + off, a.sgb + off, a.xt + off, a.sgt + off, W,
-                       H, nimg, g, a.Lp, a.p0, nch * a.K, a.L - t + 2, st)))
-      return e;
-    hipLaunchKernelGGL(k_xb3, ew_grid(n), dim3(256), 0, st, a, t, nch);
-  }
+  if (t >= 2 && (e = xlse_conv_e(a.xa + off, a.sga + off, XEpiBwd2{a, t, nch}, a.xt + off,
+                                 a.sgt + off, W, H, nimg, g, a.Lp, a.p0, nch * a.K, a.L - t + 2,
+                                 st)))
+    return e;
   WDL_HIP(hipGetLastError());
   return WDL_OK;
 }
