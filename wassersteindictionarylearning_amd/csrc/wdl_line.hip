@@ -10,6 +10,7 @@
 // up to 256 outputs.  1-D: outer = nvectors, inner = 1.  3-D: the 2-D
 // convolution of every z-slice (x then y, :768-789), then the line pass along z
 // (outer = nvectors, len = D, inner = W*H, :791-803).
+#include <atomic>
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -109,14 +110,14 @@ __global__ __launch_bounds__(kThreads) void k_line(const T* __restrict__ in, T* 
 }
 
 static int line_set_lds_attr() {
-  static bool done = false;
-  if (done) return WDL_OK;
+  static std::atomic<bool> done{false};  // kernel objects are created from several threads
+  if (done.load(std::memory_order_relaxed)) return WDL_OK;
   const int b = line_lds_bytes(make_line_geom(kLineMaxLen, 0));
   WDL_HIP(hipFuncSetAttribute((const void*)k_line<float>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, b));
   WDL_HIP(hipFuncSetAttribute((const void*)k_line<double>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, b));
-  done = true;
+  done.store(true, std::memory_order_relaxed);
   return WDL_OK;
 }
 

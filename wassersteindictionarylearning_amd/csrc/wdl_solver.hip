@@ -20,6 +20,7 @@
 // mean, n = sum_i v_i, g = sum_i r_i) are recomputed by each of the K
 // workgroups of a histogram from the previous stage's output; the K
 // workgroups of one histogram share an XCD so these re-reads hit its L2.
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -925,8 +926,8 @@ struct wdl_solver {
 };
 
 static int set_lds_attr() {
-  static bool done = false;
-  if (done) return WDL_OK;
+  static std::atomic<bool> done{false};  // solvers are created from several threads
+  if (done.load(std::memory_order_relaxed)) return WDL_OK;
   const int bytes = conv_lds_bytes(make_geom(kMaxSide, kMaxSide, 0));
   const void* fns[] = {(const void*)k_forward<1, false>, (const void*)k_forward<4, false>,
                        (const void*)k_backward<1, false>, (const void*)k_backward<4, false>};
@@ -942,7 +943,7 @@ static int set_lds_attr() {
   for (const void* f : lfns)
     WDL_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 bytes + kLogKeysBytes));
-  done = true;
+  done.store(true, std::memory_order_relaxed);
   return WDL_OK;
 }
 

@@ -18,6 +18,7 @@
 // accumulated per tile and summed over the tiles at the end (k_tgl); its
 // centring constant (the image mean of log phi, see exact_gsum) comes from
 // per-tile sums the forward writes into Mt.
+#include <atomic>
 #include <algorithm>
 #include <cmath>
 
@@ -730,8 +731,8 @@ int tiled_lds_bytes(const SolveArgs& a) {
 }
 
 int tiled_set_lds_attr() {
-  static bool done = false;
-  if (done) return WDL_OK;
+  static std::atomic<bool> done{false};  // solvers are created from several threads
+  if (done.load(std::memory_order_relaxed)) return WDL_OK;
   TileGeom big{};
   big.R = kTile + 2 * kHaloMax;
   big.S = kMaxSide + 4;
@@ -746,7 +747,7 @@ int tiled_set_lds_attr() {
                        (const void*)k_tbwd2_log<1>, (const void*)k_tbwd2_log<4>};
   for (const void* f : fns)
     WDL_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
-  done = true;
+  done.store(true, std::memory_order_relaxed);
   return WDL_OK;
 }
 

@@ -19,6 +19,7 @@
 // 678-814) as elementwise kernels before the passes and as output stages of
 // their second pass, with FP64 scratch images between them.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 
 #include "wdl_solve_common.h"
@@ -329,13 +330,17 @@ struct XStore {
   }
 };
 
+// Solvers may be created from several host threads (the server's included);
+// a repeated attribute call is harmless, so a relaxed atomic suffices.
 template <bool SIGNED, class Epi>
 static int xlse_set_attr(int lds) {
-  static int done = 0;
-  if (lds <= done) return WDL_OK;
+  static std::atomic<int> done{0};
+  if (lds <= done.load(std::memory_order_relaxed)) return WDL_OK;
   WDL_HIP(hipFuncSetAttribute((const void*)k_xlse_pass<SIGNED, Epi>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, lds));
-  done = lds;
+  int cur = done.load(std::memory_order_relaxed);
+  while (cur < lds && !done.compare_exchange_weak(cur, lds, std::memory_order_relaxed)) {
+  }
   return WDL_OK;
 }
 
