@@ -473,7 +473,7 @@ def main():
                     help="also run one 256 MiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
+        print("bench.py measures one GPU only: the library has no multi-GPU path "
               "(the P solves would shard by histogram with one K*N all-reduce per "
               "evaluation). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)

@@ -43,7 +43,8 @@ def test_version_and_footprint():
     ft = wdl.SinkhornSolver.footprint(W, H, K, P, L, wdl.FLAG_TILED)
     assert ft == f + 2 * PKN * 4 + P * K * 4 * 8 * (1 + L)
     ftl = wdl.SinkhornSolver.footprint(W, H, K, P, L, wdl.FLAG_TILED | wdl.FLAG_LOG_DOMAIN)
-    assert ftl == logf + 2 * PKN * 9 + P * K * 4 * 8 * (1 + L)
+    # log domain: also the backward's cross-atom sums (two P x N FP32 images)
+    assert ftl == logf + 2 * PKN * 9 + P * K * 4 * 8 * (1 + L) + 2 * P * W * H * 4
 
 
 def test_null_arguments_are_errors():

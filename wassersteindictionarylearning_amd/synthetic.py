@@ -54,7 +54,7 @@ def init_atoms_weights(rng, K, P, N):
 
 
 # Workloads (sizes of the benchmark configurations).
-CONFIGS = {
+WORKLOADS = {
     "plain_small": dict(n=64, P=16, K=4, gamma=2.0, L=25, loss=2, nblobs=(1, 3), sigma=(3, 8),
                         scale_factor=100.0),
     "batched_std": dict(n=128, P=64, K=5, gamma=1.0, L=100, loss=2, nblobs=(1, 4),
@@ -69,10 +69,10 @@ CONFIGS = {
 def make_problem(name: str, seed: int | None = None, P: int | None = None):
     """Inputs of one workload.  The projection workload's atoms stand for a
     learned dictionary (random blobs, as the configuration states)."""
-    c = dict(CONFIGS[name])
+    c = dict(WORKLOADS[name])
     if P is not None:
         c["P"] = P
-    rng = np.random.default_rng(20260000 + list(CONFIGS).index(name) if seed is None else seed)
+    rng = np.random.default_rng(20260000 + list(WORKLOADS).index(name) if seed is None else seed)
     n, Pn, K = c["n"], c["P"], c["K"]
     if c.get("curves"):
         obs = curve_histograms(rng, n, Pn)
