@@ -34,39 +34,76 @@ __global__ void k_ev_weights(const double* __restrict__ x, double* __restrict__ 
   }
 }
 
-__device__ inline double block_total(double v) {
-  __shared__ double tot;
-  v = block_sum(v);
-  if (threadIdx.x == 0) tot = v;
-  __syncthreads();
-  const double t = tot;
-  __syncthreads();
-  return t;
+// Atoms (:1013-1031): a = exp(x) normalised per atom; the solve's FP32 copy
+// (and its log for the log domain, :704) is normalised once more as above.
+// Each atom is split into kEvChunk-pixel chunks, one workgroup per (atom,
+// chunk), so the K atoms spread over many CUs; a per-atom total is the sum of
+// its chunks' partials in chunk order (deterministic), re-added by every
+// workgroup of the atom in the next launch.  part: 2 x K x chunks doubles.
+__device__ inline void ev_chunk(long long N, long long& base, long long& j0, long long& j1,
+                                int& i, int& c) {
+  const int nc = (int)ev_chunks(N);
+  i = blockIdx.x / nc;
+  c = blockIdx.x - i * nc;
+  base = (long long)i * N;
+  j0 = (long long)c * kEvChunk;
+  j1 = j0 + kEvChunk < N ? j0 + kEvChunk : N;
 }
 
-// Atoms (:1013-1031): a = exp(x) normalised per atom, one workgroup per atom;
-// the solve's FP32 copy (and its log for the log domain, :704) is normalised
-// once more as above.
-__global__ __launch_bounds__(kThreads) void k_ev_atoms(const double* __restrict__ xa,
-                                                       double* __restrict__ a64,
-                                                       float* __restrict__ A,
-                                                       float* __restrict__ logA, long long N) {
-  const long long base = (long long)blockIdx.x * N;
+__device__ inline double ev_total(const double* __restrict__ part, int i, long long N) {
+  __shared__ double tot;
+  const int nc = (int)ev_chunks(N);
+  if (threadIdx.x == 0) {
+    double t = 0;
+    for (int c = 0; c < nc; ++c) t += part[(long long)i * nc + c];
+    tot = t;
+  }
+  __syncthreads();
+  return tot;
+}
+
+__global__ __launch_bounds__(kThreads) void k_ev_atoms_exp(const double* __restrict__ xa,
+                                                           double* __restrict__ a64,
+                                                           double* __restrict__ part, long long N) {
+  long long base, j0, j1;
+  int i, c;
+  ev_chunk(N, base, j0, j1, i, c);
   double t = 0;
-  for (long long j = threadIdx.x; j < N; j += kThreads) {
+  for (long long j = j0 + threadIdx.x; j < j1; j += kThreads) {
     const double e = exp(xa[base + j]);
     a64[base + j] = e;
     t += e;
   }
-  const double tot = block_total(t);
+  t = block_sum(t);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(kThreads) void k_ev_atoms_norm(double* __restrict__ a64,
+                                                            double* __restrict__ part,
+                                                            long long N) {
+  long long base, j0, j1;
+  int i, c;
+  ev_chunk(N, base, j0, j1, i, c);
+  const double tot = ev_total(part, i, N);
   double t2 = 0;
-  for (long long j = threadIdx.x; j < N; j += kThreads) {
+  for (long long j = j0 + threadIdx.x; j < j1; j += kThreads) {
     const double v = a64[base + j] / tot;
     a64[base + j] = v;
     t2 += v;
   }
-  const double tot2 = block_total(t2);
-  for (long long j = threadIdx.x; j < N; j += kThreads) {
+  t2 = block_sum(t2);
+  if (threadIdx.x == 0) part[gridDim.x + blockIdx.x] = t2;
+}
+
+__global__ __launch_bounds__(kThreads) void k_ev_atoms_f32(const double* __restrict__ a64,
+                                                           const double* __restrict__ part,
+                                                           float* __restrict__ A,
+                                                           float* __restrict__ logA, long long N) {
+  long long base, j0, j1;
+  int i, c;
+  ev_chunk(N, base, j0, j1, i, c);
+  const double tot2 = ev_total(part + gridDim.x, i, N);
+  for (long long j = j0 + threadIdx.x; j < j1; j += kThreads) {
     const double v = a64[base + j] / tot2;
     A[base + j] = (float)v;
     if (logA) logA[base + j] = (float)log(v);
@@ -98,37 +135,57 @@ __global__ __launch_bounds__(kThreads) void k_ev_weights_grad(
 
 // Atom gradients (:1103-1112): the projection G - <a_i, G> is linear in the
 // per-histogram gradients, so it is applied to their sum; times a for the
-// softmax (:1125-1136).  One workgroup per atom; `zero`: weights-only solve,
-// whose atom gradients are zero.
+// softmax (:1125-1136).  Chunked as the atoms above: k_ev_atoms_dot writes
+// the chunks' partial dots, k_ev_atoms_grad adds them in order.  `zero`:
+// weights-only solve, whose atom gradients are zero.
+__global__ __launch_bounds__(kThreads) void k_ev_atoms_dot(const double* __restrict__ GDsum,
+                                                           const double* __restrict__ a64,
+                                                           double* __restrict__ part, long long N) {
+  long long base, j0, j1;
+  int i, c;
+  ev_chunk(N, base, j0, j1, i, c);
+  double d = 0;
+  for (long long j = j0 + threadIdx.x; j < j1; j += kThreads) d += a64[base + j] * GDsum[base + j];
+  d = block_sum(d);
+  if (threadIdx.x == 0) part[blockIdx.x] = d;
+}
+
 __global__ __launch_bounds__(kThreads) void k_ev_atoms_grad(const double* __restrict__ GDsum,
                                                             const double* __restrict__ a64,
+                                                            const double* __restrict__ part,
                                                             double* __restrict__ g, long long N,
                                                             int zero) {
-  const long long base = (long long)blockIdx.x * N;
-  double d = 0;
-  if (!zero)
-    for (long long j = threadIdx.x; j < N; j += kThreads) d += a64[base + j] * GDsum[base + j];
-  d = block_total(d);
-  for (long long j = threadIdx.x; j < N; j += kThreads)
+  long long base, j0, j1;
+  int i, c;
+  ev_chunk(N, base, j0, j1, i, c);
+  const double d = zero ? 0.0 : ev_total(part, i, N);
+  for (long long j = j0 + threadIdx.x; j < j1; j += kThreads)
     g[base + j] = zero ? 0.0 : (GDsum[base + j] - d) * a64[base + j];
 }
 
 int evaluate_prologue(const double* x, double* w64, double* a64, float* lam, float* A,
-                      float* logA, int P, int K, long long N, double scale, int expw,
-                      hipStream_t st) {
+                      float* logA, double* part, int P, int K, long long N, double scale,
+                      int expw, hipStream_t st) {
+  const dim3 ga((unsigned)(K * ev_chunks(N)));
   hipLaunchKernelGGL(k_ev_weights, dim3((P + 255) / 256), dim3(256), 0, st, x, w64, lam, P, K,
                      scale, expw);
-  hipLaunchKernelGGL(k_ev_atoms, dim3(K), dim3(kThreads), 0, st, x + (long long)P * K, a64, A,
-                     logA, N);
+  const double* xa = x + (long long)P * K;
+  hipLaunchKernelGGL(k_ev_atoms_exp, ga, dim3(kThreads), 0, st, xa, a64, part, N);
+  hipLaunchKernelGGL(k_ev_atoms_norm, ga, dim3(kThreads), 0, st, a64, part, N);
+  hipLaunchKernelGGL(k_ev_atoms_f32, ga, dim3(kThreads), 0, st, a64, part, A, logA, N);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int evaluate_epilogue(const double* GL, const double* GDsum, const double* lossPart,
-                      const double* w64, const double* a64, double* g, double* f, int P, int K,
-                      long long N, double scale, int expw, int weightsOnly, hipStream_t st) {
+                      const double* w64, const double* a64, double* part, double* g, double* f,
+                      int P, int K, long long N, double scale, int expw, int weightsOnly,
+                      hipStream_t st) {
+  const dim3 ga((unsigned)(K * ev_chunks(N)));
   hipLaunchKernelGGL(k_ev_weights_grad, dim3(1), dim3(kThreads), 0, st, GL, w64, lossPart, g, f,
                      P, K, kLossBlocks, scale, expw);
-  hipLaunchKernelGGL(k_ev_atoms_grad, dim3(K), dim3(kThreads), 0, st, GDsum, a64,
+  if (!weightsOnly)
+    hipLaunchKernelGGL(k_ev_atoms_dot, ga, dim3(kThreads), 0, st, GDsum, a64, part, N);
+  hipLaunchKernelGGL(k_ev_atoms_grad, ga, dim3(kThreads), 0, st, GDsum, a64, part,
                      g + (long long)P * K, N, weightsOnly);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }

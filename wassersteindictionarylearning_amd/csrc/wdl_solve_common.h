@@ -186,12 +186,18 @@ __host__ __device__ inline long long xdot_chunks(long long N) {
 // evaluate_both around the batched solve, on the device (wdl_evaluate.hip):
 // x -> normalised weights / atoms (FP64 w64, a64 and the solve's FP32 copies),
 // and the loss sum, projections and chain rule into g and *f (device).
+// part: the per-chunk atom partial sums, 2 x K x ev_chunks(N) doubles.
+constexpr int kEvChunk = 2048;
+__host__ __device__ inline long long ev_chunks(long long N) {
+  return (N + kEvChunk - 1) / kEvChunk;
+}
 int evaluate_prologue(const double* x, double* w64, double* a64, float* lam, float* A,
-                      float* logA, int P, int K, long long N, double scale, int expw,
-                      hipStream_t st);
+                      float* logA, double* part, int P, int K, long long N, double scale,
+                      int expw, hipStream_t st);
 int evaluate_epilogue(const double* GL, const double* GDsum, const double* lossPart,
-                      const double* w64, const double* a64, double* g, double* f, int P, int K,
-                      long long N, double scale, int expw, int weightsOnly, hipStream_t st);
+                      const double* w64, const double* a64, double* part, double* g, double* f,
+                      int P, int K, long long N, double scale, int expw, int weightsOnly,
+                      hipStream_t st);
 
 // Sum over the workgroup (NW waves), result valid in thread 0.
 template <int NW = kWaves>

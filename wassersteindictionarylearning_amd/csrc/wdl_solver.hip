@@ -737,7 +737,7 @@ struct wdl_solver {
   int D = 0;
   DevBuf tab, A, lam, obs, Bh, Qh, Ch, Fh, V0, V1, R0, R1, GD, GL, Q, G0, lossPart, bStore,
       bTemp, GDsum, logA, events, Lp, X0, X1, XS0, XS1, GLt, gsumPart, Mt, eta, xa, xb, xt, xdp,
-      nj, gj, sga, sgb, sgt, wla, wlb, wlc, wlt, evX, evG, evW, evA, evF;
+      nj, gj, sga, sgb, sgt, wla, wlb, wlc, wlt, evX, evG, evW, evA, evF, evP;
   bool tiled = false;  // 64 x 64 tiles with a halo (a side above kMaxSide)
   bool exact = false;  // log domain with the exact-shift line passes (WDL_FLAG_EXACT_LSE)
   TileGeom tg{};
@@ -969,6 +969,7 @@ static long long footprint(int W, int H, int K, int P, int L, int flags) {
   f += kTabLen * 4;
   f += 4LL * P * N * 8;  // the Wasserstein loss's log scalings and convolution buffers
   f += (3LL * P * K + 3LL * K * N + 1) * 8;  // evaluate_both: x, g, weights, atoms (FP64), f
+  f += 2LL * K * ev_chunks(N) * 8;            // and the atoms' per-chunk sums
   if (use_exact(flags)) {  // FP64 scratch images + sign bytes, the dots' partial sums
     f += 3LL * 2 * PKN * (8 + 1);
     f += (long long)P * K * xdot_chunks((long long)W * H) * 3 * 8;
@@ -1097,6 +1098,7 @@ int wdl_solver_create(wdl_ctx* ctx, int W, int H, int K, int P, int L, double ga
       {&s->evW, (size_t)P * K * sizeof(double)},
       {&s->evA, (size_t)K * N * sizeof(double)},
       {&s->evF, sizeof(double)},
+      {&s->evP, (size_t)2 * K * ev_chunks(N) * sizeof(double)},
   };
   for (auto& al : allocs) {
     if ((st = al.b->ensure(al.n))) {
@@ -1150,7 +1152,7 @@ int wdl_solver_destroy(wdl_solver* s) {
                     &s->G0,  &s->lossPart,    &s->bStore,      &s->bTemp,    &s->GDsum,
                     &s->logA, &s->events, &s->Lp, &s->X0, &s->X1, &s->XS0, &s->XS1, &s->GLt, &s->Mt, &s->gsumPart, &s->eta, &s->xa, &s->xb, &s->xt, &s->xdp, &s->nj, &s->gj, &s->sga, &s->sgb,
                     &s->sgt, &s->wla, &s->wlb, &s->wlc,
-                    &s->wlt, &s->evX, &s->evG, &s->evW, &s->evA, &s->evF};
+                    &s->wlt, &s->evX, &s->evG, &s->evW, &s->evA, &s->evF, &s->evP};
   for (DevBuf* b : bufs) b->release();
   delete s;
   return WDL_OK;
@@ -1553,15 +1555,15 @@ static int evaluate_device(wdl_solver* s, const double* dx, double* dg, double* 
   WDL_HIP(hipSetDevice(s->ctx->device));
   const bool logd = s->flags & WDL_FLAG_LOG_DOMAIN;
   WDL_CHECK(evaluate_prologue(dx, s->evW.as<double>(), s->evA.as<double>(), s->lam.as<float>(),
-                              s->A.as<float>(), logd ? s->logA.as<float>() : nullptr, P, K, N,
-                              s->scaleDic, s->expWeight, st) == 0,
+                              s->A.as<float>(), logd ? s->logA.as<float>() : nullptr,
+                              s->evP.as<double>(), P, K, N, s->scaleDic, s->expWeight, st) == 0,
             WDL_ERR_HIP, "evaluate_both: prologue launch failed");
   s->problem = true;
   int rc = wdl_solver_run(s, WDL_RUN_BATCHED);
   if (rc) return rc;
   WDL_CHECK(evaluate_epilogue(s->GL.as<double>(), s->GDsum.as<double>(), s->lossPart.as<double>(),
-                              s->evW.as<double>(), s->evA.as<double>(), dg, s->evF.as<double>(), P,
-                              K, N, s->scaleDic, s->expWeight,
+                              s->evW.as<double>(), s->evA.as<double>(), s->evP.as<double>(), dg,
+                              s->evF.as<double>(), P, K, N, s->scaleDic, s->expWeight,
                               (s->flags & WDL_FLAG_WEIGHTS_ONLY) ? 1 : 0, st) == 0,
             WDL_ERR_HIP, "evaluate_both: epilogue launch failed");
   WDL_HIP(hipMemcpyAsync(f, s->evF.p, sizeof(double), hipMemcpyDeviceToHost, st));
