@@ -253,6 +253,31 @@ double wdl_evaluate_both(void* instance, const double* x, double* g, int n, doub
  * state on the device. */
 int wdl_evaluate_both_device(wdl_solver* s, const double* d_x, double* d_g, double* f);
 
+/* ---- histogram files (host only) ---------------------------------------------
+ * ExportHistogramToCSV::exportHistogram (histogramIO.h:62-78): <folder>/<name>.csv,
+ * one value per line in the stream's default 6-significant-digit form.
+ * ExportHistogramToRAW::exportHistogram (histogramIO.h:208-220): <folder>/<name>.raw,
+ * the n doubles in host byte order.  0 on success, -1 when the file cannot be
+ * opened (message in wdl_last_error), WDL_ERR_ARG on null arguments. */
+int wdl_export_histogram_csv(const char* folder, const char* name, const double* values,
+                             long long n);
+int wdl_export_histogram_raw(const char* folder, const char* name, const double* values,
+                             long long n);
+/* exportHistogramsBatch (histogramIO.h:80-97, 222-238): count histograms of
+ * lengths[i] values; every file is attempted, -1 if any failed. */
+int wdl_export_histograms_csv(const char* folder, const char* const* names,
+                              const double* const* values, const long long* lengths, int count);
+int wdl_export_histograms_raw(const char* folder, const char* const* names,
+                              const double* const* values, const long long* lengths, int count);
+/* load_csv_to_pdf (histogramIO.h:314-337): reads the file's values, divides them
+ * by their sum and returns the sum, or -1 when the file cannot be opened.
+ * *n_out receives the number of values read (the reference's loop counts a
+ * trailing-whitespace read once more, with the last value); the first
+ * min(*n_out, capacity) normalised values go to out (out may be NULL when
+ * capacity is 0, to size a second call). */
+double wdl_load_csv_to_pdf(const char* filename, double* out, long long capacity,
+                           long long* n_out);
+
 #ifdef __cplusplus
 }
 #endif

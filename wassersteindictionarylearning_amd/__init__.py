@@ -79,6 +79,14 @@ _SIGS = {
     "wdl_solver_set_scale": (_i, [_vp, ctypes.c_double, _i]),
     "wdl_evaluate_both": (ctypes.c_double, [_vp, _d, _d, _i, ctypes.c_double]),
     "wdl_evaluate_both_device": (_i, [_vp, _vp, _vp, ctypes.POINTER(ctypes.c_double)]),
+    "wdl_export_histogram_csv": (_i, [ctypes.c_char_p, ctypes.c_char_p, _d, ctypes.c_longlong]),
+    "wdl_export_histogram_raw": (_i, [ctypes.c_char_p, ctypes.c_char_p, _d, ctypes.c_longlong]),
+    "wdl_export_histograms_csv": (_i, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_char_p),
+                                      ctypes.POINTER(_d), ctypes.POINTER(ctypes.c_longlong), _i]),
+    "wdl_export_histograms_raw": (_i, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_char_p),
+                                      ctypes.POINTER(_d), ctypes.POINTER(ctypes.c_longlong), _i]),
+    "wdl_load_csv_to_pdf": (ctypes.c_double, [ctypes.c_char_p, _d, ctypes.c_longlong,
+                                             ctypes.POINTER(ctypes.c_longlong)]),
     "wdl_solver_set_slot": (_i, [_vp, _i, _d, _d, _i, ctypes.c_double]),
     "wdl_solver_projection_round": (_i, [_vp]),
     "wdl_solver_get_slot": (_i, [_vp, _i, _d, _d]),
@@ -493,3 +501,41 @@ class ProjectionServer:
                 self.close()
             except Exception:
                 pass
+
+
+def export_histograms(folder, names, histograms, fmt="csv"):
+    """ExportHistogramToCSV / ExportHistogramToRAW::exportHistogramsBatch
+    (histogramIO.h:80-97, 222-238): one <folder>/<name>.csv|.raw per histogram.
+    Returns 0, or -1 when any file could not be written (as the reference)."""
+    if fmt not in ("csv", "raw"):
+        raise ValueError("fmt must be 'csv' or 'raw'")
+    if len(names) != len(histograms):
+        raise ValueError("Histogram vector and filename vector are of different sizes")
+    hs = [_f64(h).ravel() for h in histograms]
+    n = len(hs)
+    cn = (ctypes.c_char_p * max(n, 1))(*[str(x).encode() for x in names])
+    cv = (_d * max(n, 1))(*[_dp(h) for h in hs])
+    cl = (ctypes.c_longlong * max(n, 1))(*[h.size for h in hs])
+    f = lib().wdl_export_histograms_csv if fmt == "csv" else lib().wdl_export_histograms_raw
+    st = f(str(folder).encode(), cn, cv, cl, n)
+    if st not in (0, -1):
+        _check(st, "wdl_export_histograms_" + fmt)
+    return st
+
+
+def export_histogram(folder, name, histogram, fmt="csv"):
+    """ExportHistogramTo{CSV,RAW}::exportHistogram (histogramIO.h:62-78, 208-220)."""
+    return export_histograms(folder, [name], [histogram], fmt)
+
+
+def load_csv_to_pdf(filename):
+    """load_csv_to_pdf (histogramIO.h:314-337): (sum, values / sum); sum is -1
+    and values empty when the file cannot be opened."""
+    n = ctypes.c_longlong(0)
+    fn = str(filename).encode()
+    s = lib().wdl_load_csv_to_pdf(fn, None, 0, ctypes.byref(n))
+    if s == -1.0 and n.value == 0:
+        return s, np.zeros(0)
+    out = np.empty(n.value)
+    s = lib().wdl_load_csv_to_pdf(fn, _dp(out), out.size, ctypes.byref(n))
+    return s, out
