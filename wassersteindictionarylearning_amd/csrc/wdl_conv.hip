@@ -1,6 +1,7 @@
 // Standard-domain separable Gaussian convolution: the drop-in for
 // Gaussian2DKernel::convolve (kernels.h:669-699) and its FP32 device form.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 
@@ -98,9 +99,9 @@ __global__ __launch_bounds__(kThreads, 4) void k_conv_tile(const T* __restrict__
 
 template <class T>
 static int launch_conv_tiled(const wdl_gauss2d* k, const T* in, T* out, int nvec, hipStream_t s) {
-  static bool attr_set = false;
+  static std::atomic<bool> attr_set{false};  // kernel objects are used from several threads
   const int lds = tile_lds_bytes(k->tg);
-  if (!attr_set) {
+  if (!attr_set.load(std::memory_order_relaxed)) {
     TileGeom big = k->tg;
     big.R = kTile + 2 * kHaloMax;
     big.S = kMaxSide + 4;
@@ -109,7 +110,7 @@ static int launch_conv_tiled(const wdl_gauss2d* k, const T* in, T* out, int nvec
                                 hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
     WDL_HIP(hipFuncSetAttribute((const void*)k_conv_tile<float, 4>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
-    attr_set = true;
+    attr_set.store(true, std::memory_order_relaxed);
   }
   const long long work = (long long)nvec * k->tg.nx * k->tg.ny;
   const int grid = (int)std::min<long long>(work, 1 << 20);
@@ -129,8 +130,8 @@ static int launch_conv(const wdl_gauss2d* k, const T* in, T* out, int nvec, hipS
   if (k->tiled) return launch_conv_tiled<T>(k, in, out, nvec, s);
   const bool bf3 = k->flags & WDL_FLAG_BF16X3;
   const ConvGeom G = k->geom(!(k->flags & WDL_FLAG_DENSE));
-  static bool attr_set = false;
-  if (!attr_set) {
+  static std::atomic<bool> attr_set{false};  // kernel objects are used from several threads
+  if (!attr_set.load(std::memory_order_relaxed)) {
     const int img = conv_lds_bytes(make_geom(kMaxSide, kMaxSide, 0, 32)) - kTabFloats * 4;
     const void* f32[] = {(const void*)k_conv<T, false>, (const void*)k_conv4<false>};
     const void* fbf[] = {(const void*)k_conv<T, true>, (const void*)k_conv4<true>};
@@ -140,7 +141,7 @@ static int launch_conv(const wdl_gauss2d* k, const T* in, T* out, int nvec, hipS
     for (const void* f : fbf)
       WDL_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   img + ConvPrec<true>::kTabBytes));
-    attr_set = true;
+    attr_set.store(true, std::memory_order_relaxed);
   }
   const int lds = conv_lds_bytes(G) - kTabFloats * 4 +
                   (bf3 ? ConvPrec<true>::kTabBytes : ConvPrec<false>::kTabBytes);

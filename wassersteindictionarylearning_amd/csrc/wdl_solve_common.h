@@ -61,14 +61,12 @@ struct SolveArgs {
   // exact-shift log domain: gamma, FP64 scratch images and sign bytes
   double gamma;
   double* xa;
-  double* xb;
   double* xt;
   double* xdp;  // per-chunk partial sums of the weight-gradient dots
   // tiled log domain: the backward's cross-atom sums -sum_j v_j, -sum_j r_j (P x N)
   float* nj;
   float* gj;
   unsigned char* sga;
-  unsigned char* sgb;
   unsigned char* sgt;
 };
 

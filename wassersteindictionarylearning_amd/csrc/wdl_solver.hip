@@ -736,8 +736,8 @@ struct wdl_solver {
   long long N = 0;
   int D = 0;
   DevBuf tab, A, lam, obs, Bh, Qh, Ch, Fh, V0, V1, R0, R1, GD, GL, Q, G0, lossPart, bStore,
-      bTemp, GDsum, logA, events, Lp, X0, X1, XS0, XS1, GLt, gsumPart, Mt, eta, xa, xb, xt, xdp,
-      nj, gj, sga, sgb, sgt, wla, wlb, wlc, wlt, evX, evG, evW, evA, evF, evP;
+      bTemp, GDsum, logA, events, Lp, X0, X1, XS0, XS1, GLt, gsumPart, Mt, eta, xa, xt, xdp,
+      nj, gj, sga, sgt, wla, wlb, wlc, wlt, evX, evG, evW, evA, evF, evP;
   bool tiled = false;  // 64 x 64 tiles with a halo (a side above kMaxSide)
   bool exact = false;  // log domain with the exact-shift line passes (WDL_FLAG_EXACT_LSE)
   TileGeom tg{};
@@ -801,13 +801,11 @@ struct wdl_solver {
     a.wlc = wlc.as<double>();
     a.wlt = wlt.as<double>();
     a.xa = xa.as<double>();
-    a.xb = xb.as<double>();
     a.xt = xt.as<double>();
     a.xdp = xdp.as<double>();
     a.nj = nj.as<float>();
     a.gj = gj.as<float>();
     a.sga = sga.as<unsigned char>();
-    a.sgb = sgb.as<unsigned char>();
     a.sgt = sgt.as<unsigned char>();
     return a;
   }
@@ -970,8 +968,8 @@ static long long footprint(int W, int H, int K, int P, int L, int flags) {
   f += 4LL * P * N * 8;  // the Wasserstein loss's log scalings and convolution buffers
   f += (3LL * P * K + 3LL * K * N + 1) * 8;  // evaluate_both: x, g, weights, atoms (FP64), f
   f += 2LL * K * ev_chunks(N) * 8;            // and the atoms' per-chunk sums
-  if (use_exact(flags)) {  // FP64 scratch images + sign bytes, the dots' partial sums
-    f += 3LL * 2 * PKN * (8 + 1);
+  if (use_exact(flags)) {  // FP64 scratch images xa, xt + sign bytes, the dots' partial sums
+    f += 2LL * 2 * PKN * (8 + 1);
     f += (long long)P * K * xdot_chunks((long long)W * H) * 3 * 8;
   }
   if (use_tiles(W, H, flags)) {  // X0, X1 (+ sign bytes), per-tile dots, log: cross-atom sums
@@ -1083,11 +1081,9 @@ int wdl_solver_create(wdl_ctx* ctx, int W, int H, int K, int P, int L, double ga
       {&s->gsumPart, (size_t)P * kLossBlocks * sizeof(double)},
       {&s->eta, (size_t)P * sizeof(float)},
       {&s->xa, use_exact(flags) ? (size_t)2 * PKN * sizeof(double) : 0},
-      {&s->xb, use_exact(flags) ? (size_t)2 * PKN * sizeof(double) : 0},
       {&s->xt, use_exact(flags) ? (size_t)2 * PKN * sizeof(double) : 0},
       {&s->xdp, use_exact(flags) ? (size_t)P * K * xdot_chunks(N) * 3 * sizeof(double) : 0},
       {&s->sga, use_exact(flags) ? (size_t)2 * PKN : 0},
-      {&s->sgb, use_exact(flags) ? (size_t)2 * PKN : 0},
       {&s->sgt, use_exact(flags) ? (size_t)2 * PKN : 0},
       {&s->wla, (size_t)P * N * sizeof(double)},
       {&s->wlb, (size_t)P * N * sizeof(double)},
@@ -1150,7 +1146,7 @@ int wdl_solver_destroy(wdl_solver* s) {
   DevBuf* bufs[] = {&s->tab, &s->A,  &s->lam, &s->obs, &s->Bh, &s->Qh, &s->Ch,  &s->Fh,
                     &s->V0,  &s->V1, &s->R0,  &s->R1,  &s->GD, &s->GL,       &s->Q,
                     &s->G0,  &s->lossPart,    &s->bStore,      &s->bTemp,    &s->GDsum,
-                    &s->logA, &s->events, &s->Lp, &s->X0, &s->X1, &s->XS0, &s->XS1, &s->GLt, &s->Mt, &s->gsumPart, &s->eta, &s->xa, &s->xb, &s->xt, &s->xdp, &s->nj, &s->gj, &s->sga, &s->sgb,
+                    &s->logA, &s->events, &s->Lp, &s->X0, &s->X1, &s->XS0, &s->XS1, &s->GLt, &s->Mt, &s->gsumPart, &s->eta, &s->xa, &s->xt, &s->xdp, &s->nj, &s->gj, &s->sga,
                     &s->sgt, &s->wla, &s->wlb, &s->wlc,
                     &s->wlt, &s->evX, &s->evG, &s->evW, &s->evA, &s->evF, &s->evP};
   for (DevBuf* b : bufs) b->release();
