@@ -86,6 +86,20 @@ int wdl_gauss2d_log_convolve_signarray(wdl_gauss2d* k, const double* log_u,
                                        unsigned char* signs, double* out, int nvectors);
 /* FP32 device form of convolve (no equivalent in the reference; the batched path's operator). */
 int wdl_gauss2d_convolve_device(wdl_gauss2d* k, const float* d_u, float* d_out, int nvectors);
+/* Allocation-free form of wdl_gauss2d_log_convolve (d_signs == NULL) and
+ * wdl_gauss2d_log_convolve_signarray (d_signs = the packed sign array, LSB
+ * first, read and overwritten as there), on FP64 device memory, asynchronous
+ * on the context stream.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  It calls no allocator: the
+ * transposed intermediate and the sign bytes live in one static device array
+ * per GPU (WDL_NOALLOC_MAX_VALUES * 11 bytes, 22 MiB, held while the library is
+ * loaded), so nvectors*W*H may not exceed WDL_NOALLOC_MAX_VALUES (WDL_ERR_ARG
+ * otherwise).  Every call on one GPU shares that array: calls that overlap on
+ * one GPU (several streams or host threads at once) race; serialise them. */
+#define WDL_NOALLOC_MAX_VALUES 2097152
+int wdl_gauss2d_log_convolve_noalloc(wdl_gauss2d* k, const double* d_log_u, unsigned char* d_signs,
+                                     double* d_out, int nvectors);
 
 /* ---- 1-D and 3-D KernelTypes --------------------------------------------------
  * Replace Gaussian1DKernel (kernels.h:501-556) and Gaussian3DKernel

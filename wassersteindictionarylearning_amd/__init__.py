@@ -24,6 +24,7 @@ FLAG_LOG_DOMAIN, FLAG_WARM_RESTART, FLAG_DENSE, FLAG_NO_GRAPH, FLAG_WEIGHTS_ONLY
 FLAG_TILED = 32
 FLAG_EXACT_LSE = 64
 FLAG_BF16X3 = 128
+NOALLOC_MAX_VALUES = 2097152  # WDL_NOALLOC_MAX_VALUES
 RUN_BATCHED, RUN_PLAIN = 0, 1
 STAGE_FORWARD, STAGE_BACKWARD = 0, 1
 
@@ -48,6 +49,7 @@ _SIGS = {
     "wdl_gauss2d_log_convolve": (_i, [_vp, _d, _d, _i]),
     "wdl_gauss2d_log_convolve_signarray": (_i, [_vp, _d, _u8, _d, _i]),
     "wdl_gauss2d_convolve_device": (_i, [_vp, _vp, _vp, _i]),
+    "wdl_gauss2d_log_convolve_noalloc": (_i, [_vp, _vp, _vp, _vp, _i]),
     "wdl_gauss1d_create": (_i, [_vp, _i, ctypes.c_double, ctypes.POINTER(_vp)]),
     "wdl_gauss1d_destroy": (_i, [_vp]),
     "wdl_gauss1d_convolve": (_i, [_vp, _d, _d, _i]),

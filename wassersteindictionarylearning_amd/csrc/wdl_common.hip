@@ -112,6 +112,12 @@ int wdl_ctx_create(int device, wdl_ctx** out) {
   }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
+  int st = wdl::noalloc_scratch_init(device);
+  if (st) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return st;
+  }
   *out = c;
   return WDL_OK;
 }

@@ -64,6 +64,10 @@ int xlse_convolve(const double* in, const unsigned char* sin, double* out, unsig
                   double* tmp, unsigned char* stmp, int W, int H, long long nimg, double gamma,
                   hipStream_t st);
 
+// Looks up, once per device, the device address of the allocation-free entry
+// point's static scratch (wdl_log.hip); the device must be current.
+int noalloc_scratch_init(int device);
+
 // Images with a side above kMaxSide take the tiled kernels.
 inline bool needs_tiles(int W, int H) { return W > kMaxSide || H > kMaxSide; }
 

@@ -12,7 +12,10 @@
 // that and writes back in image order.  Output signs go to a byte per value
 // and are packed LSB-first into the caller's sign array (signArray.h:8-17)
 // afterwards, so no two threads write one byte.
+#include <atomic>
 #include <cmath>
+#include <mutex>
+#include <string>
 
 #include "wdl_internal.h"
 
@@ -92,27 +95,24 @@ __global__ void k_unpack_signs(const unsigned char* __restrict__ bits,
 
 static int lse_launch_bytes(int len) { return len * (int)(sizeof(double) + sizeof(float)); }
 
-// Two passes over nvec images; signs == nullptr for the unsigned form.
-static int log_convolve_dev(wdl_gauss2d* k, const double* d_in, double* d_out, unsigned char* d_bits,
-                            int nvec, hipStream_t st) {
+// Two passes over nvec images; d_bits == nullptr for the unsigned form.
+// Scratch: tmp holds the transposed FP64 intermediate (nvec*W*H values), sg
+// the sign bytes (3*nvec*W*H: input, intermediate and output signs).
+static int log_convolve_core(const wdl_gauss2d* k, const double* d_in, double* d_out,
+                             unsigned char* d_bits, int nvec, double* tmp, unsigned char* sg,
+                             hipStream_t st) {
   const long long n = (long long)k->W * k->H * nvec;
-  int rc;
-  if ((rc = k->io_tmp.ensure(n * sizeof(double)))) return rc;
   const bool sgn = d_bits != nullptr;
-  if (sgn && (rc = k->io_sign.ensure(2 * n))) return rc;
-  unsigned char* b1 = k->io_sign.as<unsigned char>();
-  unsigned char* b2 = b1 + n;
+  unsigned char* b1 = sg;
+  unsigned char* b2 = sg ? sg + n : nullptr;
   const int W = k->W, H = k->H;
   const int g1 = (int)std::min<long long>((long long)nvec * H, 65535);
   const int g2 = (int)std::min<long long>((long long)nvec * W, 65535);
-  double* tmp = k->io_tmp.as<double>();
+  int rc;
   if (k->flags & WDL_FLAG_EXACT_LSE) {  // exact-shift passes, FP32 sums (wdl_xlse.hip)
     WDL_CHECK(std::max(W, H) <= 1024, WDL_ERR_UNSUPPORTED,
               "log_convolve: WDL_FLAG_EXACT_LSE takes sides up to 1024");
-    if (sgn && (rc = k->io_sign.ensure(3 * n))) return rc;
-    b1 = k->io_sign.as<unsigned char>();
-    b2 = b1 + n;
-    unsigned char* b0 = b1 + 2 * n;
+    unsigned char* b0 = sg ? sg + 2 * n : nullptr;
     const int gp = (int)std::min<long long>((n + 255) / 256, 8192);
     if (sgn) hipLaunchKernelGGL(k_unpack_signs, dim3(gp), dim3(256), 0, st, d_bits, b0, n);
     if ((rc = xlse_convolve(d_in, sgn ? b0 : nullptr, d_out, sgn ? b2 : nullptr, tmp,
@@ -139,6 +139,45 @@ static int log_convolve_dev(wdl_gauss2d* k, const double* d_in, double* d_out, u
                        lse_launch_bytes(H), st, tmp, nullptr, d_out, nullptr, W, H, nvec, k->gamma);
   }
   WDL_HIP(hipGetLastError());
+  return WDL_OK;
+}
+
+// The kernel object's own scratch, grown on demand (hipMalloc).
+static int log_convolve_dev(wdl_gauss2d* k, const double* d_in, double* d_out, unsigned char* d_bits,
+                            int nvec, hipStream_t st) {
+  const long long n = (long long)k->W * k->H * nvec;
+  int rc;
+  if ((rc = k->io_tmp.ensure(n * sizeof(double)))) return rc;
+  if (d_bits && (rc = k->io_sign.ensure(3 * n))) return rc;
+  return log_convolve_core(k, d_in, d_out, d_bits, nvec, k->io_tmp.as<double>(),
+                           d_bits ? k->io_sign.as<unsigned char>() : nullptr, st);
+}
+
+// ---- allocation-free form ------------------------------------------------------
+// Scratch of wdl_gauss2d_log_convolve_noalloc: the FP64 intermediate of up to
+// WDL_NOALLOC_MAX_VALUES values, then 3 sign bytes per value.  One copy per
+// GPU, held while the library is loaded; every call on a GPU shares it.
+constexpr long long kNoAllocValues = WDL_NOALLOC_MAX_VALUES;
+constexpr size_t kNoAllocBytes = (size_t)kNoAllocValues * (sizeof(double) + 3);
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+static std::atomic<unsigned char*> g_scratch[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+// The device address of `scratch` on `device`, which must be current.  Looked
+// up once per device (from wdl_ctx_create); host code uses only this address,
+// never the array's name, which on the host is a stand-in and no device address.
+int noalloc_scratch_init(int device) {
+  WDL_CHECK(device >= 0 && device < kMaxDevices, WDL_ERR_UNSUPPORTED,
+            "noalloc scratch: device index above " + std::to_string(kMaxDevices - 1));
+  if (g_scratch[device].load(std::memory_order_acquire)) return WDL_OK;
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (g_scratch[device].load(std::memory_order_relaxed)) return WDL_OK;
+  void* p = nullptr;
+  WDL_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  WDL_CHECK(p != nullptr, WDL_ERR_HIP, "noalloc scratch: hipGetSymbolAddress gave NULL");
+  g_scratch[device].store(static_cast<unsigned char*>(p), std::memory_order_release);
   return WDL_OK;
 }
 
@@ -188,6 +227,26 @@ int wdl_gauss2d_log_convolve_signarray(wdl_gauss2d* k, const double* log_u, unsi
   WDL_HIP(hipMemcpyAsync(signs, k->io_bits.p, nbits, hipMemcpyDeviceToHost, s));
   WDL_HIP(hipStreamSynchronize(s));
   return WDL_OK;
+}
+
+int wdl_gauss2d_log_convolve_noalloc(wdl_gauss2d* k, const double* d_log_u, unsigned char* d_signs,
+                                     double* d_out, int nvectors) {
+  WDL_CHECK(k && d_log_u && d_out, WDL_ERR_ARG, "wdl_gauss2d_log_convolve_noalloc: NULL argument");
+  WDL_CHECK(nvectors >= 0, WDL_ERR_ARG, "wdl_gauss2d_log_convolve_noalloc: nvectors < 0");
+  const long long n = (long long)k->W * k->H * nvectors;
+  WDL_CHECK(n <= wdl::kNoAllocValues, WDL_ERR_ARG,
+            "wdl_gauss2d_log_convolve_noalloc: nvectors*W*H = " + std::to_string(n) +
+                " exceeds WDL_NOALLOC_MAX_VALUES = " + std::to_string(wdl::kNoAllocValues));
+  if (nvectors == 0) return WDL_OK;
+  const int dev = k->ctx->device;
+  WDL_HIP(hipSetDevice(dev));
+  int st;
+  if ((st = wdl::noalloc_scratch_init(dev))) return st;  // set at wdl_ctx_create already
+  unsigned char* base = wdl::g_scratch[dev].load(std::memory_order_acquire);
+  double* tmp = reinterpret_cast<double*>(base);
+  unsigned char* sg = base + (size_t)wdl::kNoAllocValues * sizeof(double);
+  return wdl::log_convolve_core(k, d_log_u, d_out, d_signs, nvectors, tmp, d_signs ? sg : nullptr,
+                                k->ctx->stream);
 }
 
 }  // extern "C"
