@@ -44,6 +44,11 @@ extern "C" {
                                    any spread of log values; FP32 sums leave out terms below e^-41 */
 #define WDL_FLAG_BF16X3 128     /* standard domain, sides <= 128: the convolution's MFMA passes in
                                    split-BF16x3 (hi.hi + hi.lo + lo.hi) instead of FP32 */
+#define WDL_FLAG_CHECKPOINT 256 /* solver: keep the b / conv_b / phi histories (inverseWasserstein.h:
+                                   551-558) for a ring of ceil(sqrt(L)) + 2 iterations plus phi at
+                                   every ceil(sqrt(L))-th iteration, and recompute each segment's
+                                   forward during the backward (about one forward pass more);
+                                   results are bit-identical to the full histories */
 
 /* Run modes of wdl_solver_run. */
 #define WDL_RUN_BATCHED 0 /* all P solves advance together, one launch per Sinkhorn stage */
@@ -142,7 +147,9 @@ int wdl_solver_set_problem(wdl_solver* s, const double* atoms, const double* lam
 int wdl_solver_set_problem_device(wdl_solver* s, const float* d_atoms, const float* d_lambdas);
 /* Warm restart (flag WDL_FLAG_WARM_RESTART): b_storage (P x K x N) seeds b^0
  * of every solve (inverseWasserstein.h:1057-1061); every run saves b^L into
- * b_temp (:1081-1085); commit copies b_temp into b_storage (:418-422). */
+ * b_temp (:1081-1085); commit copies b_temp into b_storage (:418-422).  The
+ * store starts at 1 (:380-384).  In the log domain it holds log b, as the
+ * reference's log build does, kept in FP64 on the device (FP32 otherwise). */
 int wdl_solver_set_warm_state(wdl_solver* s, const double* b_storage);
 int wdl_solver_get_warm_state(wdl_solver* s, double* b_temp);
 int wdl_solver_warm_commit(wdl_solver* s);
@@ -160,7 +167,8 @@ int wdl_solver_autotune(wdl_solver* s, int reps, int* nstreams);
  * has one Niters for all, inverseWasserstein.h:546; uneven counts serve
  * requests that arrive with their own budgets).  0 leaves histogram p idle:
  * no launch touches it, its loss reads 0 and its other outputs keep their
- * values. */
+ * values (zero on a new solver), including its share of the summed atom
+ * gradient. */
 int wdl_solver_set_iterations(wdl_solver* s, const int* iterations);
 /* Enqueue the forward L iterations and both unrolled backwards for all P
  * solves (compute_gradient_both, inverseWasserstein.h:544-674, or its log

@@ -64,3 +64,17 @@ def test_null_arguments_are_errors():
     assert L.wdl_server_wait(None, 1, 0.0, None, None, None, None) == -1
     assert L.wdl_server_stats(None, None) == -1
     assert L.wdl_server_destroy(None) == 0
+
+
+def test_checkpoint_footprint():
+    # the 256x256 log workload (K=6, P=32, L=200): 61 GB of histories in full;
+    # checkpointed: a ring of ceil(sqrt(200)) + 2 = 17 slots and 13 checkpoints
+    LOG, CK = wdl.FLAG_LOG_DOMAIN, wdl.FLAG_CHECKPOINT
+    full = wdl.SinkhornSolver.footprint(256, 256, 6, 32, 200, LOG)
+    ck = wdl.SinkhornSolver.footprint(256, 256, 6, 32, 200, LOG | CK)
+    PKN = 32 * 6 * 256 * 256
+    assert full - ck == (3 * (200 - 17) - 13) * PKN * 8
+    assert ck < 0.2 * full
+    # too few iterations for a ring smaller than L: the full histories
+    assert wdl.SinkhornSolver.footprint(64, 64, 3, 4, 6, CK) == \
+        wdl.SinkhornSolver.footprint(64, 64, 3, 4, 6, 0)

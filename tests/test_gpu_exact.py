@@ -217,3 +217,45 @@ def test_exact_projection_config_against_reference(ctx):
         print("request %d, L_p=%d: weight-gradient error %.2e" % (p, iters[p], e), flush=True)
         assert e < TOL, p
         assert rel(r["b_temp"][p], o["b_last"]) < TOL, p
+
+
+@pytest.mark.parametrize("weights_only", [False, True])
+def test_exact_many_groups_and_plain_form(ctx, weights_only):
+    # P=36 histograms in 1 and 3 stream groups and in the plain form (one solve
+    # per launch sequence): the group offsets of the scratch images, iteration
+    # counts and dot partial sums give the same bits in every split, and one
+    # histogram of each of the 3 groups matches the oracle
+    W = H = 16
+    K, P, L, g = 3, 36, 8, 1.0
+    rng = np.random.default_rng(36)
+    obs = synthetic.blob_histograms(rng, W, P, (1, 3), (2, 4))
+    atoms, lam = synthetic.init_atoms_weights(rng, K, P, W * H)
+    iters = rng.integers(5, L + 1, P).astype(np.int32)
+    f = EXACT | wdl.FLAG_WARM_RESTART | (wdl.FLAG_WEIGHTS_ONLY if weights_only else 0)
+    print("exact, P=%d in 1 and 3 groups and plain, weights only %s" % (P, weights_only), flush=True)
+    s = wdl.SinkhornSolver(ctx, W, H, K, P, L, g, 2, f)
+    s.set_observed(obs)
+    s.set_problem(atoms, lam)
+    s.set_iterations(iters)
+    out = {}
+    for name, groups, mode in (("one", 1, wdl.RUN_BATCHED), ("three", 3, wdl.RUN_BATCHED),
+                               ("plain", 3, wdl.RUN_PLAIN)):
+        s.set_streams(groups)
+        s.run(mode)
+        r = s.results()
+        r["b_temp"] = s.get_warm_state()
+        if not weights_only:
+            r["grad_dic"] = s.grad_dic()
+        out[name] = r
+    s.close()
+    for name in ("three", "plain"):
+        for k in out["one"]:
+            np.testing.assert_array_equal(out[name][k], out["one"][k], err_msg="%s %s" % (name, k))
+    ones = np.ones((K, W * H))
+    for p in (0, 12 + 5, P - 1):  # one histogram of each of the 3 groups
+        o = O.gradient_both_log(atoms, lam[p], obs[p], W, H, g, int(iters[p]), 2, b0=ones)
+        assert rel(out["one"]["grad_lambda"][p], o["grad_lambda"]) < TOL, p
+        assert rel(out["one"]["b_temp"][p], o["b_last"]) < TOL, p
+        if not weights_only:
+            assert rel(out["one"]["grad_dic"][p], o["grad_dic"]) < TOL, p
+            assert rel(out["one"]["bary"][p], o["bary"]) < TOL, p

@@ -24,6 +24,7 @@ FLAG_LOG_DOMAIN, FLAG_WARM_RESTART, FLAG_DENSE, FLAG_NO_GRAPH, FLAG_WEIGHTS_ONLY
 FLAG_TILED = 32
 FLAG_EXACT_LSE = 64
 FLAG_BF16X3 = 128
+FLAG_CHECKPOINT = 256
 NOALLOC_MAX_VALUES = 2097152  # WDL_NOALLOC_MAX_VALUES
 RUN_BATCHED, RUN_PLAIN = 0, 1
 STAGE_FORWARD, STAGE_BACKWARD = 0, 1

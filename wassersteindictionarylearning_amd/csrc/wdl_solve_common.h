@@ -34,9 +34,19 @@ struct SolveArgs {
   float* Q;
   float* G0;
   double* lossPart;
-  const float* bInit;
+  const float* bInit;   // warm-restart store b_storage / b_temp (standard domain, FP32)
   float* bSave;
+  const double* lbInit;  // log domain: log b in FP64, as the histories
+  double* lbSave;
   const int* Lp;      // per-histogram Sinkhorn iterations (<= L), or null for L
+  // Checkpointed histories: `ring` history slots hold entry j at j % ring (0:
+  // every entry has its own slot); forward launches skip histograms with fewer
+  // than `rmin` iterations (a segment's recomputation runs only the histograms
+  // whose backward reaches past it).
+  int ring;
+  int rmin;
+  float* LF0;  // standard domain: safe_log phi^s of the previous forward stage
+  float* LF1;  // (ping-pong; the V buffers themselves without checkpoints)
   int weightsOnly;    // skip the dictionary-gradient chain
   int bf3;            // split-BF16x3 convolution passes (standard domain)
   const float* logA;  // K x N log atoms (log-domain solver)
@@ -63,6 +73,11 @@ struct SolveArgs {
   double* xa;
   double* xt;
   double* xdp;  // per-chunk partial sums of the weight-gradient dots
+  // Start of the forward's scratch images in xa / xt: the backward's images of
+  // histogram p lie at p * chains * K * N, the forward's at xfwd + p * K * N
+  // (a region of its own when the backward has two chains, so the forward of
+  // one histogram group never writes what another group's backward reads).
+  long long xfwd;
   // tiled log domain: the backward's cross-atom sums -sum_j v_j, -sum_j r_j (P x N)
   float* nj;
   float* gj;
@@ -90,10 +105,23 @@ __device__ inline bool map_block(const SolveArgs& a, int& p, int& i) {
 
 __device__ inline int steps_of(const SolveArgs& a, int p) { return a.Lp ? a.Lp[p] : a.L; }
 
-// Launch t of the backward (t = L..1) is step t - (L - L_p) of histogram p;
-// histograms with fewer iterations join later.  Returns 0 when p has no step.
+// Forward launch s runs histogram p when p has an iteration s and reaches rmin.
+__device__ inline bool fwd_runs(const SolveArgs& a, int p, int s) {
+  const int lp = steps_of(a, p);
+  return s <= lp && lp >= a.rmin;
+}
+
+// Launch t of the backward (t = L..1) is step t of every histogram that has
+// that many iterations: a histogram with L_p < L starts its backward at launch
+// L_p, so histogram p's history entry of step s is written and read at the
+// same launch index s in both directions.  Returns 0 when p has no step t.
 __device__ inline int bwd_step(const SolveArgs& a, int p, int t) {
-  return t - (a.L - steps_of(a, p));
+  return t <= steps_of(a, p) ? t : 0;
+}
+
+// History slot of entry j (0-based: entry s-1 belongs to iteration s).
+__host__ __device__ inline long long hslot(const SolveArgs& a, int j) {
+  return a.ring ? (long long)(j % a.ring) : (long long)j;
 }
 
 static int stage_grid(int np, int K) { return np >= 8 ? 8 * K * ((np + 7) / 8) : np * K; }

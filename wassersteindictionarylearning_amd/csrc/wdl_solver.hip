@@ -47,7 +47,7 @@ __global__ __launch_bounds__(FwdCfg<BF3>::threads, FwdCfg<BF3>::waves_per_simd) 
   constexpr int kFwdThreads = FwdCfg<BF3>::threads, kFwdWaves = FwdCfg<BF3>::waves;
   typedef typename Vec<VW>::T V;
   int p, i;
-  if (!map_block(a, p, i) || s > steps_of(a, p)) return;
+  if (!map_block(a, p, i) || !fwd_runs(a, p, s)) return;
   extern __shared__ __align__(16) float smem[];
   void* tabs = smem;
   float* buf = smem + ConvPrec<BF3>::kTabBytes / 4;
@@ -65,9 +65,9 @@ __global__ __launch_bounds__(FwdCfg<BF3>::threads, FwdCfg<BF3>::waves_per_simd) 
   } else {
     // safe_log phi_j^{s-1} of every atom j, written by stage s-1 (each
     // workgroup logs its own phi once instead of all K of them)
-    const float* Lf = (((s - 1) & 1) ? a.V1 : a.V0) + hbase;
-    const float* Fi = a.Fh + (long long)(s - 2) * a.stage + img;  // phi_i^{s-1}
-    float* Qd = i == 0 ? a.Qh + (long long)(s - 2) * a.qstage + (long long)p * N : nullptr;
+    const float* Lf = (((s - 1) & 1) ? a.LF1 : a.LF0) + hbase;
+    const float* Fi = a.Fh + hslot(a, s - 2) * a.stage + img;  // phi_i^{s-1}
+    float* Qd = i == 0 ? a.Qh + hslot(a, s - 2) * a.qstage + (long long)p * N : nullptr;
     const float* lam = a.lam + (long long)p * K;
     struct LdF { V lq, fi; };
     px_pass<VW, kUnrollPx, kFwdThreads>(
@@ -91,7 +91,7 @@ __global__ __launch_bounds__(FwdCfg<BF3>::threads, FwdCfg<BF3>::waves_per_simd) 
   }
   ConvPrec<BF3>::template conv<kFwdWaves>(buf, tabs, G);
   {
-    float* Cd = a.Ch + (long long)(s - 1) * a.stage + img;
+    float* Cd = a.Ch + hslot(a, s - 1) * a.stage + img;
     const float* Ai = a.A + (long long)i * N;
     px_pass<VW, kUnrollPx, kFwdThreads>(
         G, [&](int gi) { return vld<VW>(Ai + gi); },
@@ -106,10 +106,11 @@ __global__ __launch_bounds__(FwdCfg<BF3>::threads, FwdCfg<BF3>::waves_per_simd) 
         });
   }
   ConvPrec<BF3>::template conv<kFwdWaves>(buf, tabs, G);
-  float* Fd = a.Fh + (long long)(s - 1) * a.stage + img;
-  // the next stage's geometric mean reads safe_log phi^s (V0 / V1 are free
-  // until the backward writes them; k_final logs phi^L itself)
-  float* Ld = s < steps_of(a, p) ? ((s & 1) ? a.V1 : a.V0) + img : nullptr;
+  float* Fd = a.Fh + hslot(a, s - 1) * a.stage + img;
+  // the next stage's geometric mean reads safe_log phi^s (LF0 / LF1 are the
+  // V buffers, free until the backward writes them, unless histories are
+  // checkpointed; k_final logs phi^L itself)
+  float* Ld = s < steps_of(a, p) ? ((s & 1) ? a.LF1 : a.LF0) + img : nullptr;
   px_pass<VW, kUnrollPx, kFwdThreads>(
       G, [&](int gi) { return 0; },
       [&](int li, int gi, bool ok, int) {
@@ -136,8 +137,8 @@ __global__ __launch_bounds__(kThreads) void k_final(SolveArgs a) {
     }
     return;
   }
-  const float* F = a.Fh + (long long)(Lp - 1) * a.stage + hbase;
-  float* QL = a.Qh + (long long)(Lp - 1) * a.qstage + (long long)p * N;
+  const float* F = a.Fh + hslot(a, Lp - 1) * a.stage + hbase;
+  float* QL = a.Qh + hslot(a, Lp - 1) * a.qstage + (long long)p * N;
   const float* lam = a.lam + (long long)p * K;
   double acc = 0, gs = 0, bad = 0;
   for (long long x = (long long)blockIdx.x * kThreads + threadIdx.x; x < N;
@@ -201,15 +202,15 @@ __global__ __launch_bounds__(BwdCfg<BF3>::threads, BwdCfg<BF3>::waves_per_simd) 
   const long long img = hbase + (long long)i * N;
   const float lpi = a.lam[(long long)p * K + i];
   const float* Ai = a.A + (long long)i * N;
-  const float* Cs = a.Ch + (long long)(s - 1) * a.stage + img;  // c^{s-1}
+  const float* Cs = a.Ch + hslot(a, s - 1) * a.stage + img;  // c^{s-1}
   const bool last = (s == steps_of(a, p));
   struct Two { V x, y; };
   struct Three { V x, y, z; };
   if (chain == 0) {
     const float* Vp = (s & 1) ? a.V0 : a.V1;
     float* Vc = (s & 1) ? a.V1 : a.V0;
-    const float* Qs = a.Qh + (long long)(s - 1) * a.qstage + (long long)p * N;  // q^s
-    const float* Fs = a.Fh + (long long)(s - 1) * a.stage + img;                // phi^s
+    const float* Qs = a.Qh + hslot(a, s - 1) * a.qstage + (long long)p * N;  // q^s
+    const float* Fs = a.Fh + hslot(a, s - 1) * a.stage + img;                // phi^s
     const float* G0 = a.G0 + (long long)p * N;
     px_pass<VW, kUnrollPx, kBwdThreads>(
         G,
@@ -255,7 +256,7 @@ __global__ __launch_bounds__(BwdCfg<BF3>::threads, BwdCfg<BF3>::waves_per_simd) 
         });
     if (s == 1) return;
     ConvPrec<BF3>::template conv<kBwdWaves>(buf, tabs, G);
-    const float* Fm = a.Fh + (long long)(s - 2) * a.stage + img;  // phi^{s-1}
+    const float* Fm = a.Fh + hslot(a, s - 2) * a.stage + img;  // phi^{s-1}
     px_pass<VW, kUnrollPx, kBwdThreads>(
         G, [&](int gi) { return vld<VW>(Fm + gi); },
         [&](int li, int gi, bool ok, V f) {
@@ -264,7 +265,7 @@ __global__ __launch_bounds__(BwdCfg<BF3>::threads, BwdCfg<BF3>::waves_per_simd) 
   } else {
     const float* Rp = (s & 1) ? a.R0 : a.R1;
     float* Rc = (s & 1) ? a.R1 : a.R0;
-    const float* Fs = a.Fh + (long long)(s - 1) * a.stage + img;  // phi^s
+    const float* Fs = a.Fh + hslot(a, s - 1) * a.stage + img;  // phi^s
     const float* G0 = a.G0 + (long long)p * N;
     const float* Qp = a.Q + (long long)p * N;
     double dot = 0, sf = 0, sg = 0;
@@ -316,8 +317,8 @@ __global__ __launch_bounds__(BwdCfg<BF3>::threads, BwdCfg<BF3>::waves_per_simd) 
           vst<VW>(buf + li, ok ? vld<VW>(buf + li) * fdiv(fdiv(-r.y, r.x), r.x) : vsplat<VW>(0.f));
         });
     ConvPrec<BF3>::template conv<kBwdWaves>(buf, tabs, G);
-    const float* Qm = a.Qh + (long long)(s - 2) * a.qstage + (long long)p * N;  // q^{s-1}
-    const float* Fm = a.Fh + (long long)(s - 2) * a.stage + img;                // phi^{s-1}
+    const float* Qm = a.Qh + hslot(a, s - 2) * a.qstage + (long long)p * N;  // q^{s-1}
+    const float* Fm = a.Fh + hslot(a, s - 2) * a.stage + img;                // phi^{s-1}
     px_pass<VW, kUnrollPx, kBwdThreads>(
         G, [&](int gi) { return fdiv(vld<VW>(Qm + gi), vld<VW>(Fm + gi)); },  // b^{s-1}
         [&](int li, int gi, bool ok, V b) {
@@ -346,7 +347,36 @@ __global__ void k_log(const float* __restrict__ x, float* __restrict__ y, long l
 
 __global__ void k_zero_u32(unsigned* p) { *p = 0u; }
 
-__global__ void k_fill(float* __restrict__ dst, float v, long long n) {
+// Checkpointed histories (WDL_FLAG_CHECKPOINT).  k_copy_ck saves phi^c of a
+// group's histograms (n values from a history slot) at a segment boundary c.
+template <class T>
+__global__ void k_copy_ck(const T* __restrict__ src, T* __restrict__ dst, long long n) {
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (long long)gridDim.x * blockDim.x)
+    dst[e] = src[e];
+}
+
+// Before the segment after boundary c is recomputed: phi^c back into its
+// history slot for the histograms whose backward reaches past the segment
+// (L_p >= rmin; the others' slots hold entries their own segments still read),
+// and in the resident standard domain its safe_log into the ping-pong the
+// forward's geometric mean reads (the value k_forward wrote there).
+template <class T>
+__global__ void k_restore(SolveArgs a, const T* __restrict__ ck, T* __restrict__ fslot,
+                          float* __restrict__ lf) {
+  const long long KN = (long long)a.K * a.N, tot = (long long)a.np * KN;
+  const long long o0 = (long long)a.p0 * KN;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < tot;
+       e += (long long)gridDim.x * blockDim.x) {
+    if (steps_of(a, a.p0 + (int)(e / KN)) < a.rmin) continue;
+    const T v = ck[o0 + e];
+    fslot[o0 + e] = v;
+    if (lf) lf[o0 + e] = safe_logf((float)v);
+  }
+}
+
+template <class T>
+__global__ void k_fill(T* __restrict__ dst, T v, long long n) {
   for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
        e += (long long)gridDim.x * blockDim.x)
     dst[e] = v;
@@ -386,7 +416,7 @@ template <int VW>
 __global__ __launch_bounds__(kThreads, 4) void k_forward_log(SolveArgs a, int s) {
   typedef typename Vec<VW>::T V;
   int p, i;
-  if (!map_block(a, p, i) || s > steps_of(a, p)) return;
+  if (!map_block(a, p, i) || !fwd_runs(a, p, s)) return;
   extern __shared__ __align__(16) float smem[];
   const ConvGeom G = a.G;
   const LogSmem sm = log_smem(smem, G);
@@ -399,14 +429,14 @@ __global__ __launch_bounds__(kThreads, 4) void k_forward_log(SolveArgs a, int s)
   struct TwoD { VD x, y; };
   // conv 1 input: log b^{s-1} (b^{s-1} = lq - phi_i, :711-724)
   if (s == 1) {
-    const float* bi = a.bInit ? a.bInit + img : nullptr;  // warm start, FP32
+    const double* bi = a.lbInit ? a.lbInit + img : nullptr;  // warm start (log b, FP64)
     log_conv_lds<VW>(
         sm.buf, sm.tabs, sm.L, G, a.events,
-        [&](int gi) { return bi ? vld<VW>(bi + gi) : vsplat<VW>(0.f); },
-        [&](int, int, int, V b, bool) { return SLog<VW>{vd(b), vsplat<VW>(1.f)}; });
+        [&](int gi) { return bi ? vldd<VW>(bi + gi) : vd(vsplat<VW>(0.f)); },
+        [&](int, int, int, VD b, bool) { return SLog<VW>{b, vsplat<VW>(1.f)}; });
   } else {
-    const double* Fp = (const double*)a.Fh + (long long)(s - 2) * a.stage + hbase;  // log phi^{s-1}
-    double* Bd = (double*)a.Bh + (long long)(s - 2) * a.stage + img;
+    const double* Fp = (const double*)a.Fh + hslot(a, s - 2) * a.stage + hbase;  // log phi^{s-1}
+    double* Bd = (double*)a.Bh + hslot(a, s - 2) * a.stage + img;
     const float* lam = a.lam + (long long)p * K;
     log_conv_lds<VW>(
         sm.buf, sm.tabs, sm.L, G, a.events,
@@ -429,7 +459,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_forward_log(SolveArgs a, int s)
         });
   }
   // conv 2 input: log a = log p - log c (:704-707); store log c
-  double* Cd = (double*)a.Ch + (long long)(s - 1) * a.stage + img;
+  double* Cd = (double*)a.Ch + hslot(a, s - 1) * a.stage + img;
   const float* LA = a.logA + (long long)i * N;
   log_conv_lds<VW>(
       sm.buf, sm.tabs, sm.L, G, a.events, [&](int gi) { return vld<VW>(LA + gi); },
@@ -438,7 +468,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_forward_log(SolveArgs a, int s)
         if (second) vstd<VW>(Cd + gi, c.x);
         return SLog<VW>{vd(la) - c.x, vsplat<VW>(1.f)};
       });
-  double* Fd = (double*)a.Fh + (long long)(s - 1) * a.stage + img;
+  double* Fd = (double*)a.Fh + hslot(a, s - 1) * a.stage + img;
   px_pass_r<VW>(
       G, [&](int) { return 0; },
       [&](int li, int gi, int row, bool ok, int) {
@@ -461,8 +491,8 @@ __global__ __launch_bounds__(kThreads) void k_final_log(SolveArgs a) {
     }
     return;
   }
-  const double* F = (const double*)a.Fh + (long long)(Lp - 1) * a.stage + hbase;
-  double* B = (double*)a.Bh + (long long)(Lp - 1) * a.stage + hbase;
+  const double* F = (const double*)a.Fh + hslot(a, Lp - 1) * a.stage + hbase;
+  double* B = (double*)a.Bh + hslot(a, Lp - 1) * a.stage + hbase;
   const float* lam = a.lam + (long long)p * K;
   double acc = 0, gs = 0;
   for (long long x = (long long)blockIdx.x * kThreads + threadIdx.x; x < N;
@@ -472,7 +502,7 @@ __global__ __launch_bounds__(kThreads) void k_final_log(SolveArgs a) {
     for (int j = 0; j < K; ++j) {
       const double b = lq - F[j * N + x];
       B[j * N + x] = b;
-      if (a.bSave) a.bSave[hbase + j * N + x] = (float)b;
+      if (a.lbSave) a.lbSave[hbase + j * N + x] = b;
     }
     const float q = (float)exp(lq);
     a.Q[(long long)p * N + x] = q;
@@ -507,7 +537,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_backward_log(SolveArgs a, int t
   const long long img = hbase + (long long)i * N;
   const float lpi = a.lam[(long long)p * K + i];
   const float* LA = a.logA + (long long)i * N;
-  const double* Cs = (const double*)a.Ch + (long long)(s - 1) * a.stage + img;  // log c^{s-1}
+  const double* Cs = (const double*)a.Ch + hslot(a, s - 1) * a.stage + img;  // log c^{s-1}
   const bool last = (s == steps_of(a, p));
   typedef typename VecD<VW>::T VD;
   struct Three { V x, y; VD z; };
@@ -527,7 +557,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_backward_log(SolveArgs a, int t
   if (chain == 0) {  // dictionary gradient, :736-770
     const float* Vp = (s & 1) ? a.V0 : a.V1;
     float* Vc = (s & 1) ? a.V1 : a.V0;
-    const double* Bs = (const double*)a.Bh + (long long)(s - 1) * a.stage + img;  // log b^s
+    const double* Bs = (const double*)a.Bh + hslot(a, s - 1) * a.stage + img;  // log b^s
     const float* G0 = a.G0 + (long long)p * N;
     log_conv_lds<VW>(
         sm.buf, sm.tabs, sm.L, G, a.events,
@@ -568,7 +598,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_backward_log(SolveArgs a, int t
         });
     if (s == 1) return;
     log_conv_lds<VW>(sm.buf, sm.tabs, sm.L, G, a.events, mid_load, mid_make);
-    const double* Fm = (const double*)a.Fh + (long long)(s - 2) * a.stage + img;  // log phi^{s-1}
+    const double* Fm = (const double*)a.Fh + hslot(a, s - 2) * a.stage + img;  // log phi^{s-1}
     px_pass_r<VW>(
         G, [&](int gi) { return vldd<VW>(Fm + gi); },
         [&](int li, int gi, int row, bool ok, VD f) {
@@ -580,7 +610,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_backward_log(SolveArgs a, int t
   } else {  // weight gradient, :774-811
     const float* Rp = (s & 1) ? a.R0 : a.R1;
     float* Rc = (s & 1) ? a.R1 : a.R0;
-    const double* Fs = (const double*)a.Fh + (long long)(s - 1) * a.stage + img;  // log phi^s
+    const double* Fs = (const double*)a.Fh + hslot(a, s - 1) * a.stage + img;  // log phi^s
     const float* G0 = a.G0 + (long long)p * N;
     const float* Qp = a.Q + (long long)p * N;
     double dot = 0, sf = 0, sg = 0;
@@ -624,7 +654,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_backward_log(SolveArgs a, int t
     }
     if (s == 1) return;
     log_conv_lds<VW>(sm.buf, sm.tabs, sm.L, G, a.events, mid_load, mid_make);
-    const double* Bm = (const double*)a.Bh + (long long)(s - 2) * a.stage + img;  // log b^{s-1}
+    const double* Bm = (const double*)a.Bh + hslot(a, s - 2) * a.stage + img;  // log b^{s-1}
     px_pass_r<VW>(
         G, [&](int gi) { return vldd<VW>(Bm + gi); },
         [&](int li, int gi, int row, bool ok, VD b) {
@@ -663,7 +693,8 @@ __global__ void k_project_update(float* __restrict__ lam, const double* __restri
 
 // b_storage <- b_temp for the active histograms (the warm restart of :418-422,
 // per histogram).
-__global__ void k_warm_commit_active(float* __restrict__ store, const float* __restrict__ temp,
+template <class T>
+__global__ void k_warm_commit_active(T* __restrict__ store, const T* __restrict__ temp,
                                      const int* __restrict__ Lp, long long KN) {
   const int p = blockIdx.y;
   if (Lp[p] <= 0) return;
@@ -737,7 +768,8 @@ struct wdl_solver {
   int D = 0;
   DevBuf tab, A, lam, obs, Bh, Qh, Ch, Fh, V0, V1, R0, R1, GD, GL, Q, G0, lossPart, bStore,
       bTemp, GDsum, logA, events, Lp, X0, X1, XS0, XS1, GLt, gsumPart, Mt, eta, xa, xt, xdp,
-      nj, gj, sga, sgt, wla, wlb, wlc, wlt, evX, evG, evW, evA, evF, evP;
+      nj, gj, sga, sgt, wla, wlb, wlc, wlt, evX, evG, evW, evA, evF, evP, CK, LF0, LF1;
+  int seg = 0, ring = 0;  // checkpointed histories: segment length, history slots (0: L slots)
   bool tiled = false;  // 64 x 64 tiles with a halo (a side above kMaxSide)
   bool exact = false;  // log domain with the exact-shift line passes (WDL_FLAG_EXACT_LSE)
   TileGeom tg{};
@@ -780,9 +812,16 @@ struct wdl_solver {
     a.G0 = G0.as<float>();
     a.lossPart = lossPart.as<double>();
     const bool warm = flags & WDL_FLAG_WARM_RESTART;
-    a.bInit = warm ? bStore.as<float>() : nullptr;
-    a.bSave = warm ? bTemp.as<float>() : nullptr;
+    const bool logw = warm && (flags & WDL_FLAG_LOG_DOMAIN);
+    a.bInit = warm && !logw ? bStore.as<float>() : nullptr;
+    a.bSave = warm && !logw ? bTemp.as<float>() : nullptr;
+    a.lbInit = logw ? bStore.as<double>() : nullptr;
+    a.lbSave = logw ? bTemp.as<double>() : nullptr;
     a.Lp = Lp.as<int>();
+    a.ring = ring;
+    a.rmin = 0;
+    a.LF0 = ring && !(flags & WDL_FLAG_LOG_DOMAIN) ? LF0.as<float>() : V0.as<float>();
+    a.LF1 = ring && !(flags & WDL_FLAG_LOG_DOMAIN) ? LF1.as<float>() : V1.as<float>();
     a.weightsOnly = (flags & WDL_FLAG_WEIGHTS_ONLY) ? 1 : 0;
     const bool logd = flags & WDL_FLAG_LOG_DOMAIN;
     a.logA = logd ? logA.as<float>() : nullptr;
@@ -803,6 +842,7 @@ struct wdl_solver {
     a.xa = xa.as<double>();
     a.xt = xt.as<double>();
     a.xdp = xdp.as<double>();
+    a.xfwd = (flags & WDL_FLAG_WEIGHTS_ONLY) ? 0 : 2 * a.stage;
     a.nj = nj.as<float>();
     a.gj = gj.as<float>();
     a.sga = sga.as<unsigned char>();
@@ -892,13 +932,33 @@ struct wdl_solver {
     const dim3 blk(kThreads);
     const int gs = stage_grid(np, K);
     int e;
+    auto forward = [&](const SolveArgs& x, int s) -> int {
+      if (exact) return exact_forward(x, st, s);
+      if (tiled) return tiled_forward(x, st, s);
+      launch_forward(x, gs, lds, st, s);
+      return WDL_OK;
+    };
+    auto backward = [&](int s) -> int {
+      if (exact) return exact_backward(a, st, s);
+      if (tiled) return tiled_backward(a, st, s);
+      launch_backward(a, gs, lds, st, s);
+      return WDL_OK;
+    };
+    const bool logd = flags & WDL_FLAG_LOG_DOMAIN;
+    const size_t esz = logd ? sizeof(double) : sizeof(float);
+    const long long gofs = (long long)p0 * K * N, gn = (long long)np * K * N;  // the group's values
+    const dim3 cgrid((unsigned)std::min<long long>((gn + 255) / 256, 4096));
     for (int s = 1; s <= L; ++s) {
-      if (exact) {
-        if ((e = exact_forward(a, st, s))) return e;
-      } else if (!tiled) {
-        launch_forward(a, gs, lds, st, s);
-      } else if ((e = tiled_forward(a, st, s))) {
-        return e;
+      if ((e = forward(a, s))) return e;
+      if (ring && s % seg == 0 && s < L) {  // checkpoint phi^s
+        char* src = (char*)Fh.p + (size_t)(hslot(a, s - 1) * a.stage + gofs) * esz;
+        char* dst = (char*)CK.p + (size_t)((long long)(s / seg - 1) * a.stage + gofs) * esz;
+        if (logd)
+          hipLaunchKernelGGL(k_copy_ck<double>, cgrid, dim3(256), 0, st, (const double*)src,
+                             (double*)dst, gn);
+        else
+          hipLaunchKernelGGL(k_copy_ck<float>, cgrid, dim3(256), 0, st, (const float*)src,
+                             (float*)dst, gn);
       }
     }
     // kLossBlocks blocks per histogram: every lossPart slot is written
@@ -908,13 +968,38 @@ struct wdl_solver {
     else
       hipLaunchKernelGGL(k_final, dim3(kLossBlocks, np), blk, 0, st, a);
     if (loss == WDL_LOSS_WASSERSTEIN && (e = wasserstein_loss(a, st))) return e;
-    for (int s = L; s >= 1; --s) {
-      if (exact) {
-        if ((e = exact_backward(a, st, s))) return e;
-      } else if (!tiled) {
-        launch_backward(a, gs, lds, st, s);
-      } else if ((e = tiled_backward(a, st, s))) {
-        return e;
+    if (!ring) {
+      for (int s = L; s >= 1; --s)
+        if ((e = backward(s))) return e;
+    } else {
+      // Segments (c, c + seg] from the last: the ring still holds the last
+      // segment of every histogram after the forward (a histogram's last
+      // ring entries are its own last iterations); an earlier segment is
+      // recomputed from its checkpoint for the histograms whose backward
+      // reaches past it, one iteration beyond it so q^e / log b^e exist.
+      const int nseg = (L + seg - 1) / seg;
+      for (int k = nseg - 1; k >= 0; --k) {
+        const int c = k * seg, end = std::min(c + seg, L);
+        if (end < L) {
+          SolveArgs r = a;
+          r.rmin = end + 1;
+          r.events = nullptr;  // the first pass counted these lines
+          if (c > 0) {
+            const long long ck = (long long)(k - 1) * a.stage, fs = hslot(a, c - 1) * a.stage;
+            float* lf = (!logd && !tiled && !exact) ? ((c & 1) ? r.LF1 : r.LF0) : nullptr;
+            const dim3 rg((unsigned)std::min<long long>((gn + 255) / 256, 4096));
+            if (logd)
+              hipLaunchKernelGGL(k_restore<double>, rg, dim3(256), 0, st, r,
+                                 CK.as<double>() + ck, Fh.as<double>() + fs, lf);
+            else
+              hipLaunchKernelGGL(k_restore<float>, rg, dim3(256), 0, st, r,
+                                 CK.as<float>() + ck, Fh.as<float>() + fs, lf);
+          }
+          for (int s = c + 1; s <= end + 1; ++s)
+            if ((e = forward(r, s))) return e;
+        }
+        for (int s = end; s > c; --s)
+          if ((e = backward(s))) return e;
       }
     }
     if (tiled && !exact && (e = tiled_reduce_gl(a, st))) return e;
@@ -953,23 +1038,44 @@ static bool use_tiles(int W, int H, int flags) {
   return !use_exact(flags) && (needs_tiles(W, H) || (flags & WDL_FLAG_TILED));
 }
 
+// Checkpointed histories: segments of seg = ceil(sqrt(L)) iterations, a ring
+// of seg + 2 history slots (a segment's entries, the one before it and the one
+// its recomputation writes beyond it) and one checkpoint of phi per segment
+// boundary.  ring = 0 (L slots) when that would not take less memory: the
+// ring's three histories, the checkpoints and two ping-pong images against
+// three histories of L slots.
+static void ckpt_shape(int L, int flags, int& seg, int& ring, int& nck) {
+  seg = ring = nck = 0;
+  if (!(flags & WDL_FLAG_CHECKPOINT)) return;
+  const int sg = (int)std::ceil(std::sqrt((double)L));
+  if (3 * (sg + 2) + (L + sg - 1) / sg - 1 + 2 >= 3 * L) return;
+  seg = sg;
+  ring = sg + 2;
+  nck = (L + sg - 1) / sg - 1;
+}
+
 static long long footprint(int W, int H, int K, int P, int L, int flags) {
   const long long N = (long long)W * H, PKN = (long long)P * K * N;
   const bool logd = flags & WDL_FLAG_LOG_DOMAIN;
-  long long f = (flags & WDL_FLAG_LOG_DOMAIN) ? 3LL * L * PKN * 8           // Bh, Ch, Fh (FP64)
-                                             : 2LL * L * PKN * 4 + (long long)L * P * N * 4;  // Ch, Fh, Qh
+  int seg, ring, nck;
+  ckpt_shape(L, flags, seg, ring, nck);
+  const long long hs = ring ? ring : L;  // history slots
+  long long f = (flags & WDL_FLAG_LOG_DOMAIN) ? 3LL * hs * PKN * 8           // Bh, Ch, Fh (FP64)
+                                             : 2LL * hs * PKN * 4 + hs * P * N * 4;  // Ch, Fh, Qh
+  if (ring) f += (long long)nck * PKN * (logd ? 8 : 4) + (logd ? 0 : 2 * PKN * 4);  // CK, LF0/1
   f += 5 * PKN * 4;                          // V0, V1, R0, R1, GD
   f += (long long)K * N * (4 + 8);           // A, GDsum
   f += (long long)P * N * 4 * 3;             // obs, Q, G0
   f += (long long)P * K * (4 + 8) + 2LL * P * kLossBlocks * 8 + (long long)P * 4;
-  if (flags & WDL_FLAG_WARM_RESTART) f += 2 * PKN * 4;
+  if (flags & WDL_FLAG_WARM_RESTART) f += 2 * PKN * (logd ? 8 : 4);  // log domain: log b, FP64
   if (flags & WDL_FLAG_LOG_DOMAIN) f += (long long)K * N * 4;  // log atoms
   f += kTabLen * 4;
   f += 4LL * P * N * 8;  // the Wasserstein loss's log scalings and convolution buffers
   f += (3LL * P * K + 3LL * K * N + 1) * 8;  // evaluate_both: x, g, weights, atoms (FP64), f
   f += 2LL * K * ev_chunks(N) * 8;            // and the atoms' per-chunk sums
   if (use_exact(flags)) {  // FP64 scratch images xa, xt + sign bytes, the dots' partial sums
-    f += 2LL * 2 * PKN * (8 + 1);
+    const long long nchb = (flags & WDL_FLAG_WEIGHTS_ONLY) ? 1 : 2, xim = nchb == 1 ? 1 : 3;
+    f += 2LL * PKN * (xim * 8 + nchb);
     f += (long long)P * K * xdot_chunks((long long)W * H) * 3 * 8;
   }
   if (use_tiles(W, H, flags)) {  // X0, X1 (+ sign bytes), per-tile dots, log: cross-atom sums
@@ -1043,6 +1149,12 @@ int wdl_solver_create(wdl_ctx* ctx, int W, int H, int K, int P, int L, double ga
   const long long ntiles = (long long)tg.nx * tg.ny;
   const size_t f4 = sizeof(float);
   const size_t hsz = (flags & WDL_FLAG_LOG_DOMAIN) ? sizeof(double) : sizeof(float);  // histories
+  int nck;
+  ckpt_shape(L, flags, s->seg, s->ring, nck);
+  const size_t hs = s->ring ? (size_t)s->ring : (size_t)L;  // history slots
+  // exact-shift scratch images per (histogram, atom): the backward's chains,
+  // plus the forward's own when there are two (SolveArgs::xfwd)
+  const size_t nchb = (flags & WDL_FLAG_WEIGHTS_ONLY) ? 1 : 2, xim = nchb == 1 ? 1 : 3;
   struct {
     DevBuf* b;
     size_t n;
@@ -1051,10 +1163,13 @@ int wdl_solver_create(wdl_ctx* ctx, int W, int H, int K, int P, int L, double ga
       {&s->A, (size_t)K * N * f4},
       {&s->lam, (size_t)P * K * f4},
       {&s->obs, (size_t)P * N * f4},
-      {&s->Bh, (flags & WDL_FLAG_LOG_DOMAIN) ? (size_t)L * PKN * hsz : 0},
-      {&s->Qh, (flags & WDL_FLAG_LOG_DOMAIN) ? 0 : (size_t)L * P * N * f4},
-      {&s->Ch, (size_t)L * PKN * hsz},
-      {&s->Fh, (size_t)L * PKN * hsz},
+      {&s->Bh, (flags & WDL_FLAG_LOG_DOMAIN) ? (size_t)hs * PKN * hsz : 0},
+      {&s->Qh, (flags & WDL_FLAG_LOG_DOMAIN) ? 0 : (size_t)hs * P * N * f4},
+      {&s->Ch, (size_t)hs * PKN * hsz},
+      {&s->Fh, (size_t)hs * PKN * hsz},
+      {&s->CK, (size_t)nck * PKN * hsz},
+      {&s->LF0, s->ring && !(flags & WDL_FLAG_LOG_DOMAIN) ? (size_t)PKN * f4 : 0},
+      {&s->LF1, s->ring && !(flags & WDL_FLAG_LOG_DOMAIN) ? (size_t)PKN * f4 : 0},
       {&s->V0, (size_t)PKN * f4},
       {&s->V1, (size_t)PKN * f4},
       {&s->R0, (size_t)PKN * f4},
@@ -1065,8 +1180,8 @@ int wdl_solver_create(wdl_ctx* ctx, int W, int H, int K, int P, int L, double ga
       {&s->G0, (size_t)P * N * f4},
       {&s->lossPart, (size_t)P * kLossBlocks * sizeof(double)},
       {&s->GDsum, (size_t)K * N * sizeof(double)},
-      {&s->bStore, (flags & WDL_FLAG_WARM_RESTART) ? (size_t)PKN * f4 : 0},
-      {&s->bTemp, (flags & WDL_FLAG_WARM_RESTART) ? (size_t)PKN * f4 : 0},
+      {&s->bStore, (flags & WDL_FLAG_WARM_RESTART) ? (size_t)PKN * hsz : 0},
+      {&s->bTemp, (flags & WDL_FLAG_WARM_RESTART) ? (size_t)PKN * hsz : 0},
       {&s->logA, (flags & WDL_FLAG_LOG_DOMAIN) ? (size_t)K * N * f4 : 0},
       {&s->events, sizeof(unsigned)},
       {&s->Lp, (size_t)P * sizeof(int)},
@@ -1080,11 +1195,11 @@ int wdl_solver_create(wdl_ctx* ctx, int W, int H, int K, int P, int L, double ga
       {&s->Mt, tiled ? (size_t)L * P * K * ntiles * sizeof(double) : 0},
       {&s->gsumPart, (size_t)P * kLossBlocks * sizeof(double)},
       {&s->eta, (size_t)P * sizeof(float)},
-      {&s->xa, use_exact(flags) ? (size_t)2 * PKN * sizeof(double) : 0},
-      {&s->xt, use_exact(flags) ? (size_t)2 * PKN * sizeof(double) : 0},
+      {&s->xa, use_exact(flags) ? (size_t)xim * PKN * sizeof(double) : 0},
+      {&s->xt, use_exact(flags) ? (size_t)xim * PKN * sizeof(double) : 0},
       {&s->xdp, use_exact(flags) ? (size_t)P * K * xdot_chunks(N) * 3 * sizeof(double) : 0},
-      {&s->sga, use_exact(flags) ? (size_t)2 * PKN : 0},
-      {&s->sgt, use_exact(flags) ? (size_t)2 * PKN : 0},
+      {&s->sga, use_exact(flags) ? (size_t)nchb * PKN : 0},
+      {&s->sgt, use_exact(flags) ? (size_t)nchb * PKN : 0},
       {&s->wla, (size_t)P * N * sizeof(double)},
       {&s->wlb, (size_t)P * N * sizeof(double)},
       {&s->wlc, (size_t)P * N * sizeof(double)},
@@ -1107,10 +1222,20 @@ int wdl_solver_create(wdl_ctx* ctx, int W, int H, int K, int P, int L, double ga
     const std::vector<int> lp((size_t)P, L);
     e = hipMemcpy(s->Lp.p, lp.data(), lp.size() * sizeof(int), hipMemcpyHostToDevice);
   }
+  // Outputs start at zero, so a histogram left idle (0 iterations) reads 0
+  // until it first runs, and adds 0 to the summed atom gradient.
+  for (DevBuf* b : {&s->GD, &s->GL, &s->Q, &s->G0, &s->lossPart, &s->gsumPart, &s->GDsum,
+                    &s->bTemp})
+    if (e == hipSuccess && b->bytes) e = hipMemsetAsync(b->p, 0, b->bytes, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e == hipSuccess && (flags & WDL_FLAG_WARM_RESTART)) {
     // b_storage starts at 1 (inverseWasserstein.h:380-384).
-    hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, ctx->stream, s->bStore.as<float>(), 1.f,
-                       PKN);
+    if (flags & WDL_FLAG_LOG_DOMAIN)
+      hipLaunchKernelGGL(k_fill<double>, dim3(1024), dim3(256), 0, ctx->stream,
+                         s->bStore.as<double>(), 1.0, PKN);
+    else
+      hipLaunchKernelGGL(k_fill<float>, dim3(1024), dim3(256), 0, ctx->stream,
+                         s->bStore.as<float>(), 1.f, PKN);
     e = hipStreamSynchronize(ctx->stream);
   }
   if (e != hipSuccess) {
@@ -1148,7 +1273,8 @@ int wdl_solver_destroy(wdl_solver* s) {
                     &s->G0,  &s->lossPart,    &s->bStore,      &s->bTemp,    &s->GDsum,
                     &s->logA, &s->events, &s->Lp, &s->X0, &s->X1, &s->XS0, &s->XS1, &s->GLt, &s->Mt, &s->gsumPart, &s->eta, &s->xa, &s->xt, &s->xdp, &s->nj, &s->gj, &s->sga,
                     &s->sgt, &s->wla, &s->wlb, &s->wlc,
-                    &s->wlt, &s->evX, &s->evG, &s->evW, &s->evA, &s->evF, &s->evP};
+                    &s->wlt, &s->evX, &s->evG, &s->evW, &s->evA, &s->evF, &s->evP,
+                    &s->CK, &s->LF0, &s->LF1};
   for (DevBuf* b : bufs) b->release();
   delete s;
   return WDL_OK;
@@ -1234,9 +1360,15 @@ int wdl_solver_set_warm_state(wdl_solver* s, const double* b_storage) {
   WDL_CHECK(s->flags & WDL_FLAG_WARM_RESTART, WDL_ERR_STATE,
             "wdl_solver_set_warm_state: solver created without WDL_FLAG_WARM_RESTART");
   const size_t n = (size_t)s->P * s->K * s->N;
+  WDL_HIP(hipSetDevice(s->ctx->device));
+  if (s->flags & WDL_FLAG_LOG_DOMAIN) {  // log b, FP64
+    WDL_HIP(hipMemcpyAsync(s->bStore.p, b_storage, n * sizeof(double), hipMemcpyHostToDevice,
+                           s->ctx->stream));
+    WDL_HIP(hipStreamSynchronize(s->ctx->stream));
+    return WDL_OK;
+  }
   std::vector<float> h(n);
   for (size_t e = 0; e < n; ++e) h[e] = (float)b_storage[e];
-  WDL_HIP(hipSetDevice(s->ctx->device));
   WDL_HIP(hipMemcpyAsync(s->bStore.p, h.data(), n * sizeof(float), hipMemcpyHostToDevice,
                          s->ctx->stream));
   WDL_HIP(hipStreamSynchronize(s->ctx->stream));
@@ -1248,8 +1380,14 @@ int wdl_solver_get_warm_state(wdl_solver* s, double* b_temp) {
   WDL_CHECK(s->flags & WDL_FLAG_WARM_RESTART, WDL_ERR_STATE,
             "wdl_solver_get_warm_state: solver created without WDL_FLAG_WARM_RESTART");
   const size_t n = (size_t)s->P * s->K * s->N;
-  std::vector<float> h(n);
   WDL_HIP(hipSetDevice(s->ctx->device));
+  if (s->flags & WDL_FLAG_LOG_DOMAIN) {  // log b, FP64
+    WDL_HIP(hipMemcpyAsync(b_temp, s->bTemp.p, n * sizeof(double), hipMemcpyDeviceToHost,
+                           s->ctx->stream));
+    WDL_HIP(hipStreamSynchronize(s->ctx->stream));
+    return WDL_OK;
+  }
+  std::vector<float> h(n);
   WDL_HIP(hipMemcpyAsync(h.data(), s->bTemp.p, n * sizeof(float), hipMemcpyDeviceToHost,
                          s->ctx->stream));
   WDL_HIP(hipStreamSynchronize(s->ctx->stream));
@@ -1262,7 +1400,8 @@ int wdl_solver_warm_commit(wdl_solver* s) {
   WDL_CHECK(s->flags & WDL_FLAG_WARM_RESTART, WDL_ERR_STATE,
             "wdl_solver_warm_commit: solver created without WDL_FLAG_WARM_RESTART");
   WDL_HIP(hipSetDevice(s->ctx->device));
-  WDL_HIP(hipMemcpyAsync(s->bStore.p, s->bTemp.p, (size_t)s->P * s->K * s->N * sizeof(float),
+  const size_t esz = (s->flags & WDL_FLAG_LOG_DOMAIN) ? sizeof(double) : sizeof(float);
+  WDL_HIP(hipMemcpyAsync(s->bStore.p, s->bTemp.p, (size_t)s->P * s->K * s->N * esz,
                          hipMemcpyDeviceToDevice, s->ctx->stream));
   return WDL_OK;
 }
@@ -1473,8 +1612,12 @@ int wdl_solver_set_slot(wdl_solver* s, int slot, const double* observed, const d
   WDL_HIP(hipMemcpyAsync(s->eta.as<float>() + slot, &et, sizeof(float), hipMemcpyHostToDevice, st));
   if (s->flags & WDL_FLAG_WARM_RESTART) {
     // the start value of b_storage, 1 in both builds of the reference (:380-384)
-    hipLaunchKernelGGL(k_fill, dim3(256), dim3(256), 0, st,
-                       s->bStore.as<float>() + (long long)slot * K * N, 1.f, (long long)K * N);
+    if (s->flags & WDL_FLAG_LOG_DOMAIN)
+      hipLaunchKernelGGL(k_fill<double>, dim3(256), dim3(256), 0, st,
+                         s->bStore.as<double>() + (long long)slot * K * N, 1.0, (long long)K * N);
+    else
+      hipLaunchKernelGGL(k_fill<float>, dim3(256), dim3(256), 0, st,
+                         s->bStore.as<float>() + (long long)slot * K * N, 1.f, (long long)K * N);
     WDL_HIP(hipGetLastError());
   }
   WDL_HIP(hipStreamSynchronize(st));
@@ -1492,9 +1635,13 @@ int wdl_solver_projection_round(wdl_solver* s) {
   hipLaunchKernelGGL(k_project_update, dim3((s->P + 63) / 64), dim3(64), 0, str, s->lam.as<float>(),
                      s->GL.as<double>(), s->Lp.as<int>(), s->eta.as<float>(), s->P, s->K);
   const long long KN = (long long)s->K * s->N;
-  hipLaunchKernelGGL(k_warm_commit_active, dim3((unsigned)std::min<long long>(64, (KN + 255) / 256),
-                     (unsigned)s->P), dim3(256), 0, str, s->bStore.as<float>(), s->bTemp.as<float>(),
-                     s->Lp.as<int>(), KN);
+  const dim3 cg((unsigned)std::min<long long>(64, (KN + 255) / 256), (unsigned)s->P);
+  if (s->flags & WDL_FLAG_LOG_DOMAIN)
+    hipLaunchKernelGGL(k_warm_commit_active<double>, cg, dim3(256), 0, str, s->bStore.as<double>(),
+                       s->bTemp.as<double>(), s->Lp.as<int>(), KN);
+  else
+    hipLaunchKernelGGL(k_warm_commit_active<float>, cg, dim3(256), 0, str, s->bStore.as<float>(),
+                       s->bTemp.as<float>(), s->Lp.as<int>(), KN);
   WDL_HIP(hipGetLastError());
   return WDL_OK;
 }

@@ -103,11 +103,11 @@ template <int VW>
 __global__ __launch_bounds__(256) void k_tq(SolveArgs a, int s) {
   typedef typename Vec<VW>::T V;
   const int p = a.p0 + (int)blockIdx.y;
-  if (s > steps_of(a, p)) return;
+  if (!fwd_runs(a, p, s)) return;
   const long long N = a.N;
   const int K = a.K;
-  const float* F = a.Fh + (long long)(s - 2) * a.stage + (long long)p * K * N;
-  float* Q = a.Qh + (long long)(s - 2) * a.qstage + (long long)p * N;
+  const float* F = a.Fh + hslot(a, s - 2) * a.stage + (long long)p * K * N;
+  float* Q = a.Qh + hslot(a, s - 2) * a.qstage + (long long)p * N;
   const float* lam = a.lam + (long long)p * K;
   for (long long x = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * VW; x < N;
        x += (long long)gridDim.x * blockDim.x * VW) {
@@ -122,7 +122,7 @@ template <int VW>
 __global__ __launch_bounds__(kThreads, 4) void k_tfwd1(SolveArgs a, int s) {
   typedef typename Vec<VW>::T V;
   int p, i, tile;
-  if (!map_tile(a, p, i, tile) || s > steps_of(a, p)) return;
+  if (!map_tile(a, p, i, tile) || !fwd_runs(a, p, s)) return;
   extern __shared__ __align__(16) float smem[];
   const TileGeom T = a.T;
   const TileSmem sm = tile_smem(smem, T);
@@ -136,8 +136,8 @@ __global__ __launch_bounds__(kThreads, 4) void k_tfwd1(SolveArgs a, int s) {
         T, tx, ty, [&](int gi) { return bi ? vld<VW>(bi + gi) : vsplat<VW>(1.f); },
         [&](int li, int, bool ok, V b) { vst<VW>(sm.buf + li, ok ? b : vsplat<VW>(0.f)); });
   } else {
-    const float* Qp = a.Qh + (long long)(s - 2) * a.qstage + (long long)p * N;  // q^{s-1}
-    const float* Fp = a.Fh + (long long)(s - 2) * a.stage + img;                // phi^{s-1}
+    const float* Qp = a.Qh + hslot(a, s - 2) * a.qstage + (long long)p * N;  // q^{s-1}
+    const float* Fp = a.Fh + hslot(a, s - 2) * a.stage + img;                // phi^{s-1}
     struct Two { V q, f; };
     tile_region_pass<VW>(
         T, tx, ty,
@@ -152,7 +152,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_tfwd1(SolveArgs a, int s) {
         });
   }
   tile_conv(sm.buf, sm.tabs, T);
-  float* Cd = a.Ch + (long long)(s - 1) * a.stage + img;
+  float* Cd = a.Ch + hslot(a, s - 1) * a.stage + img;
   tile_inner_pass<VW>(
       T, tx, ty, [&](int) { return 0; },
       [&](int li, int gi, bool ok, int) {
@@ -165,7 +165,7 @@ template <int VW>
 __global__ __launch_bounds__(kThreads, 4) void k_tfwd2(SolveArgs a, int s) {
   typedef typename Vec<VW>::T V;
   int p, i, tile;
-  if (!map_tile(a, p, i, tile) || s > steps_of(a, p)) return;
+  if (!map_tile(a, p, i, tile) || !fwd_runs(a, p, s)) return;
   extern __shared__ __align__(16) float smem[];
   const TileGeom T = a.T;
   const TileSmem sm = tile_smem(smem, T);
@@ -174,7 +174,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_tfwd2(SolveArgs a, int s) {
   const long long N = a.N;
   const long long img = ((long long)p * a.K + i) * N;
   const float* Ai = a.A + (long long)i * N;
-  const float* Cs = a.Ch + (long long)(s - 1) * a.stage + img;
+  const float* Cs = a.Ch + hslot(a, s - 1) * a.stage + img;
   struct Two { V a, c; };
   tile_region_pass<VW>(
       T, tx, ty,
@@ -188,7 +188,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_tfwd2(SolveArgs a, int s) {
         vst<VW>(sm.buf + li, ok ? fdiv(r.a, r.c) : vsplat<VW>(0.f));
       });
   tile_conv(sm.buf, sm.tabs, T);
-  float* Fd = a.Fh + (long long)(s - 1) * a.stage + img;
+  float* Fd = a.Fh + hslot(a, s - 1) * a.stage + img;
   double lsum = 0;  // sum of log phi over the tile, for the centred weight-gradient dot
   tile_inner_pass<VW>(
       T, tx, ty, [&](int) { return 0; },
@@ -223,15 +223,15 @@ __global__ __launch_bounds__(kThreads, 4) void k_tbwd1(SolveArgs a, int t) {
   const long long img = hbase + (long long)i * N;
   const float lpi = a.lam[(long long)p * K + i];
   const float* Ai = a.A + (long long)i * N;
-  const float* Cs = a.Ch + (long long)(s - 1) * a.stage + img;  // c^{s-1}
-  const float* Fs = a.Fh + (long long)(s - 1) * a.stage + img;  // phi^s
+  const float* Cs = a.Ch + hslot(a, s - 1) * a.stage + img;  // c^{s-1}
+  const float* Fs = a.Fh + hslot(a, s - 1) * a.stage + img;  // phi^s
   const float* G0 = a.G0 + (long long)p * N;
   const bool last = (s == steps_of(a, p));
   struct Three { V x, y, z; };
   struct Two { V x, y; };
   if (chain == 0) {
     const float* Vp = (s & 1) ? a.V0 : a.V1;
-    const float* Qs = a.Qh + (long long)(s - 1) * a.qstage + (long long)p * N;  // q^s
+    const float* Qs = a.Qh + hslot(a, s - 1) * a.qstage + (long long)p * N;  // q^s
     tile_region_pass<VW>(
         T, tx, ty,
         [&](int gi) {
@@ -355,7 +355,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_tbwd2(SolveArgs a, int t) {
       T, tx, ty, [&](int gi) { return vld<VW>(X + gi); },
       [&](int li, int, bool ok, V x) { vst<VW>(sm.buf + li, ok ? x : vsplat<VW>(0.f)); });
   tile_conv(sm.buf, sm.tabs, T);
-  const float* Fm = a.Fh + (long long)(s - 2) * a.stage + img;  // phi^{s-1}
+  const float* Fm = a.Fh + hslot(a, s - 2) * a.stage + img;  // phi^{s-1}
   if (chain == 0) {
     float* Vc = ((s & 1) ? a.V1 : a.V0) + img;
     tile_inner_pass<VW>(
@@ -365,7 +365,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_tbwd2(SolveArgs a, int t) {
         });
   } else {
     float* Rc = ((s & 1) ? a.R1 : a.R0) + img;
-    const float* Qm = a.Qh + (long long)(s - 2) * a.qstage + (long long)p * N;  // q^{s-1}
+    const float* Qm = a.Qh + hslot(a, s - 2) * a.qstage + (long long)p * N;  // q^{s-1}
     tile_inner_pass<VW>(
         T, tx, ty, [&](int gi) { return fdiv(vld<VW>(Qm + gi), vld<VW>(Fm + gi)); },  // b^{s-1}
         [&](int li, int gi, bool ok, V b) {
@@ -422,10 +422,10 @@ __global__ __launch_bounds__(256) void k_tlogb(SolveArgs a, int s) {
        e += (long long)gridDim.x * blockDim.x) {
     const int p = a.p0 + (int)(e / N);
     const long long x = e % N;
-    if (s > steps_of(a, p)) continue;
+    if (!fwd_runs(a, p, s)) continue;
     const long long hb = (long long)p * a.K * N;
-    const double* Fp = (const double*)a.Fh + (long long)(s - 2) * a.stage + hb + x;
-    double* Bp = (double*)a.Bh + (long long)(s - 2) * a.stage + hb + x;
+    const double* Fp = (const double*)a.Fh + hslot(a, s - 2) * a.stage + hb + x;
+    double* Bp = (double*)a.Bh + hslot(a, s - 2) * a.stage + hb + x;
     const float* lam = a.lam + (long long)p * a.K;
     double lq = 0;
     for_atoms_l<4>(
@@ -468,7 +468,7 @@ __global__ __launch_bounds__(kThreads, 6) void k_tfwd1_log(SolveArgs a, int s) {
   typedef typename Vec<VW>::T V;
   typedef typename VecD<VW>::T VD;
   int p, i, tile;
-  if (!map_tile(a, p, i, tile) || s > steps_of(a, p)) return;
+  if (!map_tile(a, p, i, tile) || !fwd_runs(a, p, s)) return;
   extern __shared__ __align__(16) float smem[];
   const TileGeom T = a.T;
   const TileSmem sm = tile_smem(smem, T);
@@ -479,19 +479,19 @@ __global__ __launch_bounds__(kThreads, 6) void k_tfwd1_log(SolveArgs a, int s) {
   const long long hbase = (long long)p * K * N;
   const long long img = hbase + (long long)i * N;
   if (s == 1) {
-    const float* bi = a.bInit ? a.bInit + img : nullptr;
+    const double* bi = a.lbInit ? a.lbInit + img : nullptr;  // warm start (log b, FP64)
     tile_log_conv<VW>(
         sm.buf, sm.tabs, sm.L, T, tx, ty, a.events,
-        [&](int gi) { return bi ? vld<VW>(bi + gi) : vsplat<VW>(0.f); },
-        [&](int, int, V b, bool) { return SLog<VW>{vd(b), vsplat<VW>(1.f)}; });
+        [&](int gi) { return bi ? vldd<VW>(bi + gi) : vd(vsplat<VW>(0.f)); },
+        [&](int, int, VD b, bool) { return SLog<VW>{b, vsplat<VW>(1.f)}; });
   } else {
     // log b^{s-1}, written to the history by k_tlogb
-    const double* Bd = (const double*)a.Bh + (long long)(s - 2) * a.stage + img;
+    const double* Bd = (const double*)a.Bh + hslot(a, s - 2) * a.stage + img;
     tile_log_conv<VW>(
         sm.buf, sm.tabs, sm.L, T, tx, ty, a.events, [&](int gi) { return vldd<VW>(Bd + gi); },
         [&](int, int, const VD& b, bool) { return SLog<VW>{b, vsplat<VW>(1.f)}; });
   }
-  double* Cd = (double*)a.Ch + (long long)(s - 1) * a.stage + img;
+  double* Cd = (double*)a.Ch + hslot(a, s - 1) * a.stage + img;
   tile_inner_pass<VW>(
       T, tx, ty, [&](int) { return 0; },
       [&](int li, int gi, bool ok, int) {
@@ -505,7 +505,7 @@ __global__ __launch_bounds__(kThreads, 6) void k_tfwd2_log(SolveArgs a, int s) {
   typedef typename Vec<VW>::T V;
   typedef typename VecD<VW>::T VD;
   int p, i, tile;
-  if (!map_tile(a, p, i, tile) || s > steps_of(a, p)) return;
+  if (!map_tile(a, p, i, tile) || !fwd_runs(a, p, s)) return;
   extern __shared__ __align__(16) float smem[];
   const TileGeom T = a.T;
   const TileSmem sm = tile_smem(smem, T);
@@ -513,7 +513,7 @@ __global__ __launch_bounds__(kThreads, 6) void k_tfwd2_log(SolveArgs a, int s) {
   const int tx = tile % T.nx, ty = tile / T.nx;
   const long long N = a.N;
   const long long img = ((long long)p * a.K + i) * N;
-  const double* Cs = (const double*)a.Ch + (long long)(s - 1) * a.stage + img;
+  const double* Cs = (const double*)a.Ch + hslot(a, s - 1) * a.stage + img;
   const float* LA = a.logA + (long long)i * N;
   struct TwoL { VD c; V la; };
   tile_log_conv<VW>(
@@ -525,7 +525,7 @@ __global__ __launch_bounds__(kThreads, 6) void k_tfwd2_log(SolveArgs a, int s) {
         return r;
       },
       [&](int, int, const TwoL& r, bool) { return SLog<VW>{vd(r.la) - r.c, vsplat<VW>(1.f)}; });
-  double* Fd = (double*)a.Fh + (long long)(s - 1) * a.stage + img;
+  double* Fd = (double*)a.Fh + hslot(a, s - 1) * a.stage + img;
   double lsum = 0;
   tile_inner_pass<VW>(
       T, tx, ty, [&](int) { return 0; },
@@ -562,7 +562,7 @@ __global__ __launch_bounds__(kThreads, 6) void k_tbwd1_log(SolveArgs a, int t) {
   const long long img = hbase + (long long)i * N;
   const float lpi = a.lam[(long long)p * K + i];
   const float* LA = a.logA + (long long)i * N;
-  const double* Cs = (const double*)a.Ch + (long long)(s - 1) * a.stage + img;  // log c^{s-1}
+  const double* Cs = (const double*)a.Ch + hslot(a, s - 1) * a.stage + img;  // log c^{s-1}
   const float* G0 = a.G0 + (long long)p * N;
   const float* NJ = a.nj + (long long)p * N;
   const float* GJ = a.gj + (long long)p * N;
@@ -580,7 +580,7 @@ __global__ __launch_bounds__(kThreads, 6) void k_tbwd1_log(SolveArgs a, int t) {
   };
   if (chain == 0) {
     const float* Vp = (s & 1) ? a.V0 : a.V1;
-    const double* Bs = (const double*)a.Bh + (long long)(s - 1) * a.stage + img;  // log b^s
+    const double* Bs = (const double*)a.Bh + hslot(a, s - 1) * a.stage + img;  // log b^s
     tile_log_conv<VW>(
         sm.buf, sm.tabs, sm.L, T, tx, ty, a.events,
         [&](int gi) {
@@ -621,7 +621,7 @@ __global__ __launch_bounds__(kThreads, 6) void k_tbwd1_log(SolveArgs a, int t) {
         });
   } else {
     const float* Rp = (s & 1) ? a.R0 : a.R1;
-    const double* Fs = (const double*)a.Fh + (long long)(s - 1) * a.stage + img;  // log phi^s
+    const double* Fs = (const double*)a.Fh + hslot(a, s - 1) * a.stage + img;  // log phi^s
     const float* Qp = a.Q + (long long)p * N;
     const double m = last ? 0.0 : mean_log_phi(a, s, p, i);
     double dot = 0;
@@ -702,7 +702,7 @@ __global__ __launch_bounds__(kThreads, 6) void k_tbwd2_log(SolveArgs a, int t) {
       },
       [&](int, int, const In& r, bool) { return SLog<VW>{r.x, r.s}; });
   if (chain == 0) {
-    const double* Fm = (const double*)a.Fh + (long long)(s - 2) * a.stage + img;  // log phi^{s-1}
+    const double* Fm = (const double*)a.Fh + hslot(a, s - 2) * a.stage + img;  // log phi^{s-1}
     float* Vc = ((s & 1) ? a.V1 : a.V0) + img;
     tile_inner_pass<VW>(
         T, tx, ty, [&](int gi) { return vldd<VW>(Fm + gi); },
@@ -712,7 +712,7 @@ __global__ __launch_bounds__(kThreads, 6) void k_tbwd2_log(SolveArgs a, int t) {
           vst<VW>(Vc + gi, v.s * vexpf(vf(v.x - f)));  // -v (:762)
         });
   } else {
-    const double* Bm = (const double*)a.Bh + (long long)(s - 2) * a.stage + img;  // log b^{s-1}
+    const double* Bm = (const double*)a.Bh + hslot(a, s - 2) * a.stage + img;  // log b^{s-1}
     float* Rc = ((s & 1) ? a.R1 : a.R0) + img;
     tile_inner_pass<VW>(
         T, tx, ty, [&](int gi) { return vldd<VW>(Bm + gi); },

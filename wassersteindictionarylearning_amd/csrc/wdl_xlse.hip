@@ -417,7 +417,7 @@ struct XEpiBwd1 {
     const int p = a.p0 + (int)(pc / nch);
     const int s = bwd_step(a, p, t);
     const long long pimg = ((long long)p * a.K + i) * N;
-    const double cs = ((const double*)a.Ch)[(long long)(s - 1) * a.stage + pimg + x];
+    const double cs = ((const double*)a.Ch)[hslot(a, s - 1) * a.stage + pimg + x];
     if (chain == 0) {
       float* gd = a.GD + pimg + x;
       const float prev = s == steps_of(a, p) ? 0.f : *gd;
@@ -445,10 +445,10 @@ struct XEpiBwd2 {
     const long long pimg = ((long long)p * a.K + i) * N;
     const float sg = sgb ? 1.f : -1.f;
     if (chain == 0) {
-      const double f = ((const double*)a.Fh)[(long long)(s - 2) * a.stage + pimg + x];
+      const double f = ((const double*)a.Fh)[hslot(a, s - 2) * a.stage + pimg + x];
       ((s & 1) ? a.V1 : a.V0)[pimg + x] = sg * expf((float)(v - f));
     } else {
-      const double b = ((const double*)a.Bh)[(long long)(s - 2) * a.stage + pimg + x];
+      const double b = ((const double*)a.Bh)[hslot(a, s - 2) * a.stage + pimg + x];
       ((s & 1) ? a.R1 : a.R0)[pimg + x] = sg * expf((float)(v + b));
     }
   }
@@ -511,14 +511,14 @@ __global__ void k_xf1(SolveArgs a, int s) {
   XLOOP(tot) {
     const int p = a.p0 + (int)(e / N);
     const long long x = e % N;
-    if (s > steps_of(a, p)) continue;
+    if (!fwd_runs(a, p, s)) continue;
     const long long hb = (long long)p * a.K * N;
     if (s == 1) {
       for (int i = 0; i < a.K; ++i)
-        a.xa[hb + i * N + x] = a.bInit ? (double)a.bInit[hb + i * N + x] : 0.0;
+        a.xa[a.xfwd + hb + i * N + x] = a.lbInit ? a.lbInit[hb + i * N + x] : 0.0;
     } else {
-      const double* Fp = (const double*)a.Fh + (long long)(s - 2) * a.stage + hb + x;
-      double* Bp = (double*)a.Bh + (long long)(s - 2) * a.stage + hb + x;
+      const double* Fp = (const double*)a.Fh + hslot(a, s - 2) * a.stage + hb + x;
+      double* Bp = (double*)a.Bh + hslot(a, s - 2) * a.stage + hb + x;
       double lq = 0;
       const float* lam = a.lam + (long long)p * a.K;
       for_atoms_l<4>(
@@ -527,7 +527,7 @@ __global__ void k_xf1(SolveArgs a, int s) {
       for (int i = 0; i < a.K; ++i) {
         const double b = lq - Fp[i * N];
         Bp[i * N] = b;
-        a.xa[hb + i * N + x] = b;  // image (p, i) of the one-chain layout
+        a.xa[a.xfwd + hb + i * N + x] = b;  // image (p, i) of the forward's layout
       }
     }
   }
@@ -559,7 +559,7 @@ __global__ void k_xb1(SolveArgs a, int t, int nch) {
           });
     }
     const float xv = lpi * acc + own;
-    const long long h = (long long)(s - 1) * a.stage + r.img + r.x;
+    const long long h = hslot(a, s - 1) * a.stage + r.img + r.x;
     const double lh = chain == 0 ? ((const double*)a.Bh)[h] : -((const double*)a.Fh)[h];
     const long long o = xoff(a, nch, r);
     a.xa[o] = hlog(fabsf(xv)) + lh;
@@ -587,7 +587,7 @@ __global__ __launch_bounds__(kThreads) void k_xdot_chunk(SolveArgs a, int t) {
   const long long hb = (long long)p * a.K * N, img = hb + (long long)i * N;
   const float lpi = a.lam[(long long)p * a.K + i];
   const float* Rp = (s & 1) ? a.R0 : a.R1;
-  const double* F = (const double*)a.Fh + (long long)(s - 1) * a.stage + img;
+  const double* F = (const double*)a.Fh + hslot(a, s - 1) * a.stage + img;
   const long long xe = (long long)(c + 1) * kXDotChunk, x1 = xe < N ? xe : N;
   double sfg = 0, sf = 0, sg = 0;
   for (long long x = (long long)c * kXDotChunk + threadIdx.x; x < x1; x += blockDim.x) {
@@ -652,13 +652,16 @@ int exact_forward(const SolveArgs& a, hipStream_t st, int s) {
   int e;
   hipLaunchKernelGGL(k_xf1, ew_grid((long long)a.np * a.N), dim3(256), 0, st, a, s);
   // c^{s-1}, and log p - c^{s-1} as the second convolution's input
-  const XEpiFwd1 epi{(double*)a.Ch + (long long)(s - 1) * a.stage + off, a.xa + off, a.logA, a.K,
+  double* xa = a.xa + a.xfwd + off;
+  double* xt = a.xt + a.xfwd + off;
+  const XEpiFwd1 epi{(double*)a.Ch + hslot(a, s - 1) * a.stage + off, xa, a.logA, a.K,
                      a.N};
-  if ((e = xlse_conv_e(a.xa + off, nullptr, epi, a.xt + off, nullptr, W, H, nimg, g, a.Lp, a.p0,
-                       a.K, s, st)))
+  const int thr = std::max(s, a.rmin);  // lines of histograms that run iteration s
+  if ((e = xlse_conv_e(xa, nullptr, epi, xt, nullptr, W, H, nimg, g, a.Lp, a.p0,
+                       a.K, thr, st)))
     return e;
-  if ((e = xlse_conv(a.xa + off, nullptr, (double*)a.Fh + (long long)(s - 1) * a.stage + off,
-                     nullptr, a.xt + off, nullptr, W, H, nimg, g, a.Lp, a.p0, a.K, s, st)))
+  if ((e = xlse_conv(xa, nullptr, (double*)a.Fh + hslot(a, s - 1) * a.stage + off,
+                     nullptr, xt, nullptr, W, H, nimg, g, a.Lp, a.p0, a.K, thr, st)))
     return e;
   WDL_HIP(hipGetLastError());
   return WDL_OK;
@@ -679,13 +682,12 @@ int exact_backward(const SolveArgs& a, hipStream_t st, int t) {
     hipLaunchKernelGGL(k_xb1, ew_grid(n), dim3(256), 0, st, a, t, nch);
   }
   hipLaunchKernelGGL(k_xdot_fin, ew_grid((long long)a.np * a.K), dim3(256), 0, st, a, t);
-  // histogram p runs this step when L_p >= L - t + 1, the second convolution when >= L - t + 2
+  // histogram p runs step t when L_p >= t (the second convolution for t >= 2)
   if ((e = xlse_conv_e(a.xa + off, a.sga + off, XEpiBwd1{a, t, nch, off}, a.xt + off, a.sgt + off,
-                       W, H, nimg, g, a.Lp, a.p0, nch * a.K, a.L - t + 1, st)))
+                       W, H, nimg, g, a.Lp, a.p0, nch * a.K, t, st)))
     return e;
   if (t >= 2 && (e = xlse_conv_e(a.xa + off, a.sga + off, XEpiBwd2{a, t, nch}, a.xt + off,
-                                 a.sgt + off, W, H, nimg, g, a.Lp, a.p0, nch * a.K, a.L - t + 2,
-                                 st)))
+                                 a.sgt + off, W, H, nimg, g, a.Lp, a.p0, nch * a.K, t, st)))
     return e;
   WDL_HIP(hipGetLastError());
   return WDL_OK;
