@@ -19,6 +19,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -67,7 +68,26 @@ def conv_flops(n, D):
     return 2 * pass_flops(n, n, D), 4 * n ** 3
 
 
-def projection_leg(ctx, requests=512, chunk=512):
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def reference_fixture(name):
+    """Outputs of the reference's FP64 log-domain solver on this benchmark's own
+    inputs (tests/golden/large_*.npz, written by tests/golden/make_large.py from
+    the compiled reference), or None when absent."""
+    path = os.path.join(GOLDEN, name + ".npz")
+    if not os.path.exists(path):
+        return None
+    with np.load(path, allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def rel_err(a, b):
+    a, b = np.asarray(a, np.float64).ravel(), np.asarray(b, np.float64).ravel()
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def projection_leg(ctx, requests=4096, chunk=512, one_launch=True):
     """Uneven batched projection: a fixed dictionary (K=8 blob atoms, 128x128),
     4096 histograms, per-request Sinkhorn iterations uniform in 20..100, weights
     gradient only, b seeded from the warm-restart store.  The scalings of this
@@ -75,7 +95,10 @@ def projection_leg(ctx, requests=512, chunk=512):
     domain overflows FP32 within ~10 iterations and FP64 within ~90), so it runs
     in the log domain with exact per-output shifts.  The first `requests` of
     the 4096 run, in chunks of `chunk` (the unrolled log histories of 512
-    requests take 165 GB), each launched eagerly once."""
+    requests take 165 GB), each launched eagerly once.  The weight gradients of
+    the requests the reference fixture holds are compared with it.  With
+    one_launch, all requests also run as one batched solve with checkpointed
+    histories (235 GB for 4096 requests instead of 1.3 TB in full)."""
     c = synthetic.make_problem("projection")
     n, K, L, g = c["n"], c["K"], c["L"], c["gamma"]
     P = min(requests, c["P"])
@@ -87,6 +110,8 @@ def projection_leg(ctx, requests=512, chunk=512):
     stream = ctx.stream
     e0, e1 = _hip.Event(), _hip.Event()
     total_ms, iters, finite = 0.0, 0, 0
+    fx = reference_fixture("large_projection")
+    errs = {}
     for c0 in range(0, P, chunk):
         sl = slice(c0, c0 + chunk)
         s.set_observed(c["obs"][sl])
@@ -101,8 +126,42 @@ def projection_leg(ctx, requests=512, chunk=512):
         iters += int(c["iterations"][sl].sum())
         r = s.results(barycenters=False)
         finite += int((np.isfinite(r["grad_lambda"]).all(1) & np.isfinite(r["loss"])).sum())
+        if fx is not None:
+            for j, p in enumerate(fx["requests"]):
+                if c0 <= p < c0 + chunk:
+                    errs[int(p)] = rel_err(r["grad_lambda"][p - c0], fx["grad_lambda"][j])
         log("projection chunk %d: %.1f ms" % (c0 // chunk, e0.elapsed_ms(e1)))
     s.close()
+    one = None
+    if one_launch:
+        s = wdl.SinkhornSolver(ctx, n, n, K, P, L, g, c["loss"], flags | wdl.FLAG_CHECKPOINT)
+        s.set_observed(c["obs"][:P])
+        s.set_problem(atoms, c["weights"][:P])
+        s.set_iterations(c["iterations"][:P])
+        ctx.synchronize()
+        e0.record(stream)
+        s.run()
+        e1.record(stream)
+        e1.synchronize()
+        ms1 = e0.elapsed_ms(e1)
+        r = s.results(barycenters=False)
+        errs1 = {}
+        if fx is not None:
+            for j, p in enumerate(fx["requests"]):
+                if p < P:
+                    errs1[int(p)] = rel_err(r["grad_lambda"][p], fx["grad_lambda"][j])
+        one = {"requests": P, "ms": round(ms1, 3),
+               "value": round(int(c["iterations"][:P].sum()) / (ms1 / 1e3), 1),
+               "unit": "Sinkhorn-iters/s",
+               "hbm_footprint_gb": round(wdl.SinkhornSolver.footprint(
+                   n, n, K, P, L, flags | wdl.FLAG_CHECKPOINT) / 1e9, 2),
+               "finite_results": int((np.isfinite(r["grad_lambda"]).all(1)
+                                      & np.isfinite(r["loss"])).sum()),
+               "err_vs_fp64_grad_lambda": max(errs1.values()) if errs1 else None,
+               "form": "all requests in one batched solve, histories checkpointed "
+                       "(ring of ceil(sqrt(L))+2 iterations, segments recomputed in the backward)"}
+        log("projection, one launch of %d (checkpointed): %.1f ms" % (P, ms1))
+        s.close()
     e0.destroy()
     e1.destroy()
     return {"workload": "projection: 4096 requests, 128x128, K=8 fixed blob dictionary, gamma=1, "
@@ -110,7 +169,11 @@ def projection_leg(ctx, requests=512, chunk=512):
                         "log domain with exact per-output shifts",
             "requests": P, "of": c["P"], "chunk": chunk, "sinkhorn_iters": iters, "finite_results": finite,
             "ms": round(total_ms, 3), "value": round(iters / (total_ms / 1e3), 1),
-            "unit": "Sinkhorn-iters/s"}
+            "unit": "Sinkhorn-iters/s", "one_launch": one,
+            "err_vs_fp64": {"grad_lambda": max(errs.values()) if errs else None,
+                            "requests_checked": sorted(errs),
+                            "against": "reference compute_gradient_both_log_signArray (FP64, "
+                                       "tests/golden/large_projection.npz)"}}
 
 
 def served_leg(duration_s=20.0, rate=2.0, rounds=3, step=0.5, budget=200 << 30, seed=11):
@@ -121,24 +184,14 @@ def served_leg(duration_s=20.0, rate=2.0, rounds=3, step=0.5, budget=200 << 30, 
     rounds (log domain, exact shifts: blob dictionaries make the scalings
     diverge).  The native server batches requests continuously within a
     device-memory budget; latency is submit to completion."""
-    rng = np.random.default_rng(seed)
-    sizes, probs = [64, 128, 256, 512], [0.4, 0.3, 0.2, 0.1]
     flags = wdl.FLAG_LOG_DOMAIN | wdl.FLAG_EXACT_LSE
     sv = wdl.ProjectionServer(0, budget, 16)
+    wl, arrivals, rng = synthetic.served_workload(seed, rate, duration_s)
+    sizes = sorted(wl)
     dicts = {}
     for n in sizes:
-        K = int(rng.integers(4, 11))
-        g = float(rng.choice([0.5, 1.0, 2.0]))
-        atoms = synthetic.blob_histograms(rng, n, K, (1, 5), (3, 20))
+        K, g, atoms = wl[n]
         dicts[n] = (sv.add_dictionary(atoms, n, n, g, wdl.LOSS_QUADRATIC, 500, flags), K, g)
-    arrivals, t = [], 0.0
-    while True:
-        t += rng.exponential(1.0 / rate)
-        if t > duration_s:
-            break
-        n = int(rng.choice(sizes, p=probs))
-        L = int(rng.integers(20, 501))
-        arrivals.append((t, n, L, synthetic.blob_histograms(rng, n, 1, (1, 5), (3, 20))[0]))
     # one warm-up request per size (pool creation, graph capture) before the clock
     warm = [sv.submit(dicts[n][0], synthetic.blob_histograms(rng, n, 1, (1, 3), (3, 10))[0], 20, 1,
                       step) for n in sizes]
@@ -156,6 +209,15 @@ def served_leg(duration_s=20.0, rate=2.0, rounds=3, step=0.5, budget=200 << 30, 
         handles.append(sv.submit(dicts[n][0], h, L, rounds, step))
     res = [sv.wait(hd, 900.0) for hd in handles]
     wall = time.perf_counter() - t0
+    fx = reference_fixture("large_served_rounds")
+    err = None
+    if fx is not None and rounds == int(fx["rounds"]) and step == float(fx["step"]):
+        ew = [rel_err(res[int(k)][0], fx["weights"][j]) for j, k in enumerate(fx["requests"])]
+        el = [abs(res[int(k)][1] - fx["loss"][j]) / abs(fx["loss"][j])
+              for j, k in enumerate(fx["requests"])]
+        err = {"weights": max(ew), "loss": max(el), "requests_checked": fx["requests"].tolist(),
+               "against": "the same projection rounds driven by the reference's FP64 "
+                          "compute_gradient_both_log_signArray (tests/golden/large_served_rounds.npz)"}
     st = sv.stats()
     sv.close()
     lat = np.array([r[2] for r in res])
@@ -179,7 +241,8 @@ def served_leg(duration_s=20.0, rate=2.0, rounds=3, step=0.5, budget=200 << 30, 
                          "p99": round(float(np.percentile(qu, 99)), 2)},
             "memory_waits": st["memory_waits"] - st0["memory_waits"],
             "pools_created": st["pools_created"], "pools_released": st["pools_released"],
-            "rounds_launched": st["rounds"] - st0["rounds"], "hbm_budget_gb": budget / 1e9}
+            "rounds_launched": st["rounds"] - st0["rounds"], "hbm_budget_gb": budget / 1e9,
+            "err_vs_fp64": err}
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_backward_batched_std_traffic.md")
@@ -227,6 +290,17 @@ def log_leg(ctx, steps=2):
         e1.synchronize()
         stage[name + "_stage_ms"] = round(e0.elapsed_ms(e1) / 5, 4)
     events = s.log_range_events()
+    fx = reference_fixture("large_c3")
+    err = None
+    if fx is not None and int(fx["L"]) == L:
+        s.run()  # the stage timings above overwrote results of the last full run
+        r = s.results()
+        gd = s.grad_dic()
+        err = {"bary": rel_err(r["bary"][0], fx["bary"]),
+               "grad_lambda": rel_err(r["grad_lambda"][0], fx["grad_lambda"]),
+               "grad_dic": rel_err(gd[0], fx["grad_dic"]), "histograms_checked": [0],
+               "against": "reference compute_gradient_both_log_signArray (FP64, L=200, "
+                          "tests/golden/large_c3.npz)"}
     footprint = wdl.SinkhornSolver.footprint(n, n, K, P, L, wdl.FLAG_LOG_DOMAIN)
     s.close()
     e0.destroy()
@@ -235,19 +309,59 @@ def log_leg(ctx, steps=2):
                        "gamma=0.5, L=200, KL loss, all gradients",
            "steps": steps, "ms_per_step": round(ms, 3),
            "value": round(P * L / (ms / 1e3), 1), "unit": "Sinkhorn-iters/s",
-           "log_range_events": events, "hbm_footprint_gb": round(footprint / 1e9, 2)}
+           "log_range_events": events, "hbm_footprint_gb": round(footprint / 1e9, 2),
+           "err_vs_fp64": err}
     out.update(stage)
     return out
 
 
+def host_cpus():
+    """The host's CPU as lscpu describes it, and the physical cores this
+    process may run on (its affinity mask, one per core however many
+    hardware threads it has)."""
+    info = {}
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in txt.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core",
+                             "CPU(s)"):
+                info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    cpus = sorted(os.sched_getaffinity(0))
+    cores = set()
+    for cpu in cpus:
+        try:
+            base = "/sys/devices/system/cpu/cpu%d/topology/" % cpu
+            with open(base + "physical_package_id") as f1, open(base + "core_id") as f2:
+                cores.add((f1.read().strip(), f2.read().strip()))
+        except OSError:
+            cores.add(("?", str(cpu)))
+    info["visible_cpus"] = len(cpus)
+    info["visible_physical_cores"] = len(cores)
+    return info
+
+
 def cpu_baseline(c, gpu, budget_s=15.0):
     """Reference CPU path (oracle/_ref, compiled from the reference sources) on
-    a bounded sample of the same workload, and the GPU error against it."""
+    a bounded sample of the same workload, and the GPU error against it.
+    OpenMP runs one thread per visible physical core (at most the process's
+    OMP_NUM_THREADS), threads bound close, one place per core."""
+    hc = host_cpus()
+    nthreads = hc["visible_physical_cores"]
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        nthreads = min(nthreads, int(os.environ["OMP_NUM_THREADS"]))
+    # read by the OpenMP runtime when oracle/_ref loads it, below
+    os.environ["OMP_NUM_THREADS"] = str(nthreads)
+    os.environ["OMP_PROC_BIND"] = "close"
+    os.environ["OMP_PLACES"] = "cores"
     from oracle import ref
     if not ref.available():
         return None, None, []
     n, K, L, g, loss = c["n"], c["K"], c["L"], c["gamma"], c["loss"]
-    out = {"cores": ref.num_threads(), "kind": "reference"}
+    out = {"cores": ref.num_threads(), "kind": "reference", "omp": "OMP_PROC_BIND=close "
+           "OMP_PLACES=cores OMP_NUM_THREADS=%d" % nthreads, "host": hc}
     errs = {"grad_lambda": [], "grad_dic": [], "bary": []}
     refs = []
     t_sse, done = 0.0, 0
@@ -280,7 +394,7 @@ def cpu_baseline(c, gpu, budget_s=15.0):
     ref.gradient_both(c["atoms"], c["weights"][0], c["obs"][0], n, n, g, L, loss, variant="sse")
     out["one_thread_value"] = L / (time.perf_counter() - t0)
     ref.set_threads(out["cores"])
-    out["cpu_model"] = cpu_model()
+    out["cpu_model"] = hc.get("Model name") or cpu_model()
     # the log domain (LogSignArrayGaussian2DKernel) on the 256x256 log workload:
     # 2 Sinkhorn iterations of one histogram (its O(n^3) exact passes take ~minutes
     # per histogram at L=200), as iterations/s per histogram
@@ -292,6 +406,32 @@ def cpu_baseline(c, gpu, budget_s=15.0):
                          "unit": "Sinkhorn-iters/s",
                          "sample": "compute_gradient_both_log_signArray, 256x256, K=6, gamma=0.5, "
                                    "KL, one curve histogram, 2 iterations (forward + backward)"}
+    # the projection workload (128x128, K=8 blob dictionary, gamma=1) and the
+    # served workload's 64x64 and 128x128 dictionaries: the reference's
+    # log-domain solver, which computes both chains (a projection needs the
+    # weight chain only), a few iterations of one request each
+    cp = synthetic.make_problem("projection", P=1)
+    atoms_p = synthetic.blob_histograms(np.random.default_rng(7), cp["n"], cp["K"], cp["nblobs"],
+                                        cp["sigma"])
+    t0 = time.perf_counter()
+    ref.gradient_both(atoms_p, cp["weights"][0], cp["obs"][0], cp["n"], cp["n"], cp["gamma"], 4,
+                      cp["loss"], variant="log")
+    out["projection"] = {"value": round(4 / (time.perf_counter() - t0), 3),
+                         "unit": "Sinkhorn-iters/s",
+                         "sample": "compute_gradient_both_log_signArray (both gradient chains), "
+                                   "one projection request, 128x128, K=8, gamma=1, 4 iterations"}
+    wl, arrivals, _ = synthetic.served_workload()
+    served = {}
+    for ns, Ls in ((64, 20), (128, 4)):
+        Ks, gs, atoms_s = wl[ns]
+        h = next(a[3] for a in arrivals if a[1] == ns)
+        t0 = time.perf_counter()
+        ref.gradient_both(atoms_s, np.full(Ks, 1.0 / Ks), h, ns, ns, gs, Ls, 2, variant="log")
+        served[str(ns)] = {"value": round(Ls / (time.perf_counter() - t0), 3),
+                           "unit": "Sinkhorn-iters/s",
+                           "sample": "one served request, %dx%d, K=%d, gamma=%g, %d iterations"
+                                     % (ns, ns, Ks, gs, Ls)}
+    out["served"] = served
     # Gaussian3DKernel::convolve (kernels.h:762-807) on one 64^3 volume
     v = np.random.default_rng(1).uniform(0, 1, LINE_3D[0] ** 3)
     t0 = time.perf_counter()
@@ -465,7 +605,7 @@ def main():
     ap.add_argument("--no-served", action="store_true", help="skip the served projection leg")
     ap.add_argument("--no-bf16x3", action="store_true", help="skip the split-BF16x3 leg")
     ap.add_argument("--no-line", action="store_true", help="skip the 1-D / 3-D kernel leg")
-    ap.add_argument("--projection-requests", type=int, default=512,
+    ap.add_argument("--projection-requests", type=int, default=4096,
                     help="how many of the 4096 projection requests to run")
     ap.add_argument("--served-seconds", type=float, default=20.0,
                     help="arrival window of the served leg")
@@ -586,6 +726,9 @@ def main():
         "flop_count": "algorithmic: 4n^3 per image-convolution, 4 per (histogram, atom)",
         "executed_mfma_flops_per_launch": convs_bwd * f_exec,
         "executed_mfma_tflops": round(executed, 2),
+        "executed_frac": round(executed / PEAK_FP32_MFMA, 4),
+        "hbm_frac": round((traffic if traffic else bytes_bwd) / (bwd_ms * 1e-3) / 1e9 / PEAK_HBM, 4),
+        "hbm_frac_basis": "measured traffic" if traffic else "algorithmic bytes",
         "hbm_algorithmic_bytes_per_launch": bytes_bwd,
         "hbm_achieved_gbs": round(bytes_bwd / (bwd_ms * 1e-3) / 1e9, 1),
         "band_halfwidth": D,

@@ -66,6 +66,31 @@ WORKLOADS = {
 }
 
 
+def served_workload(seed=11, rate=2.0, duration_s=20.0, sizes=(64, 128, 256, 512),
+                    probs=(0.4, 0.3, 0.2, 0.1), max_iterations=500):
+    """The served projection loop's inputs: one random-blob dictionary per grid
+    size (K uniform 4..10, gamma from {0.5, 1, 2}, blobs 1-5 with sigma 3-20),
+    then Poisson arrivals at `rate` per second for `duration_s` seconds, each a
+    blob histogram with a grid size drawn with `probs` and a Sinkhorn iteration
+    count uniform in 20..max_iterations.  Returns (dictionaries {n: (K, gamma,
+    atoms)}, arrivals [(t, n, L, histogram)], the generator for later draws)."""
+    rng = np.random.default_rng(seed)
+    dicts = {}
+    for n in sizes:
+        K = int(rng.integers(4, 11))
+        g = float(rng.choice([0.5, 1.0, 2.0]))
+        dicts[n] = (K, g, blob_histograms(rng, n, K, (1, 5), (3, 20)))
+    arrivals, t = [], 0.0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > duration_s:
+            break
+        n = int(rng.choice(sizes, p=list(probs)))
+        L = int(rng.integers(20, max_iterations + 1))
+        arrivals.append((t, n, L, blob_histograms(rng, n, 1, (1, 5), (3, 20))[0]))
+    return dicts, arrivals, rng
+
+
 def make_problem(name: str, seed: int | None = None, P: int | None = None):
     """Inputs of one workload.  The projection workload's atoms stand for a
     learned dictionary (random blobs, as the configuration states)."""
