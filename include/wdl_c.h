@@ -41,7 +41,7 @@ extern "C" {
 #define WDL_FLAG_WEIGHTS_ONLY 16 /* only dL/dlambda (:635-673): projection onto a fixed dictionary */
 #define WDL_FLAG_TILED 32       /* 64x64-tile kernels at any size (taken anyway above 128 per side) */
 #define WDL_FLAG_EXACT_LSE 64   /* log domain: exact per-output maximum (kernels.h:1082-1091) for
-                                   any spread of log values; FP32 sums leave out terms below e^-41 */
+                                   any spread of log values; FP32 sums leave out terms below e^-25 */
 #define WDL_FLAG_BF16X3 128     /* standard domain, sides <= 128: the convolution's MFMA passes in
                                    split-BF16x3 (hi.hi + hi.lo + lo.hi) instead of FP32 */
 #define WDL_FLAG_CHECKPOINT 256 /* solver: keep the b / conv_b / phi histories (inverseWasserstein.h:

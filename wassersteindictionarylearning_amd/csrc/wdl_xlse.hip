@@ -8,11 +8,12 @@
 // along a line is handled (the shifted MFMA form of wdl_logconv.h needs every
 // line within ~80 nats).  Prefix and suffix maxima of the line bound every
 // term by distance, which fixes for each output a window outside which all
-// terms lie more than 41 nats below m_j (each below e^-41 relative: at most
-// len * 2e-18 of the sum); the window gives m_j exactly and holds every term
-// of the sum that counts.  Terms are formed in FP64, exponentiated and summed
-// in FP32 (each term <= ~1).  Cost per output: the window's terms, ~2 sqrt(41
-// gamma) + 1 when the log values are smooth, up to len when they are not.
+// terms lie more than kXMargin = 25 nats below m_j (each below e^-25 relative:
+// at most len * 1.4e-11 of the sum, under the FP32 rounding of the sum); the
+// window gives m_j exactly and holds every term of the sum that counts.  Terms
+// are formed in FP64, exponentiated and summed in FP32 (each term <= ~1).  Cost
+// per output: the window's terms, ~2 sqrt(25 gamma) + 1 when the log values are
+// smooth, up to len when they are not.
 //
 // The solver stages (exact_forward / exact_backward) run the log-domain
 // recursions of compute_gradient_both_log_signArray (inverseWasserstein.h:
@@ -27,7 +28,10 @@
 namespace wdl {
 
 constexpr int kXWaves = 4;  // lines per workgroup, one wave each
-constexpr float kXMargin = 41.f;  // terms outside a window lie this far below m_j
+// Terms outside a window lie this far below m_j: each below e^-25 = 1.4e-11
+// relative, at most len * 1.4e-11 <= 1.4e-8 of the sum for lines up to 1024,
+// below the FP32 rounding of the sum itself.
+constexpr float kXMargin = 25.f;
 constexpr int kXPad = 32;         // -inf positions on both sides of a staged line
 
 // Dynamic LDS of k_xlse_pass: per line its values (FP64 scaled by log2 e, FP32),
@@ -121,9 +125,9 @@ __device__ inline float xlse_window_sum(const double* X2, const float* SG, const
 // dir = 1, or prefix maxima stepping left, dir = -1; lim: positions available
 // on that side).  The bound B[jc + dir d] - d^2/gamma falls strictly with d,
 // so the distances 4, 8, 12, 16 that pass form a prefix: their count locates
-// the first failing multiple of 4, and one more test at 2 below it gives the
-// first failing even distance d; the radius is d - 1.  Windows wider than 16
-// continue in steps of 2.
+// the first failing multiple of 4, a test at 2 below it the first failing
+// even distance d, and one at d - 1 the first failing distance; the radius is
+// the distance before it.  Windows wider than 16 continue in steps of 2.
 __device__ inline int xlse_radius(const float* B, const float* CF, int jc, int dir, int lim,
                                   float thr, float igf) {
   // the four reads are independent (distances clamped to lim stay in the line
@@ -146,7 +150,41 @@ __device__ inline int xlse_radius(const float* B, const float* CF, int jc, int d
     d = 18;
     while (d <= lim && B[jc + dir * d] - CF[d] > thr) d += 2;
   }
+  // d fails and d - 2 passes (or is 0, the centre); d - 1 decides
+  const int o = d - 1, oo = min(max(o, 0), lim);
+  if (o >= 1 && !((o <= lim) & (B[jc + dir * oo] - CF[oo] > thr))) d = o;
   return min(d - 1, lim);
+}
+
+// Inclusive max-scan over the 64 lanes of a wave with DPP lane moves (row
+// shifts by 1, 2, 4, 8 within each row of 16 lanes, then the broadcasts of
+// lanes 15 and 31 into the rows above): lane i ends with the maximum of lanes
+// 0..i, lane 63 with the wave's.  A lane whose source lies outside its row (or
+// whose row a broadcast leaves out) keeps its own value, which max absorbs.
+// VALU lane moves, where __shfl would be one LDS-crossbar round trip per step.
+__device__ inline int wave_max_scan(int x) {
+  x = max(x, __builtin_amdgcn_update_dpp(x, x, 0x111, 0xf, 0xf, false));  // row_shr:1
+  x = max(x, __builtin_amdgcn_update_dpp(x, x, 0x112, 0xf, 0xf, false));  // row_shr:2
+  x = max(x, __builtin_amdgcn_update_dpp(x, x, 0x114, 0xf, 0xf, false));  // row_shr:4
+  x = max(x, __builtin_amdgcn_update_dpp(x, x, 0x118, 0xf, 0xf, false));  // row_shr:8
+  x = max(x, __builtin_amdgcn_update_dpp(x, x, 0x142, 0xa, 0xf, false));  // row_bcast:15
+  x = max(x, __builtin_amdgcn_update_dpp(x, x, 0x143, 0xc, 0xf, false));  // row_bcast:31
+  return x;
+}
+// The same for floats (the order of FP32 values as fmaxf orders them, NaN-free).
+template <int CTRL, int ROWS>
+__device__ inline float dpp_move(float v) {
+  const int b = __float_as_int(v);
+  return __int_as_float(__builtin_amdgcn_update_dpp(b, b, CTRL, ROWS, 0xf, false));
+}
+__device__ inline float wave_max_scan(float x) {
+  x = fmaxf(x, dpp_move<0x111, 0xf>(x));
+  x = fmaxf(x, dpp_move<0x112, 0xf>(x));
+  x = fmaxf(x, dpp_move<0x114, 0xf>(x));
+  x = fmaxf(x, dpp_move<0x118, 0xf>(x));
+  x = fmaxf(x, dpp_move<0x142, 0xa>(x));
+  x = fmaxf(x, dpp_move<0x143, 0xc>(x));
+  return x;
 }
 
 // Waves per line of k_xlse_pass: long lines share their workgroup's LDS over
@@ -166,13 +204,13 @@ constexpr int kXMaxThreads = kXWaves * 4 * 64;
 // x_k - (j-k)^2/gamma <= PM[k] - (j-k)^2/gamma for k <= j (PM: prefix maxima)
 // and <= SM[k] - (k-j)^2/gamma for k >= j (SM: suffix maxima), and both bounds
 // fall as k moves away from j.  With m0 the maximum of the five terms
-// |j-k| <= 2, the lane's window ends on each side at the first even distance d
-// where the bound at j +- d is at most m0 - 41: every term beyond lies more
-// than 41 nats below m0 <= m_j.  The 64 lanes share one window [j-DL, j+DR]
+// |j-k| <= 2, the lane's window ends on each side before the first distance d
+// where the bound at j +- d is at most m0 - kXMargin: every term beyond lies
+// more than kXMargin nats below m0 <= m_j.  The 64 lanes share one window [j-DL, j+DR]
 // (the widest of theirs), so the loops over it run in step with the penalty
 // t^2/gamma read from one LDS table.  The window gives m_j exactly (terms
 // outside it are below m0); the sum adds every term within it, so the terms
-// left out are each below e^-41 relative to the largest.  m_j is found in FP32
+// left out are each below e^-kXMargin relative to the largest.  m_j is found in FP32
 // (any shift near the maximum serves: the output adds back the shift the terms
 // were formed with); exponents are formed in FP64 in log2 units and
 // exponentiated on the hardware exp2.
@@ -221,45 +259,67 @@ __global__ __launch_bounds__(kXMaxThreads) void k_xlse_pass(
     XF[-1 - lane] = XF[len + lane] = -INFINITY;
     SG[-1 - lane] = SG[len + lane] = 0.f;
   }
-  for (long long l0 = (long long)blockIdx.x * kXWaves; l0 < nlines;
-       l0 += (long long)gridDim.x * kXWaves) {
-    const long long line = l0 + w;
-    bool active = line < nlines;
-    if (active && active_thr > 0) {
-      const long long img = line / rows;
-      active = Lp[p0 + (int)(img / ipp)] >= active_thr;
-    }
-    if (active)  // stage the line: 64 values per wave and step
-      for (int k = sub * 64 + lane; k < len; k += wpl * 64) {
-        const double x = in[line * len + k];
-        X2[k] = x * kLog2e;
-        XF[k] = (float)x;
-        if (SIGNED) SG[k] = sin[line * len + k] ? 1.f : -1.f;  // getSign (signArray.h:19-22)
+  // Per line slot: whether its line runs (1) or is skipped, for the store.
+  unsigned char* ACT = SO0 + kXWaves * len;
+  // The next line group's values (at most kXPer per lane: lines up to 1024
+  // over 4 waves), sign bytes and activity, loaded one group ahead so the
+  // loads' latency passes under the current group's work.
+  constexpr int kXPer = 4;
+  double pv[kXPer];
+  unsigned char ps[kXPer];
+  bool pact = false;
+  auto fetch = [&](long long l0n) {
+    const long long ln = l0n + w;
+    const bool inb = ln < nlines;
+    pact = inb;
+    if (inb && active_thr > 0) pact = Lp[p0 + (int)((ln / rows) / ipp)] >= active_thr;
+#pragma unroll
+    for (int u = 0; u < kXPer; ++u) {
+      const int k = sub * 64 + lane + u * wpl * 64;
+      if (inb && k < len) {
+        pv[u] = in[ln * len + k];
+        if (SIGNED) ps[u] = sin[ln * len + k];
       }
+    }
+  };
+  const long long stride = (long long)gridDim.x * kXWaves;
+  long long l0 = (long long)blockIdx.x * kXWaves;
+  if (l0 < nlines) fetch(l0);
+  for (; l0 < nlines; l0 += stride) {
+    const bool active = pact;
+    if (active) {  // stage the line: 64 values per wave and step
+#pragma unroll
+      for (int u = 0; u < kXPer; ++u) {
+        const int k = sub * 64 + lane + u * wpl * 64;
+        if (k < len) {
+          X2[k] = pv[u] * kLog2e;
+          XF[k] = (float)pv[u];
+          if (SIGNED) SG[k] = ps[u] ? 1.f : -1.f;  // getSign (signArray.h:19-22)
+        }
+      }
+    }
+    if (sub == 0 && lane == 0) ACT[w] = active ? 1 : 0;
     __syncthreads();
+    if (l0 + stride < nlines) fetch(l0 + stride);
     // prefix maxima (wave 0 of the line) and suffix maxima (its last wave):
-    // inclusive max-scans across the wave, 64 positions per step, with a
-    // running carry; __shfl_up hands lanes below sh their own value
+    // inclusive max-scans across the wave (wave_max_scan), 64 positions per
+    // step, with a running carry from lane 63
     if (active && sub == 0) {
       float carry = -INFINITY;
       for (int base = 0; base < len; base += 64) {
         const int k = base + lane;
-        float v = k < len ? XF[k] : -INFINITY;
-        for (int sh = 1; sh < 64; sh <<= 1) v = fmaxf(v, __shfl_up(v, sh));
-        v = fmaxf(v, carry);
+        float v = fmaxf(wave_max_scan(k < len ? XF[k] : -INFINITY), carry);
         if (k < len) PM[k] = v;
-        carry = __shfl(v, 63);
+        carry = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
       }
     }
     if (active && sub == wpl - 1) {
       float carry = -INFINITY;
       for (int top = len - 1; top >= 0; top -= 64) {
         const int k = top - lane;
-        float v = k >= 0 ? XF[k] : -INFINITY;
-        for (int sh = 1; sh < 64; sh <<= 1) v = fmaxf(v, __shfl_up(v, sh));
-        v = fmaxf(v, carry);
+        float v = fmaxf(wave_max_scan(k >= 0 ? XF[k] : -INFINITY), carry);
         if (k >= 0) SM[k] = v;
-        carry = __shfl(v, 63);
+        carry = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
       }
     }
     __syncthreads();
@@ -277,12 +337,8 @@ __global__ __launch_bounds__(kXMaxThreads) void k_xlse_pass(
           dr = xlse_radius(SM, CF, jc, 1, len - 1 - jc, thr, igf);
           dl = xlse_radius(PM, CF, jc, -1, jc, thr, igf);
         }
-        for (int sh = 1; sh < 64; sh <<= 1) {
-          dl = max(dl, __shfl_xor(dl, sh));
-          dr = max(dr, __shfl_xor(dr, sh));
-        }
-        const int DL = __builtin_amdgcn_readfirstlane(dl);
-        const int DR = __builtin_amdgcn_readfirstlane(dr);
+        const int DL = __builtin_amdgcn_readlane(wave_max_scan(dl), 63);
+        const int DR = __builtin_amdgcn_readlane(wave_max_scan(dr), 63);
         const bool clamp = DL > kXPad || DR > kXPad;
         const float m = clamp ? xlse_window_max<true>(XF, CF, CFs, jc, len, DL, DR)
                               : xlse_window_max<false>(XF, CF, CFs, jc, len, DL, DR);
@@ -304,11 +360,10 @@ __global__ __launch_bounds__(kXMaxThreads) void k_xlse_pass(
     {
       const int ws = threadIdx.x % kXWaves;
       const long long ln = l0 + ws;
-      bool act = ln < nlines;
+      const bool act = ACT[ws] != 0;  // implies ln < nlines
       long long ob = 0;
       if (act) {
         const long long img = ln / rows;
-        act = active_thr <= 0 || Lp[p0 + (int)(img / ipp)] >= active_thr;
         ob = img * (long long)rows * len + (ln - img * rows);
       }
       if (act)
@@ -330,6 +385,9 @@ struct XStore {
   }
 };
 
+// Compute units of the device, for the pass's grid (set with the LDS attributes).
+static std::atomic<int> g_xlse_cus{0};
+
 // Solvers may be created from several host threads (the server's included);
 // a repeated attribute call is harmless, so a relaxed atomic suffices.
 template <bool SIGNED, class Epi>
@@ -350,8 +408,12 @@ static void xlse_launch(const double* in, const unsigned char* sin, Epi epi, int
                         long long nimg, double ig, const int* Lp, int p0, int ipp, int thr,
                         hipStream_t st) {
   const long long nl = nimg * rows;
-  const int grid = (int)std::min<long long>((nl + kXWaves - 1) / kXWaves, 1 << 20);
   const int lds = xlse_lds_bytes(len);
+  // twice the workgroups the CUs hold at once, each looping over line groups
+  const int perCU = std::max(1, std::min(160 * 1024 / lds, 8));
+  const int cus = g_xlse_cus.load(std::memory_order_relaxed);
+  const long long cap = 2LL * (cus > 0 ? cus : 256) * perCU;
+  const int grid = (int)std::min<long long>((nl + kXWaves - 1) / kXWaves, cap);
   const int block = kXWaves * 64 * xlse_waves_per_line(len);
   hipLaunchKernelGGL((k_xlse_pass<SIGNED, Epi>), dim3(grid), dim3(block), lds, st, in, sin, epi,
                      rows, len, nl, ig, Lp, p0, ipp, thr);
@@ -457,6 +519,12 @@ struct XEpiBwd2 {
 int xlse_set_lds_attr(int maxlen) {
   const int b = xlse_lds_bytes(maxlen);
   int e;
+  if (g_xlse_cus.load(std::memory_order_relaxed) == 0) {
+    int dev = 0, cus = 0;
+    WDL_HIP(hipGetDevice(&dev));
+    WDL_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    g_xlse_cus.store(std::max(cus, 1), std::memory_order_relaxed);
+  }
   if ((e = xlse_set_attr<true, XStore>(b)) || (e = xlse_set_attr<false, XStore>(b)) ||
       (e = xlse_set_attr<false, XEpiFwd1>(b)) || (e = xlse_set_attr<true, XEpiBwd1>(b)) ||
       (e = xlse_set_attr<true, XEpiBwd2>(b)))
