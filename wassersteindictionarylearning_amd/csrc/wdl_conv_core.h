@@ -419,6 +419,49 @@ __device__ inline void px_pass(const ConvGeom& G, Load&& load, Apply&& apply) {
   }
 }
 
+// Wave scans with DPP lane moves (VALU; a __shfl is one LDS-crossbar round
+// trip per step).  Row shifts by 1, 2, 4, 8 within each row of 16 lanes, then
+// the broadcast of lane 15 into the next row (half scan: lanes 31 and 63 hold
+// their half's result), then of lane 31 into the upper half (lane i holds the
+// result over lanes 0..i, lane 63 the wave's).  A lane whose source lies
+// outside its row, or whose row a broadcast leaves out, keeps its own value,
+// which max and min absorb.  Every lane of the wave must be active.
+template <int CTRL, int ROWS>
+__device__ inline int dpp_move(int v) {
+  return __builtin_amdgcn_update_dpp(v, v, CTRL, ROWS, 0xf, false);
+}
+template <int CTRL, int ROWS>
+__device__ inline float dpp_move(float v) {
+  return __int_as_float(dpp_move<CTRL, ROWS>(__float_as_int(v)));
+}
+template <class T, class Op>
+__device__ inline T half_scan(T x, Op op) {
+  x = op(x, dpp_move<0x111, 0xf>(x));  // row_shr:1
+  x = op(x, dpp_move<0x112, 0xf>(x));  // row_shr:2
+  x = op(x, dpp_move<0x114, 0xf>(x));  // row_shr:4
+  x = op(x, dpp_move<0x118, 0xf>(x));  // row_shr:8
+  return op(x, dpp_move<0x142, 0xa>(x));  // row_bcast:15
+}
+template <class T, class Op>
+__device__ inline T wave_scan(T x, Op op) {
+  const T h = half_scan(x, op);
+  return op(h, dpp_move<0x143, 0xc>(h));  // row_bcast:31 after the halves
+}
+struct MaxOp {
+  __device__ int operator()(int a, int b) const { return max(a, b); }
+  __device__ float operator()(float a, float b) const { return fmaxf(a, b); }
+};
+struct MinOp {
+  __device__ float operator()(float a, float b) const { return fminf(a, b); }
+};
+// The wave's maximum / minimum in every lane.
+__device__ inline float wave_max(float x) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wave_scan(x, MaxOp())), 63));
+}
+__device__ inline float wave_min(float x) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wave_scan(x, MinOp())), 63));
+}
+
 // safe_log (loss.h:3-5) in FP32: log(max(1e-250, x)); 1e-250 is below the
 // FP32 range, so every x <= 0 (and NaN, as std::max returns its first
 // argument) maps to log(1e-250).

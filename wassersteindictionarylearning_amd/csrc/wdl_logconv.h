@@ -155,7 +155,7 @@ __device__ inline void log_conv_lds(float* buf, const float* tabs, const LogLds&
   // row maxima (and finite minima for the range check)
   if constexpr (VW == 4) {
     // Two rows per wave step, 32 lanes each (a row has at most 32 float4
-    // groups), reduced with shuffles: no contended LDS atomics.  make only
+    // groups), reduced with DPP scans: no contended LDS atomics.  make only
     // reads LDS here.
     const int q = G.Wp / 4, half = lane >> 5, gcol = (lane & 31) * 4;
     for (int r0 = 2 * wave; r0 < G.Hp; r0 += 2 * kWaves) {
@@ -167,11 +167,9 @@ __device__ inline void log_conv_lds(float* buf, const float* tabs, const LogLds&
         mx = vmax_all(xf);
         mn = vmin_fin(xf);
       }
-      for (int sh = 1; sh < 32; sh <<= 1) {
-        mx = fmaxf(mx, __shfl_xor(mx, sh));
-        mn = fminf(mn, __shfl_xor(mn, sh));
-      }
-      if ((lane & 31) == 0 && row < G.H) {
+      mx = half_scan(mx, MaxOp());  // lanes 31 and 63 hold their half's
+      mn = half_scan(mn, MinOp());
+      if ((lane & 31) == 31 && row < G.H) {
         L.mrow[row] = fkey(mx);
         L.rmin[row] = fkey(mn);
       }
@@ -205,7 +203,7 @@ __device__ inline void log_conv_lds(float* buf, const float* tabs, const LogLds&
   store_transposed(buf, G.S2, G.Hp, G.Wp, acc);       // buf = S1^T (Wp lines of Hp)
   __syncthreads();
   // Column maxima (and finite minima) of T = M_r + log|S1|: one wave per
-  // line c (image column), reduced with shuffles (no LDS atomics).
+  // line c (image column), reduced with DPP scans (no LDS atomics).
   for (int c = wave; c < kMaxSide; c += kWaves) {
     float mx = -INFINITY, mn = INFINITY;
     if (c < G.W)
@@ -214,10 +212,8 @@ __device__ inline void log_conv_lds(float* buf, const float* tabs, const LogLds&
         mx = fmaxf(mx, t);
         mn = fminf(mn, vmin_fin(t));
       }
-    for (int sh = 1; sh < 64; sh <<= 1) {
-      mx = fmaxf(mx, __shfl_xor(mx, sh));
-      mn = fminf(mn, __shfl_xor(mn, sh));
-    }
+    mx = wave_max(mx);
+    mn = wave_min(mn);
     if (lane == 0) {
       L.mcol[c] = fkey(mx);
       L.rmin[c] = fkey(mn);

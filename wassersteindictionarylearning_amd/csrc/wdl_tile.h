@@ -198,7 +198,7 @@ __device__ inline void tile_log_conv(float* buf, const float* tabs, const LogLds
   __syncthreads();
   const int ncol = min(kTile, T.W - tx * kTile);  // inner columns inside the image
   // Column maxima (and finite minima) of T = M_r + log|S1|: one wave per
-  // inner column, reduced with shuffles (no LDS atomics, no index division).
+  // inner column, reduced with DPP scans (no LDS atomics, no index division).
   const int lane = threadIdx.x & 63, wave = wave_id();
   for (int c = wave; c < kTile; c += kWaves) {
     float mx = -INFINITY, mn = INFINITY;
@@ -211,10 +211,8 @@ __device__ inline void tile_log_conv(float* buf, const float* tabs, const LogLds
           mn = fminf(mn, vmin_fin(t));
         }
       }
-    for (int sh = 1; sh < 64; sh <<= 1) {
-      mx = fmaxf(mx, __shfl_xor(mx, sh));
-      mn = fminf(mn, __shfl_xor(mn, sh));
-    }
+    mx = wave_max(mx);
+    mn = wave_min(mn);
     if (lane == 0) {
       L.mcol[c] = fkey(mx);
       L.rmin[c] = fkey(mn);

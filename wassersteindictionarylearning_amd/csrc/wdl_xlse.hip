@@ -156,37 +156,6 @@ __device__ inline int xlse_radius(const float* B, const float* CF, int jc, int d
   return min(d - 1, lim);
 }
 
-// Inclusive max-scan over the 64 lanes of a wave with DPP lane moves (row
-// shifts by 1, 2, 4, 8 within each row of 16 lanes, then the broadcasts of
-// lanes 15 and 31 into the rows above): lane i ends with the maximum of lanes
-// 0..i, lane 63 with the wave's.  A lane whose source lies outside its row (or
-// whose row a broadcast leaves out) keeps its own value, which max absorbs.
-// VALU lane moves, where __shfl would be one LDS-crossbar round trip per step.
-__device__ inline int wave_max_scan(int x) {
-  x = max(x, __builtin_amdgcn_update_dpp(x, x, 0x111, 0xf, 0xf, false));  // row_shr:1
-  x = max(x, __builtin_amdgcn_update_dpp(x, x, 0x112, 0xf, 0xf, false));  // row_shr:2
-  x = max(x, __builtin_amdgcn_update_dpp(x, x, 0x114, 0xf, 0xf, false));  // row_shr:4
-  x = max(x, __builtin_amdgcn_update_dpp(x, x, 0x118, 0xf, 0xf, false));  // row_shr:8
-  x = max(x, __builtin_amdgcn_update_dpp(x, x, 0x142, 0xa, 0xf, false));  // row_bcast:15
-  x = max(x, __builtin_amdgcn_update_dpp(x, x, 0x143, 0xc, 0xf, false));  // row_bcast:31
-  return x;
-}
-// The same for floats (the order of FP32 values as fmaxf orders them, NaN-free).
-template <int CTRL, int ROWS>
-__device__ inline float dpp_move(float v) {
-  const int b = __float_as_int(v);
-  return __int_as_float(__builtin_amdgcn_update_dpp(b, b, CTRL, ROWS, 0xf, false));
-}
-__device__ inline float wave_max_scan(float x) {
-  x = fmaxf(x, dpp_move<0x111, 0xf>(x));
-  x = fmaxf(x, dpp_move<0x112, 0xf>(x));
-  x = fmaxf(x, dpp_move<0x114, 0xf>(x));
-  x = fmaxf(x, dpp_move<0x118, 0xf>(x));
-  x = fmaxf(x, dpp_move<0x142, 0xa>(x));
-  x = fmaxf(x, dpp_move<0x143, 0xc>(x));
-  return x;
-}
-
 // Waves per line of k_xlse_pass: long lines share their workgroup's LDS over
 // more waves (the LDS per line is fixed by len), for latency hiding.
 __host__ __device__ inline int xlse_waves_per_line(int len) {
@@ -302,13 +271,13 @@ __global__ __launch_bounds__(kXMaxThreads) void k_xlse_pass(
     __syncthreads();
     if (l0 + stride < nlines) fetch(l0 + stride);
     // prefix maxima (wave 0 of the line) and suffix maxima (its last wave):
-    // inclusive max-scans across the wave (wave_max_scan), 64 positions per
+    // inclusive max-scans across the wave (wave_scan), 64 positions per
     // step, with a running carry from lane 63
     if (active && sub == 0) {
       float carry = -INFINITY;
       for (int base = 0; base < len; base += 64) {
         const int k = base + lane;
-        float v = fmaxf(wave_max_scan(k < len ? XF[k] : -INFINITY), carry);
+        float v = fmaxf(wave_scan(k < len ? XF[k] : -INFINITY, MaxOp()), carry);
         if (k < len) PM[k] = v;
         carry = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
       }
@@ -317,7 +286,7 @@ __global__ __launch_bounds__(kXMaxThreads) void k_xlse_pass(
       float carry = -INFINITY;
       for (int top = len - 1; top >= 0; top -= 64) {
         const int k = top - lane;
-        float v = fmaxf(wave_max_scan(k >= 0 ? XF[k] : -INFINITY), carry);
+        float v = fmaxf(wave_scan(k >= 0 ? XF[k] : -INFINITY, MaxOp()), carry);
         if (k >= 0) SM[k] = v;
         carry = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
       }
@@ -337,8 +306,8 @@ __global__ __launch_bounds__(kXMaxThreads) void k_xlse_pass(
           dr = xlse_radius(SM, CF, jc, 1, len - 1 - jc, thr, igf);
           dl = xlse_radius(PM, CF, jc, -1, jc, thr, igf);
         }
-        const int DL = __builtin_amdgcn_readlane(wave_max_scan(dl), 63);
-        const int DR = __builtin_amdgcn_readlane(wave_max_scan(dr), 63);
+        const int DL = __builtin_amdgcn_readlane(wave_scan(dl, MaxOp()), 63);
+        const int DR = __builtin_amdgcn_readlane(wave_scan(dr, MaxOp()), 63);
         const bool clamp = DL > kXPad || DR > kXPad;
         const float m = clamp ? xlse_window_max<true>(XF, CF, CFs, jc, len, DL, DR)
                               : xlse_window_max<false>(XF, CF, CFs, jc, len, DL, DR);
