@@ -112,11 +112,16 @@ def projection_leg(ctx, requests=4096, chunk=512, one_launch=True):
     total_ms, iters, finite = 0.0, 0, 0
     fx = reference_fixture("large_projection")
     errs = {}
+    tuning = None
     for c0 in range(0, P, chunk):
         sl = slice(c0, c0 + chunk)
         s.set_observed(c["obs"][sl])
         s.set_problem(atoms, c["weights"][sl])
         s.set_iterations(c["iterations"][sl])
+        if tuning is None:  # the launch shape, tuned on the first chunk
+            s.autotune(1)
+            tuning = s.tuning()
+            log("projection tuning: %s" % tuning)
         ctx.synchronize()
         e0.record(stream)
         s.run()
@@ -135,6 +140,7 @@ def projection_leg(ctx, requests=4096, chunk=512, one_launch=True):
     one = None
     if one_launch:
         s = wdl.SinkhornSolver(ctx, n, n, K, P, L, g, c["loss"], flags | wdl.FLAG_CHECKPOINT)
+        s.set_tuning(tuning["streams"], 0, 0, tuning["line_waves"])
         s.set_observed(c["obs"][:P])
         s.set_problem(atoms, c["weights"][:P])
         s.set_iterations(c["iterations"][:P])
@@ -169,7 +175,7 @@ def projection_leg(ctx, requests=4096, chunk=512, one_launch=True):
                         "log domain with exact per-output shifts",
             "requests": P, "of": c["P"], "chunk": chunk, "sinkhorn_iters": iters, "finite_results": finite,
             "ms": round(total_ms, 3), "value": round(iters / (total_ms / 1e3), 1),
-            "unit": "Sinkhorn-iters/s", "one_launch": one,
+            "unit": "Sinkhorn-iters/s", "tuning": tuning, "one_launch": one,
             "err_vs_fp64": {"grad_lambda": max(errs.values()) if errs else None,
                             "requests_checked": sorted(errs),
                             "against": "reference compute_gradient_both_log_signArray (FP64, "
@@ -271,6 +277,8 @@ def log_leg(ctx, steps=2):
     s.set_observed(c["obs"])
     s.set_problem(c["atoms"], c["weights"])
     stream = ctx.stream
+    s.autotune(1)
+    tuning = s.tuning()
     s.run()  # warm-up and graph capture
     ctx.synchronize()
     e0, e1 = _hip.Event(), _hip.Event()
@@ -310,6 +318,7 @@ def log_leg(ctx, steps=2):
            "steps": steps, "ms_per_step": round(ms, 3),
            "value": round(P * L / (ms / 1e3), 1), "unit": "Sinkhorn-iters/s",
            "log_range_events": events, "hbm_footprint_gb": round(footprint / 1e9, 2),
+           "tuning": tuning,
            "err_vs_fp64": err}
     out.update(stage)
     return out
@@ -632,7 +641,7 @@ def main():
         nstreams = args.streams
     else:
         nstreams = s.autotune(3)
-        log("autotune: %d histogram groups (streams)" % nstreams)
+        log("autotune: %s" % s.tuning())
     stream = ctx.stream
 
     for _ in range(args.warmup):
@@ -748,7 +757,8 @@ def main():
         "config": {"workload": "batched standard domain: compute_gradient_both for all P "
                                "histograms per step", "n": n, "P": P, "K": K, "L": L,
                    "gamma": g, "loss": "quadratic", "band_skip": not args.dense,
-                   "streams": nstreams},
+                   "streams": nstreams, "tuning": s.tuning(),
+                   "tuned_by": "wdl_solver_autotune" if not args.streams else "--streams"},
         "latency_ms": {"p50": round(float(np.percentile(step_ms, 50)), 4),
                        "p99": round(float(np.percentile(step_ms, 99)), 4),
                        "what": "per-step (whole batch) latency"},

@@ -48,6 +48,8 @@ struct SolveArgs {
   float* LF0;  // standard domain: safe_log phi^s of the previous forward stage
   float* LF1;  // (ping-pong; the V buffers themselves without checkpoints)
   int weightsOnly;    // skip the dictionary-gradient chain
+  int fwdT, bwdT;     // threads per workgroup of the FP32 resident forward / backward stages
+  int xwpl;           // waves per line of the exact-shift passes (0: by line length)
   int bf3;            // split-BF16x3 convolution passes (standard domain)
   const float* logA;  // K x N log atoms (log-domain solver)
   unsigned* events;   // lines whose spread exceeded kLogRange (log-domain solver)
@@ -203,6 +205,16 @@ int exact_backward(const SolveArgs& a, hipStream_t st, int t);
 // WassersteinLoss at the barycenters: fills G0, lossPart, gsumPart (wdl_xlse.hip).
 int wasserstein_loss(const SolveArgs& a, hipStream_t st);
 constexpr int kXMaxLen = 1024;  // longest line of the exact-shift passes
+// Waves per line of k_xlse_pass: long lines share their workgroup's LDS over
+// more waves (the LDS per line is fixed by len), for latency hiding.  The
+// default by length, or a requested count of 1, 2 or 4 (a lane stages at most
+// 4 values of a line, so lines longer than 256 * wpl take more waves).
+__host__ __device__ inline int xlse_waves_per_line(int len, int want = 0) {
+  const int need = len <= 256 ? 1 : (len <= 512 ? 2 : 4);
+  if (want == 1 || want == 2 || want == 4) return want < need ? need : want;
+  return len <= 128 ? 1 : (len <= 512 ? 2 : 4);
+}
+
 // Pixels per workgroup of the exact-shift weight-gradient dot, and chunks per image.
 constexpr int kXDotChunk = 4096;
 __host__ __device__ inline long long xdot_chunks(long long N) {

@@ -32,19 +32,21 @@ namespace wdl {
 // Forward iteration s (1..L): b^{s-1} (geometric-mean step of iteration s-1,
 // :577-592), c^{s-1} = K b^{s-1} (:568), a = p / c (:569-574), phi^s = K a (:575).
 //
-// FP32 passes: sixteen waves per workgroup (two workgroups per CU).  The
-// forward launch has one workgroup per (histogram, atom), fewer than the CUs
-// hold, so each image is worked on by twice the waves of the other stages.
-// The split-BF16x3 passes need more registers and keep eight.
-template <bool BF3> struct FwdCfg {
-  static constexpr int threads = BF3 ? kThreads : 1024;
-  static constexpr int waves = threads / 64;
-  static constexpr int waves_per_simd = BF3 ? 4 : 8;
+// Workgroup shape of a stage: THREADS threads, two workgroups per CU (the
+// image's LDS allows no more).  FP32 passes: 768 or 1024 threads for the
+// forward (its launch has one workgroup per (histogram, atom), fewer than
+// the CUs hold, so each image can take more waves), 512 or 768 for the
+// backward, picked per solver by the tuner (default 1024 / 768); the
+// split-BF16x3 passes need more registers and keep 512.
+template <int THREADS> struct StageCfg {
+  static constexpr int threads = THREADS;
+  static constexpr int waves = THREADS / 64;
+  static constexpr int waves_per_simd = THREADS / 128;  // 2 workgroups over 4 SIMDs
 };
-template <int VW, bool BF3>
-__global__ __launch_bounds__(FwdCfg<BF3>::threads, FwdCfg<BF3>::waves_per_simd) void k_forward(
+template <int VW, bool BF3, int THREADS>
+__global__ __launch_bounds__(THREADS, StageCfg<THREADS>::waves_per_simd) void k_forward(
     SolveArgs a, int s) {
-  constexpr int kFwdThreads = FwdCfg<BF3>::threads, kFwdWaves = FwdCfg<BF3>::waves;
+  constexpr int kFwdThreads = THREADS, kFwdWaves = StageCfg<THREADS>::waves;
   typedef typename Vec<VW>::T V;
   int p, i;
   if (!map_block(a, p, i) || !fwd_runs(a, p, s)) return;
@@ -175,17 +177,12 @@ __global__ __launch_bounds__(kThreads) void k_final(SolveArgs a) {
 // Backward step s (L..1).  chain 0: dictionary gradient (:600-629);
 // chain 1: weight gradient (:635-673).
 //
-// FP32 passes: twelve waves per workgroup, two workgroups per CU (six waves
-// per SIMD within the register budget); the split-BF16x3 passes keep eight.
-template <bool BF3> struct BwdCfg {
-  static constexpr int threads = BF3 ? kThreads : 768;
-  static constexpr int waves = threads / 64;
-  static constexpr int waves_per_simd = BF3 ? 4 : 6;
-};
-template <int VW, bool BF3>
-__global__ __launch_bounds__(BwdCfg<BF3>::threads, BwdCfg<BF3>::waves_per_simd) void k_backward(
+// Workgroup shape as the forward's (StageCfg): 512 or 768 threads for the
+// FP32 passes, 512 for split-BF16x3.
+template <int VW, bool BF3, int THREADS>
+__global__ __launch_bounds__(THREADS, StageCfg<THREADS>::waves_per_simd) void k_backward(
     SolveArgs a, int t) {
-  constexpr int kBwdThreads = BwdCfg<BF3>::threads, kBwdWaves = BwdCfg<BF3>::waves;
+  constexpr int kBwdThreads = THREADS, kBwdWaves = StageCfg<THREADS>::waves;
   typedef typename Vec<VW>::T V;
   int p, i, chain;
   if (!map_block_bwd(a, p, i, chain)) return;
@@ -704,7 +701,17 @@ __global__ void k_warm_commit_active(T* __restrict__ store, const T* __restrict_
     store[o + e] = temp[o + e];
 }
 
-// Vector width 4 when rows hold whole float4 groups.
+// Vector width 4 when rows hold whole float4 groups; the workgroup shape of
+// the FP32 stages is a.fwdT / a.bwdT.
+template <int VW, bool BF3, int THREADS>
+static void fwd_launch(const SolveArgs& a, int grid, int lds, hipStream_t st, int s) {
+  hipLaunchKernelGGL((k_forward<VW, BF3, THREADS>), dim3(grid), dim3(THREADS), lds, st, a, s);
+}
+template <int VW, bool BF3, int THREADS>
+static void bwd_launch(const SolveArgs& a, int grid, int lds, hipStream_t st, int s) {
+  hipLaunchKernelGGL((k_backward<VW, BF3, THREADS>), dim3(grid), dim3(THREADS), lds, st, a, s);
+}
+
 static void launch_forward(const SolveArgs& a, int grid, int lds, hipStream_t st, int s) {
   if (a.logA) {
     if (a.G.W % 4 == 0)
@@ -715,18 +722,14 @@ static void launch_forward(const SolveArgs& a, int grid, int lds, hipStream_t st
   }
   const bool v4 = a.G.W % 4 == 0;
   if (a.bf3) {
-    if (v4)
-      hipLaunchKernelGGL((k_forward<4, true>), dim3(grid), dim3(FwdCfg<true>::threads), lds, st,
-                         a, s);
-    else
-      hipLaunchKernelGGL((k_forward<1, true>), dim3(grid), dim3(FwdCfg<true>::threads), lds, st,
-                         a, s);
-  } else if (v4) {
-    hipLaunchKernelGGL((k_forward<4, false>), dim3(grid), dim3(FwdCfg<false>::threads), lds, st,
-                       a, s);
+    if (v4) fwd_launch<4, true, kThreads>(a, grid, lds, st, s);
+    else fwd_launch<1, true, kThreads>(a, grid, lds, st, s);
+  } else if (a.fwdT == 768) {
+    if (v4) fwd_launch<4, false, 768>(a, grid, lds, st, s);
+    else fwd_launch<1, false, 768>(a, grid, lds, st, s);
   } else {
-    hipLaunchKernelGGL((k_forward<1, false>), dim3(grid), dim3(FwdCfg<false>::threads), lds, st,
-                       a, s);
+    if (v4) fwd_launch<4, false, 1024>(a, grid, lds, st, s);
+    else fwd_launch<1, false, 1024>(a, grid, lds, st, s);
   }
 }
 
@@ -741,18 +744,14 @@ static void launch_backward(const SolveArgs& a, int grid1, int lds, hipStream_t 
   }
   const bool v4 = a.G.W % 4 == 0;
   if (a.bf3) {
-    if (v4)
-      hipLaunchKernelGGL((k_backward<4, true>), dim3(blocks), dim3(BwdCfg<true>::threads), lds,
-                         st, a, s);
-    else
-      hipLaunchKernelGGL((k_backward<1, true>), dim3(blocks), dim3(BwdCfg<true>::threads), lds,
-                         st, a, s);
-  } else if (v4) {
-    hipLaunchKernelGGL((k_backward<4, false>), dim3(blocks), dim3(BwdCfg<false>::threads), lds,
-                         st, a, s);
+    if (v4) bwd_launch<4, true, kThreads>(a, blocks, lds, st, s);
+    else bwd_launch<1, true, kThreads>(a, blocks, lds, st, s);
+  } else if (a.bwdT == 512) {
+    if (v4) bwd_launch<4, false, 512>(a, blocks, lds, st, s);
+    else bwd_launch<1, false, 512>(a, blocks, lds, st, s);
   } else {
-    hipLaunchKernelGGL((k_backward<1, false>), dim3(blocks), dim3(BwdCfg<false>::threads), lds,
-                         st, a, s);
+    if (v4) bwd_launch<4, false, 768>(a, blocks, lds, st, s);
+    else bwd_launch<1, false, 768>(a, blocks, lds, st, s);
   }
 }
 
@@ -770,6 +769,7 @@ struct wdl_solver {
       bTemp, GDsum, logA, events, Lp, X0, X1, XS0, XS1, GLt, gsumPart, Mt, eta, xa, xt, xdp,
       nj, gj, sga, sgt, wla, wlb, wlc, wlt, evX, evG, evW, evA, evF, evP, CK, LF0, LF1;
   int seg = 0, ring = 0;  // checkpointed histories: segment length, history slots (0: L slots)
+  int fwdT = 1024, bwdT = 768, xwpl = 0;  // launch shape (wdl_solver_set_tuning)
   bool tiled = false;  // 64 x 64 tiles with a halo (a side above kMaxSide)
   bool exact = false;  // log domain with the exact-shift line passes (WDL_FLAG_EXACT_LSE)
   TileGeom tg{};
@@ -820,6 +820,9 @@ struct wdl_solver {
     a.Lp = Lp.as<int>();
     a.ring = ring;
     a.rmin = 0;
+    a.fwdT = fwdT;
+    a.bwdT = bwdT;
+    a.xwpl = xwpl;
     a.LF0 = ring && !(flags & WDL_FLAG_LOG_DOMAIN) ? LF0.as<float>() : V0.as<float>();
     a.LF1 = ring && !(flags & WDL_FLAG_LOG_DOMAIN) ? LF1.as<float>() : V1.as<float>();
     a.weightsOnly = (flags & WDL_FLAG_WEIGHTS_ONLY) ? 1 : 0;
@@ -864,11 +867,9 @@ struct wdl_solver {
   std::vector<hipEvent_t> joins;
   hipEvent_t fork = nullptr;
 
-  int set_groups(int n) {
-    n = std::max(1, std::min(n, P));
-    if (n == ngroups) return WDL_OK;
+  // The captured graph holds the launch shape; a new shape captures again.
+  int drop_graph() {
     WDL_HIP(hipStreamSynchronize(ctx->stream));
-    release_streams();
     if (exec) {
       (void)hipGraphExecDestroy(exec);
       exec = nullptr;
@@ -877,6 +878,15 @@ struct wdl_solver {
       (void)hipGraphDestroy(graph);
       graph = nullptr;
     }
+    return WDL_OK;
+  }
+
+  int set_groups(int n) {
+    n = std::max(1, std::min(n, P));
+    if (n == ngroups) return WDL_OK;
+    int st;
+    if ((st = drop_graph())) return st;
+    release_streams();
     ngroups = n;
     if (n == 1) return WDL_OK;
     WDL_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
@@ -1012,12 +1022,17 @@ static int set_lds_attr() {
   static std::atomic<bool> done{false};  // solvers are created from several threads
   if (done.load(std::memory_order_relaxed)) return WDL_OK;
   const int bytes = conv_lds_bytes(make_geom(kMaxSide, kMaxSide, 0));
-  const void* fns[] = {(const void*)k_forward<1, false>, (const void*)k_forward<4, false>,
-                       (const void*)k_backward<1, false>, (const void*)k_backward<4, false>};
+  const void* fns[] = {
+      (const void*)k_forward<1, false, 768>,   (const void*)k_forward<4, false, 768>,
+      (const void*)k_forward<1, false, 1024>,  (const void*)k_forward<4, false, 1024>,
+      (const void*)k_backward<1, false, 512>,  (const void*)k_backward<4, false, 512>,
+      (const void*)k_backward<1, false, 768>,  (const void*)k_backward<4, false, 768>};
   for (const void* f : fns)
     WDL_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
-  const void* bfns[] = {(const void*)k_forward<1, true>, (const void*)k_forward<4, true>,
-                        (const void*)k_backward<1, true>, (const void*)k_backward<4, true>};
+  const void* bfns[] = {(const void*)k_forward<1, true, kThreads>,
+                        (const void*)k_forward<4, true, kThreads>,
+                        (const void*)k_backward<1, true, kThreads>,
+                        (const void*)k_backward<4, true, kThreads>};
   for (const void* f : bfns)
     WDL_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 bytes + ConvPrec<true>::kTabBytes - ConvPrec<false>::kTabBytes));
@@ -1540,40 +1555,112 @@ int wdl_solver_get_grad_dic(wdl_solver* s, double* grad_dic) {
   return WDL_OK;
 }
 
+int wdl_solver_set_tuning(wdl_solver* s, int nstreams, int fwd_threads, int bwd_threads,
+                          int line_waves) {
+  WDL_CHECK(s, WDL_ERR_ARG, "wdl_solver_set_tuning: NULL argument");
+  WDL_CHECK(nstreams >= 0 && nstreams <= 16, WDL_ERR_ARG,
+            "wdl_solver_set_tuning: nstreams must be in 0..16");
+  WDL_CHECK(fwd_threads == 0 || fwd_threads == 768 || fwd_threads == 1024, WDL_ERR_ARG,
+            "wdl_solver_set_tuning: fwd_threads must be 0, 768 or 1024");
+  WDL_CHECK(bwd_threads == 0 || bwd_threads == 512 || bwd_threads == 768, WDL_ERR_ARG,
+            "wdl_solver_set_tuning: bwd_threads must be 0, 512 or 768");
+  WDL_CHECK(line_waves == 0 || line_waves == 1 || line_waves == 2 || line_waves == 4, WDL_ERR_ARG,
+            "wdl_solver_set_tuning: line_waves must be 0, 1, 2 or 4");
+  WDL_HIP(hipSetDevice(s->ctx->device));
+  int st;
+  if (nstreams && (st = s->set_groups(nstreams))) return st;
+  const int f = fwd_threads ? fwd_threads : s->fwdT, b = bwd_threads ? bwd_threads : s->bwdT;
+  const int w = line_waves ? xlse_waves_per_line(std::max(s->W, s->H), line_waves) : s->xwpl;
+  if (f != s->fwdT || b != s->bwdT || w != s->xwpl) {
+    if ((st = s->drop_graph())) return st;
+    s->fwdT = f;
+    s->bwdT = b;
+    s->xwpl = w;
+  }
+  return WDL_OK;
+}
+
+int wdl_solver_get_tuning(wdl_solver* s, int* nstreams, int* fwd_threads, int* bwd_threads,
+                          int* line_waves) {
+  WDL_CHECK(s, WDL_ERR_ARG, "wdl_solver_get_tuning: NULL argument");
+  if (nstreams) *nstreams = s->ngroups;
+  const bool fp32 = !(s->flags & (WDL_FLAG_LOG_DOMAIN | WDL_FLAG_BF16X3)) && !s->tiled;
+  if (fwd_threads) *fwd_threads = fp32 ? s->fwdT : 0;
+  if (bwd_threads) *bwd_threads = fp32 ? s->bwdT : 0;
+  if (line_waves) *line_waves = s->exact ? xlse_waves_per_line(std::max(s->W, s->H), s->xwpl) : 0;
+  return WDL_OK;
+}
+
+// Times `reps` batched runs of the solver as it stands (after one run that
+// captures the graph and warms up); -1 on error.
+static float time_runs(wdl_solver* s, int reps, hipEvent_t e0, hipEvent_t e1) {
+  if (wdl_solver_run(s, WDL_RUN_BATCHED)) return -1.f;
+  hipStream_t st = s->ctx->stream;
+  if (hipEventRecord(e0, st) != hipSuccess) return -1.f;
+  for (int r = 0; r < reps; ++r)
+    if (wdl_solver_run(s, WDL_RUN_BATCHED)) return -1.f;
+  float ms = 0.f;
+  if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+      hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+    return -1.f;
+  return ms;
+}
+
 int wdl_solver_autotune(wdl_solver* s, int reps, int* nstreams) {
   WDL_CHECK(s, WDL_ERR_ARG, "wdl_solver_autotune: NULL argument");
   WDL_CHECK(reps >= 1, WDL_ERR_ARG, "wdl_solver_autotune: reps must be >= 1");
   WDL_CHECK(s->observed && s->problem, WDL_ERR_STATE,
             "wdl_solver_autotune: set_observed and set_problem must be called first");
   WDL_HIP(hipSetDevice(s->ctx->device));
-  hipStream_t st = s->ctx->stream;
   hipEvent_t e0, e1;
   WDL_HIP(hipEventCreate(&e0));
   WDL_HIP(hipEventCreate(&e1));
-  int best = s->ngroups, rc = WDL_OK;
-  float bestMs = 0.f;
-  // candidate histogram-group counts: each group advances on its own stream
-  for (int n = 1; n <= 8 && n <= s->P && rc == WDL_OK; n *= 2) {
-    if ((rc = s->set_groups(n))) break;
-    if ((rc = wdl_solver_run(s, WDL_RUN_BATCHED))) break;  // graph capture and warm-up
-    float ms = 0.f;
-    if (hipEventRecord(e0, st) != hipSuccess) rc = WDL_ERR_HIP;
-    for (int r = 0; r < reps && rc == WDL_OK; ++r) rc = wdl_solver_run(s, WDL_RUN_BATCHED);
-    if (rc == WDL_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                         hipEventElapsedTime(&ms, e0, e1) != hipSuccess)) {
-      set_error("wdl_solver_autotune: event timing failed");
+  int rc = WDL_OK;
+  // Coordinate descent over the launch shape: the histogram-group count
+  // first, then the shape the solver form has (workgroup widths of the FP32
+  // resident stages, or waves per line of the exact-shift passes), each
+  // candidate timed over `reps` batched runs.
+  struct Shape { int groups, fwd, bwd, wpl; };
+  // the shape in use, waves per line resolved (0 would keep a trial's value)
+  const int len = std::max(s->W, s->H);
+  auto current = [&]() {
+    return Shape{s->ngroups, s->fwdT, s->bwdT, s->exact ? xlse_waves_per_line(len, s->xwpl) : 0};
+  };
+  Shape best = current();
+  float bestMs = -1.f;
+  auto trial = [&](const Shape& c) {
+    if (rc) return;
+    if ((rc = wdl_solver_set_tuning(s, c.groups, c.fwd, c.bwd, c.wpl))) return;
+    const float ms = time_runs(s, reps, e0, e1);
+    if (ms < 0.f) {
+      set_error("wdl_solver_autotune: a timed run failed");
       rc = WDL_ERR_HIP;
+      return;
     }
-    if (rc == WDL_OK && (bestMs == 0.f || ms < bestMs)) {
+    if (bestMs < 0.f || ms < bestMs) {
       bestMs = ms;
-      best = n;
+      best = current();
     }
+  };
+  for (int n = 1; n <= 8 && n <= s->P; n *= 2) trial(Shape{n, best.fwd, best.bwd, best.wpl});
+  const bool fp32 = !(s->flags & (WDL_FLAG_LOG_DOMAIN | WDL_FLAG_BF16X3)) && !s->tiled;
+  if (fp32) {
+    const int shapes[][2] = {{1024, 768}, {768, 768}, {1024, 512}, {768, 512}};
+    const Shape base = best;
+    for (const auto& sh : shapes)
+      if (sh[0] != base.fwd || sh[1] != base.bwd) trial(Shape{base.groups, sh[0], sh[1], base.wpl});
+  }
+  if (s->exact) {
+    const Shape base = best;
+    for (int w : {1, 2, 4})
+      if (xlse_waves_per_line(len, w) == w && w != base.wpl)
+        trial(Shape{base.groups, base.fwd, base.bwd, w});
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc) return rc;
-  if ((rc = s->set_groups(best))) return rc;
-  if (nstreams) *nstreams = best;
+  if ((rc = wdl_solver_set_tuning(s, best.groups, best.fwd, best.bwd, best.wpl))) return rc;
+  if (nstreams) *nstreams = best.groups;
   return WDL_OK;
 }
 

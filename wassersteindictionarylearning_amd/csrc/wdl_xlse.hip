@@ -156,11 +156,6 @@ __device__ inline int xlse_radius(const float* B, const float* CF, int jc, int d
   return min(d - 1, lim);
 }
 
-// Waves per line of k_xlse_pass: long lines share their workgroup's LDS over
-// more waves (the LDS per line is fixed by len), for latency hiding.
-__host__ __device__ inline int xlse_waves_per_line(int len) {
-  return len <= 128 ? 1 : (len <= 512 ? 2 : 4);
-}
 constexpr int kXMaxThreads = kXWaves * 4 * 64;
 
 // One pass over `nlines` lines of length `len` (line-major input), writing the
@@ -190,7 +185,7 @@ __global__ __launch_bounds__(kXMaxThreads) void k_xlse_pass(
     int active_thr) {
   extern __shared__ __align__(16) double xs[];
   const int lp = len + 2 * kXPad;
-  const int wpl = xlse_waves_per_line(len);
+  const int wpl = blockDim.x / (kXWaves * 64);  // waves per line
   const int wave = wave_id(), lane = threadIdx.x & 63;
   const int w = wave % kXWaves, sub = wave / kXWaves;  // line slot, wave within the line
   const float igf = (float)inv_gamma;
@@ -375,7 +370,7 @@ static int xlse_set_attr(int lds) {
 template <bool SIGNED, class Epi>
 static void xlse_launch(const double* in, const unsigned char* sin, Epi epi, int rows, int len,
                         long long nimg, double ig, const int* Lp, int p0, int ipp, int thr,
-                        hipStream_t st) {
+                        int wpl, hipStream_t st) {
   const long long nl = nimg * rows;
   const int lds = xlse_lds_bytes(len);
   // twice the workgroups the CUs hold at once, each looping over line groups
@@ -383,7 +378,7 @@ static void xlse_launch(const double* in, const unsigned char* sin, Epi epi, int
   const int cus = g_xlse_cus.load(std::memory_order_relaxed);
   const long long cap = 2LL * (cus > 0 ? cus : 256) * perCU;
   const int grid = (int)std::min<long long>((nl + kXWaves - 1) / kXWaves, cap);
-  const int block = kXWaves * 64 * xlse_waves_per_line(len);
+  const int block = kXWaves * 64 * xlse_waves_per_line(len, wpl);
   hipLaunchKernelGGL((k_xlse_pass<SIGNED, Epi>), dim3(grid), dim3(block), lds, st, in, sin, epi,
                      rows, len, nl, ig, Lp, p0, ipp, thr);
 }
@@ -394,15 +389,16 @@ static void xlse_launch(const double* in, const unsigned char* sin, Epi epi, int
 template <class Epi>
 static int xlse_conv_e(const double* in, const unsigned char* sin, Epi epi, double* tmp,
                        unsigned char* stmp, int W, int H, long long nimg, double gamma,
-                       const int* Lp, int p0, int ipp, int thr, hipStream_t st) {
+                       const int* Lp, int p0, int ipp, int thr, hipStream_t st, int wpl = 0) {
   const double ig = 1.0 / gamma;
   const int thr_ = Lp ? thr : 0;
   if (sin) {
-    xlse_launch<true>(in, sin, XStore{tmp, stmp}, H, W, nimg, ig, Lp, p0, ipp, thr_, st);  // rows
-    xlse_launch<true>(tmp, stmp, epi, W, H, nimg, ig, Lp, p0, ipp, thr_, st);  // columns
+    xlse_launch<true>(in, sin, XStore{tmp, stmp}, H, W, nimg, ig, Lp, p0, ipp, thr_, wpl, st);
+    xlse_launch<true>(tmp, stmp, epi, W, H, nimg, ig, Lp, p0, ipp, thr_, wpl, st);  // columns
   } else {
-    xlse_launch<false>(in, nullptr, XStore{tmp, nullptr}, H, W, nimg, ig, Lp, p0, ipp, thr_, st);
-    xlse_launch<false>(tmp, nullptr, epi, W, H, nimg, ig, Lp, p0, ipp, thr_, st);
+    xlse_launch<false>(in, nullptr, XStore{tmp, nullptr}, H, W, nimg, ig, Lp, p0, ipp, thr_, wpl,
+                       st);
+    xlse_launch<false>(tmp, nullptr, epi, W, H, nimg, ig, Lp, p0, ipp, thr_, wpl, st);
   }
   WDL_HIP(hipGetLastError());
   return WDL_OK;
@@ -410,9 +406,9 @@ static int xlse_conv_e(const double* in, const unsigned char* sin, Epi epi, doub
 
 static int xlse_conv(const double* in, const unsigned char* sin, double* out, unsigned char* sout,
                      double* tmp, unsigned char* stmp, int W, int H, long long nimg, double gamma,
-                     const int* Lp, int p0, int ipp, int thr, hipStream_t st) {
+                     const int* Lp, int p0, int ipp, int thr, hipStream_t st, int wpl = 0) {
   return xlse_conv_e(in, sin, XStore{out, sin ? sout : nullptr}, tmp, stmp, W, H, nimg, gamma, Lp,
-                     p0, ipp, thr, st);
+                     p0, ipp, thr, st, wpl);
 }
 
 // Solver output stages fused into the second pass (the image layout of the
@@ -695,10 +691,10 @@ int exact_forward(const SolveArgs& a, hipStream_t st, int s) {
                      a.N};
   const int thr = std::max(s, a.rmin);  // lines of histograms that run iteration s
   if ((e = xlse_conv_e(xa, nullptr, epi, xt, nullptr, W, H, nimg, g, a.Lp, a.p0,
-                       a.K, thr, st)))
+                       a.K, thr, st, a.xwpl)))
     return e;
   if ((e = xlse_conv(xa, nullptr, (double*)a.Fh + hslot(a, s - 1) * a.stage + off,
-                     nullptr, xt, nullptr, W, H, nimg, g, a.Lp, a.p0, a.K, thr, st)))
+                     nullptr, xt, nullptr, W, H, nimg, g, a.Lp, a.p0, a.K, thr, st, a.xwpl)))
     return e;
   WDL_HIP(hipGetLastError());
   return WDL_OK;
@@ -721,10 +717,11 @@ int exact_backward(const SolveArgs& a, hipStream_t st, int t) {
   hipLaunchKernelGGL(k_xdot_fin, ew_grid((long long)a.np * a.K), dim3(256), 0, st, a, t);
   // histogram p runs step t when L_p >= t (the second convolution for t >= 2)
   if ((e = xlse_conv_e(a.xa + off, a.sga + off, XEpiBwd1{a, t, nch, off}, a.xt + off, a.sgt + off,
-                       W, H, nimg, g, a.Lp, a.p0, nch * a.K, t, st)))
+                       W, H, nimg, g, a.Lp, a.p0, nch * a.K, t, st, a.xwpl)))
     return e;
   if (t >= 2 && (e = xlse_conv_e(a.xa + off, a.sga + off, XEpiBwd2{a, t, nch}, a.xt + off,
-                                 a.sgt + off, W, H, nimg, g, a.Lp, a.p0, nch * a.K, t, st)))
+                                 a.sgt + off, W, H, nimg, g, a.Lp, a.p0, nch * a.K, t, st,
+                                 a.xwpl)))
     return e;
   WDL_HIP(hipGetLastError());
   return WDL_OK;
