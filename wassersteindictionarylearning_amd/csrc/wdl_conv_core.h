@@ -488,6 +488,15 @@ __device__ inline floatx4 fdiv(floatx4 x, floatx4 y) {
 __device__ inline floatx4 fdiv(float x, floatx4 y) {
   return floatx4{fdiv(x, y.x), fdiv(x, y.y), fdiv(x, y.z), fdiv(x, y.w)};
 }
+// Natural log from the hardware log2 (v_log_f32, about 1 ulp) for values that
+// only pick a shift or a range key (the shifted passes' line maxima: the result
+// does not depend on the shift's exact value).  The instruction flushes
+// denormal inputs, so values below FLT_MIN are scaled by 2^64 first.
+__device__ inline float fast_logf(float x) {
+  const bool small = x < 1.17549435e-38f;
+  const float y = small ? x * 18446744073709551616.f : x;
+  return (__builtin_amdgcn_logf(y) - (small ? 64.f : 0.f)) * 0.693147180559945309f;
+}
 __device__ inline float vexpf(float x) { return expf(x); }
 __device__ inline floatx4 vexpf(floatx4 x) { return floatx4{expf(x.x), expf(x.y), expf(x.z), expf(x.w)}; }
 __device__ inline float vlogf(float x) { return logf(x); }

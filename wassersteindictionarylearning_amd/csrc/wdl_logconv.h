@@ -58,6 +58,15 @@ __device__ inline double vhlog(float x) { return hlog(x); }
 __device__ inline doublex4 vhlog(floatx4 x) {
   return doublex4{hlog(x.x), hlog(x.y), hlog(x.z), hlog(x.w)};
 }
+// hlog for a make's first evaluation (line maxima only: fast_logf), the
+// accurate one for its second (the values that are convolved).
+__device__ inline double vhlog_sel(float x, bool second) {
+  return second ? hlog(x) : (double)fast_logf(x);
+}
+__device__ inline doublex4 vhlog_sel(floatx4 x, bool second) {
+  if (second) return vhlog(x);
+  return doublex4{fast_logf(x.x), fast_logf(x.y), fast_logf(x.z), fast_logf(x.w)};
+}
 
 template <int VW>
 __device__ inline typename VecD<VW>::T vldd(const double* p) {
@@ -208,7 +217,7 @@ __device__ inline void log_conv_lds(float* buf, const float* tabs, const LogLds&
     float mx = -INFINITY, mn = INFINITY;
     if (c < G.W)
       for (int r = lane; r < G.H; r += 64) {
-        const float t = funkey(L.mrow[r]) + logf(fabsf(buf[c * G.S2 + r]));
+        const float t = funkey(L.mrow[r]) + fast_logf(fabsf(buf[c * G.S2 + r]));
         mx = fmaxf(mx, t);
         mn = fminf(mn, vmin_fin(t));
       }

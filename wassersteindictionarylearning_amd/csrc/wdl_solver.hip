@@ -574,9 +574,9 @@ __global__ __launch_bounds__(kThreads, 4) void k_backward_log(SolveArgs a, int t
           }
           return r;
         },
-        [&](int, int, int, const Three& r, bool) {
+        [&](int, int, int, const Three& r, bool second) {
           const V x = lpi * r.x + r.y;
-          return SLog<VW>{vhlog(vabs(x)) + r.z, vsign(x)};
+          return SLog<VW>{vhlog_sel(vabs(x), second) + r.z, vsign(x)};
         });
     float* GDp = a.GD + img;
     px_pass_r<VW>(
@@ -635,7 +635,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_backward_log(SolveArgs a, int t
     auto wmake = [&](int, int, int, const Three& r, bool second) {
       if (second) acc_dot(r);
       const V x = lpi * r.x + r.y;
-      return SLog<VW>{vhlog(vabs(x)) - r.z, vsign(x)};
+      return SLog<VW>{vhlog_sel(vabs(x), second) - r.z, vsign(x)};
     };
     if (s == 1) {
       px_pass_r<VW>(G, wload, [&](int, int, int, bool ok, const Three& r) {

@@ -595,9 +595,9 @@ __global__ __launch_bounds__(kThreads, 6) void k_tbwd1_log(SolveArgs a, int t) {
           }
           return r;
         },
-        [&](int, int, const Three& r, bool) {
+        [&](int, int, const Three& r, bool second) {
           const V x = lpi * r.x + r.y;
-          return SLog<VW>{vhlog(vabs(x)) + r.z, vsign(x)};
+          return SLog<VW>{vhlog_sel(vabs(x), second) + r.z, vsign(x)};
         });
     float* GDp = a.GD + img;
     tile_inner_pass<VW>(
@@ -647,7 +647,7 @@ __global__ __launch_bounds__(kThreads, 6) void k_tbwd1_log(SolveArgs a, int t) {
                         [&](int li, int, const Three& r, bool second) {
                           if (second && tile_inner(T, li)) acc_dot(r);
                           const V x = lpi * r.x + r.y;
-                          return SLog<VW>{vhlog(vabs(x)) - r.z, vsign(x)};
+                          return SLog<VW>{vhlog_sel(vabs(x), second) - r.z, vsign(x)};
                         });
     }
     double tot = block_sum(dot);

@@ -206,7 +206,7 @@ __device__ inline void tile_log_conv(float* buf, const float* tabs, const LogLds
       for (int r = lane; r < T.R; r += 64) {
         const float m = funkey(L.mrow[r]);
         if (m > -INFINITY) {
-          const float t = m + logf(fabsf(buf[c * T.S + r]));
+          const float t = m + fast_logf(fabsf(buf[c * T.S + r]));
           mx = fmaxf(mx, t);
           mn = fminf(mn, vmin_fin(t));
         }
