@@ -140,7 +140,7 @@ def projection_leg(ctx, requests=4096, chunk=512, one_launch=True):
     one = None
     if one_launch:
         s = wdl.SinkhornSolver(ctx, n, n, K, P, L, g, c["loss"], flags | wdl.FLAG_CHECKPOINT)
-        s.set_tuning(tuning["streams"], 0, 0, tuning["line_waves"])
+        s.set_tuning(tuning["streams"], 0, 0, tuning["line_waves"], tuning["line_group"])
         s.set_observed(c["obs"][:P])
         s.set_problem(atoms, c["weights"][:P])
         s.set_iterations(c["iterations"][:P])

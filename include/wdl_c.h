@@ -160,18 +160,20 @@ int wdl_solver_set_streams(wdl_solver* s, int nstreams);
 /* The launch shape of batched runs: histogram groups (streams, 1..16), threads
  * per workgroup of the standard-domain FP32 resident forward stage (768 or 1024)
  * and backward stage (512 or 768), waves per line of the exact-shift passes (1,
- * 2 or 4; raised to what the line length needs).  0 keeps a value.  Results do
- * not depend on it.  No counterpart in the reference; set by the tuner below. */
+ * 2 or 4; raised to what the line length needs) and their lines per workgroup
+ * (4 or 8; 4 where 8 would not fit).  0 keeps a value.  Results do not depend
+ * on it.  No counterpart in the reference; set by the tuner below. */
 int wdl_solver_set_tuning(wdl_solver* s, int nstreams, int fwd_threads, int bwd_threads,
-                          int line_waves);
+                          int line_waves, int line_group);
 /* The launch shape in use; a field the solver form does not use reads 0.  Any
  * pointer may be NULL. */
 int wdl_solver_get_tuning(wdl_solver* s, int* nstreams, int* fwd_threads, int* bwd_threads,
-                          int* line_waves);
+                          int* line_waves, int* line_group);
 /* Tunes the launch shape by coordinate descent, timing `reps` batched runs per
  * candidate: the histogram-group count (1, 2, 4, 8, at most P), then the
  * workgroup widths of the FP32 resident stages (forward x backward: 1024/768,
- * 768/768, 1024/512, 768/512) or the exact-shift passes' waves per line; keeps
+ * 768/768, 1024/512, 768/512) or the exact-shift passes' waves per line and
+ * then lines per workgroup; keeps
  * the fastest and writes the group count to *nstreams (may be NULL; the rest
  * through wdl_solver_get_tuning).  Runs the solve, so results of earlier runs
  * are overwritten (the warm-restart store is not committed).  No counterpart in

@@ -59,16 +59,20 @@ def test_exact_line_waves_same_results(ctx, W):
     s = make(ctx, W, 2, 3, 5, f, 2)
     print("exact shifts %dx%d, waves per line" % (W, W), flush=True)
     ref = outputs(s, False)
-    seen = set()
+    seen, groups = set(), set()
     for w in (1, 2, 4):
-        s.set_tuning(0, 0, 0, w)
-        got = s.tuning()["line_waves"]
-        assert got >= w and got in (1, 2, 4)
-        seen.add(got)
-        r = outputs(s, False)
-        for k in ("loss", "grad_lambda", "bary"):
-            np.testing.assert_array_equal(r[k], ref[k], err_msg="%d %s" % (w, k))
+        for lg in (4, 8):
+            s.set_tuning(0, 0, 0, w, lg)
+            t = s.tuning()
+            assert t["line_waves"] >= w and t["line_waves"] in (1, 2, 4)
+            assert t["line_group"] in (4, 8) and (t["line_group"] == 4 or lg == 8)
+            seen.add(t["line_waves"])
+            groups.add(t["line_group"])
+            r = outputs(s, False)
+            for k in ("loss", "grad_lambda", "bary"):
+                np.testing.assert_array_equal(r[k], ref[k], err_msg="%d/%d %s" % (w, lg, k))
     assert len(seen) == (3 if W <= 256 else 2)
+    assert groups == {4, 8}
     s.close()
 
 

@@ -74,9 +74,9 @@ _SIGS = {
     "wdl_solver_run_stage": (_i, [_vp, _i, _i]),
     "wdl_solver_set_streams": (_i, [_vp, _i]),
     "wdl_solver_autotune": (_i, [_vp, _i, ctypes.POINTER(_i)]),
-    "wdl_solver_set_tuning": (_i, [_vp, _i, _i, _i, _i]),
+    "wdl_solver_set_tuning": (_i, [_vp, _i, _i, _i, _i, _i]),
     "wdl_solver_get_tuning": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i),
-                                   ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+                                   ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     "wdl_solver_set_iterations": (_i, [_vp, ctypes.POINTER(_i)]),
     "wdl_solver_get_results": (_i, [_vp, _d, _d, _d, _d]),
     "wdl_solver_get_grad_dic": (_i, [_vp, _d]),
@@ -370,16 +370,16 @@ class SinkhornSolver:
         _check(lib().wdl_solver_autotune(self.handle, reps, ctypes.byref(n)), "wdl_solver_autotune")
         return n.value
 
-    def set_tuning(self, nstreams=0, fwd_threads=0, bwd_threads=0, line_waves=0):
+    def set_tuning(self, nstreams=0, fwd_threads=0, bwd_threads=0, line_waves=0, line_group=0):
         _check(lib().wdl_solver_set_tuning(self.handle, nstreams, fwd_threads, bwd_threads,
-                                           line_waves), "wdl_solver_set_tuning")
+                                           line_waves, line_group), "wdl_solver_set_tuning")
 
     def tuning(self) -> dict:
-        v = [_i() for _ in range(4)]
+        v = [_i() for _ in range(5)]
         _check(lib().wdl_solver_get_tuning(self.handle, *[ctypes.byref(x) for x in v]),
                "wdl_solver_get_tuning")
         return dict(streams=v[0].value, fwd_threads=v[1].value, bwd_threads=v[2].value,
-                    line_waves=v[3].value)
+                    line_waves=v[3].value, line_group=v[4].value)
 
     def run_stage(self, stage: int, step: int):
         _check(lib().wdl_solver_run_stage(self.handle, stage, step), "wdl_solver_run_stage")

@@ -50,6 +50,7 @@ struct SolveArgs {
   int weightsOnly;    // skip the dictionary-gradient chain
   int fwdT, bwdT;     // threads per workgroup of the FP32 resident forward / backward stages
   int xwpl;           // waves per line of the exact-shift passes (0: by line length)
+  int xlw;            // lines per workgroup of the exact-shift passes (4 or 8)
   int bf3;            // split-BF16x3 convolution passes (standard domain)
   const float* logA;  // K x N log atoms (log-domain solver)
   unsigned* events;   // lines whose spread exceeded kLogRange (log-domain solver)
@@ -205,6 +206,29 @@ int exact_backward(const SolveArgs& a, hipStream_t st, int t);
 // WassersteinLoss at the barycenters: fills G0, lossPart, gsumPart (wdl_xlse.hip).
 int wasserstein_loss(const SolveArgs& a, hipStream_t st);
 constexpr int kXMaxLen = 1024;  // longest line of the exact-shift passes
+constexpr int kXWaves = 4;  // lines per workgroup by default (4 or 8; see xlse_lines)
+constexpr int kXPad = 32;         // -inf positions on both sides of a staged line
+
+// Dynamic LDS of k_xlse_pass: per line its values (FP64 scaled by log2 e, FP32),
+// input signs, prefix and suffix maxima, outputs and output signs; per
+// workgroup the penalty tables t^2 / gamma (FP32) and t^2 log2(e) / gamma (FP64)
+// for t = 0..len+2 (distances up to len - 1, and 2 for lines shorter than 2)
+// and, centred, for t = -kXPad..kXPad.
+constexpr int kXSym = 2 * kXPad + 1;
+__host__ __device__ inline int xlse_tab_len(int len) { return len + 3; }
+__host__ __device__ inline int xlse_lds_bytes(int len, int lw = kXWaves) {
+  const int lp = len + 2 * kXPad;
+  return lw * (lp * 8 + len * 8 + lp * 4 * 2 + len * 4 * 2 + len) +
+         (xlse_tab_len(len) + kXSym) * 12 + 64;
+}
+
+// Lines per workgroup of a pass: 8 when asked for and the workgroup fits
+// (at most 1024 threads, its LDS within a CU's), else 4.
+constexpr int kXLdsMax = 160 * 1024;
+inline int xlse_lines(int len, int wpl, int want) {
+  return (want == 8 && 8 * 64 * wpl <= 1024 && xlse_lds_bytes(len, 8) <= kXLdsMax) ? 8 : 4;
+}
+
 // Waves per line of k_xlse_pass: long lines share their workgroup's LDS over
 // more waves (the LDS per line is fixed by len), for latency hiding.  The
 // default by length, or a requested count of 1, 2 or 4 (a lane stages at most

@@ -769,7 +769,7 @@ struct wdl_solver {
       bTemp, GDsum, logA, events, Lp, X0, X1, XS0, XS1, GLt, gsumPart, Mt, eta, xa, xt, xdp,
       nj, gj, sga, sgt, wla, wlb, wlc, wlt, evX, evG, evW, evA, evF, evP, CK, LF0, LF1;
   int seg = 0, ring = 0;  // checkpointed histories: segment length, history slots (0: L slots)
-  int fwdT = 1024, bwdT = 768, xwpl = 0;  // launch shape (wdl_solver_set_tuning)
+  int fwdT = 1024, bwdT = 768, xwpl = 0, xlw = 4;  // launch shape (wdl_solver_set_tuning)
   bool tiled = false;  // 64 x 64 tiles with a halo (a side above kMaxSide)
   bool exact = false;  // log domain with the exact-shift line passes (WDL_FLAG_EXACT_LSE)
   TileGeom tg{};
@@ -823,6 +823,7 @@ struct wdl_solver {
     a.fwdT = fwdT;
     a.bwdT = bwdT;
     a.xwpl = xwpl;
+    a.xlw = xlw;
     a.LF0 = ring && !(flags & WDL_FLAG_LOG_DOMAIN) ? LF0.as<float>() : V0.as<float>();
     a.LF1 = ring && !(flags & WDL_FLAG_LOG_DOMAIN) ? LF1.as<float>() : V1.as<float>();
     a.weightsOnly = (flags & WDL_FLAG_WEIGHTS_ONLY) ? 1 : 0;
@@ -1556,7 +1557,7 @@ int wdl_solver_get_grad_dic(wdl_solver* s, double* grad_dic) {
 }
 
 int wdl_solver_set_tuning(wdl_solver* s, int nstreams, int fwd_threads, int bwd_threads,
-                          int line_waves) {
+                          int line_waves, int line_group) {
   WDL_CHECK(s, WDL_ERR_ARG, "wdl_solver_set_tuning: NULL argument");
   WDL_CHECK(nstreams >= 0 && nstreams <= 16, WDL_ERR_ARG,
             "wdl_solver_set_tuning: nstreams must be in 0..16");
@@ -1566,28 +1567,36 @@ int wdl_solver_set_tuning(wdl_solver* s, int nstreams, int fwd_threads, int bwd_
             "wdl_solver_set_tuning: bwd_threads must be 0, 512 or 768");
   WDL_CHECK(line_waves == 0 || line_waves == 1 || line_waves == 2 || line_waves == 4, WDL_ERR_ARG,
             "wdl_solver_set_tuning: line_waves must be 0, 1, 2 or 4");
+  WDL_CHECK(line_group == 0 || line_group == 4 || line_group == 8, WDL_ERR_ARG,
+            "wdl_solver_set_tuning: line_group must be 0, 4 or 8");
   WDL_HIP(hipSetDevice(s->ctx->device));
   int st;
   if (nstreams && (st = s->set_groups(nstreams))) return st;
   const int f = fwd_threads ? fwd_threads : s->fwdT, b = bwd_threads ? bwd_threads : s->bwdT;
-  const int w = line_waves ? xlse_waves_per_line(std::max(s->W, s->H), line_waves) : s->xwpl;
-  if (f != s->fwdT || b != s->bwdT || w != s->xwpl) {
+  const int len = std::max(s->W, s->H);
+  const int w = line_waves ? xlse_waves_per_line(len, line_waves) : s->xwpl;
+  const int g = line_group ? xlse_lines(len, xlse_waves_per_line(len, w), line_group) : s->xlw;
+  if (f != s->fwdT || b != s->bwdT || w != s->xwpl || g != s->xlw) {
     if ((st = s->drop_graph())) return st;
     s->fwdT = f;
     s->bwdT = b;
     s->xwpl = w;
+    s->xlw = g;
   }
   return WDL_OK;
 }
 
 int wdl_solver_get_tuning(wdl_solver* s, int* nstreams, int* fwd_threads, int* bwd_threads,
-                          int* line_waves) {
+                          int* line_waves, int* line_group) {
   WDL_CHECK(s, WDL_ERR_ARG, "wdl_solver_get_tuning: NULL argument");
   if (nstreams) *nstreams = s->ngroups;
   const bool fp32 = !(s->flags & (WDL_FLAG_LOG_DOMAIN | WDL_FLAG_BF16X3)) && !s->tiled;
   if (fwd_threads) *fwd_threads = fp32 ? s->fwdT : 0;
   if (bwd_threads) *bwd_threads = fp32 ? s->bwdT : 0;
-  if (line_waves) *line_waves = s->exact ? xlse_waves_per_line(std::max(s->W, s->H), s->xwpl) : 0;
+  const int len = std::max(s->W, s->H);
+  if (line_waves) *line_waves = s->exact ? xlse_waves_per_line(len, s->xwpl) : 0;
+  if (line_group)
+    *line_group = s->exact ? xlse_lines(len, xlse_waves_per_line(len, s->xwpl), s->xlw) : 0;
   return WDL_OK;
 }
 
@@ -1620,17 +1629,19 @@ int wdl_solver_autotune(wdl_solver* s, int reps, int* nstreams) {
   // first, then the shape the solver form has (workgroup widths of the FP32
   // resident stages, or waves per line of the exact-shift passes), each
   // candidate timed over `reps` batched runs.
-  struct Shape { int groups, fwd, bwd, wpl; };
+  struct Shape { int groups, fwd, bwd, wpl, lw; };
   // the shape in use, waves per line resolved (0 would keep a trial's value)
   const int len = std::max(s->W, s->H);
   auto current = [&]() {
-    return Shape{s->ngroups, s->fwdT, s->bwdT, s->exact ? xlse_waves_per_line(len, s->xwpl) : 0};
+    const int w = xlse_waves_per_line(len, s->xwpl);
+    return Shape{s->ngroups, s->fwdT, s->bwdT, s->exact ? w : 0,
+                 s->exact ? xlse_lines(len, w, s->xlw) : 0};
   };
   Shape best = current();
   float bestMs = -1.f;
   auto trial = [&](const Shape& c) {
     if (rc) return;
-    if ((rc = wdl_solver_set_tuning(s, c.groups, c.fwd, c.bwd, c.wpl))) return;
+    if ((rc = wdl_solver_set_tuning(s, c.groups, c.fwd, c.bwd, c.wpl, c.lw))) return;
     const float ms = time_runs(s, reps, e0, e1);
     if (ms < 0.f) {
       set_error("wdl_solver_autotune: a timed run failed");
@@ -1642,24 +1653,29 @@ int wdl_solver_autotune(wdl_solver* s, int reps, int* nstreams) {
       best = current();
     }
   };
-  for (int n = 1; n <= 8 && n <= s->P; n *= 2) trial(Shape{n, best.fwd, best.bwd, best.wpl});
+  for (int n = 1; n <= 8 && n <= s->P; n *= 2) trial(Shape{n, best.fwd, best.bwd, best.wpl, best.lw});
   const bool fp32 = !(s->flags & (WDL_FLAG_LOG_DOMAIN | WDL_FLAG_BF16X3)) && !s->tiled;
   if (fp32) {
     const int shapes[][2] = {{1024, 768}, {768, 768}, {1024, 512}, {768, 512}};
     const Shape base = best;
     for (const auto& sh : shapes)
-      if (sh[0] != base.fwd || sh[1] != base.bwd) trial(Shape{base.groups, sh[0], sh[1], base.wpl});
+      if (sh[0] != base.fwd || sh[1] != base.bwd)
+        trial(Shape{base.groups, sh[0], sh[1], base.wpl, base.lw});
   }
   if (s->exact) {
     const Shape base = best;
     for (int w : {1, 2, 4})
       if (xlse_waves_per_line(len, w) == w && w != base.wpl)
-        trial(Shape{base.groups, base.fwd, base.bwd, w});
+        trial(Shape{base.groups, base.fwd, base.bwd, w, base.lw});
+    const Shape b2 = best;  // then 4 or 8 lines per workgroup at the best waves per line
+    for (int g : {4, 8})
+      if (xlse_lines(len, b2.wpl, g) == g && g != b2.lw)
+        trial(Shape{b2.groups, b2.fwd, b2.bwd, b2.wpl, g});
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc) return rc;
-  if ((rc = wdl_solver_set_tuning(s, best.groups, best.fwd, best.bwd, best.wpl))) return rc;
+  if ((rc = wdl_solver_set_tuning(s, best.groups, best.fwd, best.bwd, best.wpl, best.lw))) return rc;
   if (nstreams) *nstreams = best.groups;
   return WDL_OK;
 }

@@ -27,25 +27,11 @@
 
 namespace wdl {
 
-constexpr int kXWaves = 4;  // lines per workgroup, one wave each
 // Terms outside a window lie this far below m_j: each below e^-25 = 1.4e-11
 // relative, at most len * 1.4e-11 <= 1.4e-8 of the sum for lines up to 1024,
 // below the FP32 rounding of the sum itself.
 constexpr float kXMargin = 25.f;
-constexpr int kXPad = 32;         // -inf positions on both sides of a staged line
 
-// Dynamic LDS of k_xlse_pass: per line its values (FP64 scaled by log2 e, FP32),
-// input signs, prefix and suffix maxima, outputs and output signs; per
-// workgroup the penalty tables t^2 / gamma (FP32) and t^2 log2(e) / gamma (FP64)
-// for t = 0..len+2 (distances up to len - 1, and 2 for lines shorter than 2)
-// and, centred, for t = -kXPad..kXPad.
-constexpr int kXSym = 2 * kXPad + 1;
-__host__ __device__ inline int xlse_tab_len(int len) { return len + 3; }
-__host__ __device__ inline int xlse_lds_bytes(int len) {
-  const int lp = len + 2 * kXPad;
-  return kXWaves * (lp * 8 + len * 8 + lp * 4 * 2 + len * 4 * 2 + len) +
-         (xlse_tab_len(len) + kXSym) * 12 + 64;
-}
 
 // Maximum of XF_k - t^2/gamma over k = jc + t, t = -DL..DR.  Without CLAMP
 // every k lies in the padded line and |t| <= kXPad, and the penalty comes from
@@ -156,14 +142,14 @@ __device__ inline int xlse_radius(const float* B, const float* CF, int jc, int d
   return min(d - 1, lim);
 }
 
-constexpr int kXMaxThreads = kXWaves * 4 * 64;
+constexpr int kXMaxThreads = 1024;
 
 // One pass over `nlines` lines of length `len` (line-major input), writing the
 // transposed layout (line index fastest) of each image of `rows` lines.  A line
 // is skipped (output untouched) when active_thr > 0 and Lp[p0 + image / ipp] <
 // active_thr.
 //
-// kXWaves lines per workgroup, xlse_waves_per_line(len) waves each; one output
+// LW lines per workgroup, xlse_waves_per_line(len) waves each; one output
 // j per lane, 64 consecutive outputs at a time.  Every term of output j is
 // x_k - (j-k)^2/gamma <= PM[k] - (j-k)^2/gamma for k <= j (PM: prefix maxima)
 // and <= SM[k] - (k-j)^2/gamma for k >= j (SM: suffix maxima), and both bounds
@@ -178,33 +164,33 @@ constexpr int kXMaxThreads = kXWaves * 4 * 64;
 // (any shift near the maximum serves: the output adds back the shift the terms
 // were formed with); exponents are formed in FP64 in log2 units and
 // exponentiated on the hardware exp2.
-template <bool SIGNED, class Epi>
+template <bool SIGNED, class Epi, int LW>
 __global__ __launch_bounds__(kXMaxThreads) void k_xlse_pass(
     const double* __restrict__ in, const unsigned char* __restrict__ sin, Epi epi, int rows,
     int len, long long nlines, double inv_gamma, const int* __restrict__ Lp, int p0, int ipp,
     int active_thr) {
   extern __shared__ __align__(16) double xs[];
   const int lp = len + 2 * kXPad;
-  const int wpl = blockDim.x / (kXWaves * 64);  // waves per line
+  const int wpl = blockDim.x / (LW * 64);  // waves per line
   const int wave = wave_id(), lane = threadIdx.x & 63;
-  const int w = wave % kXWaves, sub = wave / kXWaves;  // line slot, wave within the line
+  const int w = wave % LW, sub = wave / LW;  // line slot, wave within the line
   const float igf = (float)inv_gamma;
   const double kLog2e = 1.4426950408889634;
-  // layout: X2 [kXWaves][lp], O0 [kXWaves][len], C2 [nt], C2s [kXSym] (doubles);
-  // XF, SG [kXWaves][lp], PM, SM [kXWaves][len], CF [nt], CFs [kXSym] (floats);
-  // SO0 [kXWaves][len]
+  // layout: X2 [LW][lp], O0 [LW][len], C2 [nt], C2s [kXSym] (doubles);
+  // XF, SG [LW][lp], PM, SM [LW][len], CF [nt], CFs [kXSym] (floats);
+  // SO0 [LW][len]
   const int nt = xlse_tab_len(len);
   double* X2 = xs + w * lp + kXPad;
-  double* O0 = xs + kXWaves * lp;
+  double* O0 = xs + LW * lp;
   double* O = O0 + w * len;
-  double* C2 = O0 + kXWaves * len;
+  double* C2 = O0 + LW * len;
   double* C2s = C2 + nt + kXPad;  // C2s[t], t = -kXPad..kXPad
   float* F0 = reinterpret_cast<float*>(C2 + nt + kXSym);
   float* XF = F0 + w * lp + kXPad;
-  float* SG = F0 + kXWaves * lp + w * lp + kXPad;
-  float* PM = F0 + 2 * kXWaves * lp + w * len;
-  float* SM = F0 + 2 * kXWaves * lp + kXWaves * len + w * len;
-  float* CF = F0 + 2 * kXWaves * lp + 2 * kXWaves * len;
+  float* SG = F0 + LW * lp + w * lp + kXPad;
+  float* PM = F0 + 2 * LW * lp + w * len;
+  float* SM = F0 + 2 * LW * lp + LW * len + w * len;
+  float* CF = F0 + 2 * LW * lp + 2 * LW * len;
   float* CFs = CF + nt + kXPad;
   unsigned char* SO0 = reinterpret_cast<unsigned char*>(CF + nt + kXSym);
   unsigned char* SO = SO0 + w * len;
@@ -224,7 +210,7 @@ __global__ __launch_bounds__(kXMaxThreads) void k_xlse_pass(
     SG[-1 - lane] = SG[len + lane] = 0.f;
   }
   // Per line slot: whether its line runs (1) or is skipped, for the store.
-  unsigned char* ACT = SO0 + kXWaves * len;
+  unsigned char* ACT = SO0 + LW * len;
   // The next line group's values (at most kXPer per lane: lines up to 1024
   // over 4 waves), sign bytes and activity, loaded one group ahead so the
   // loads' latency passes under the current group's work.
@@ -246,8 +232,8 @@ __global__ __launch_bounds__(kXMaxThreads) void k_xlse_pass(
       }
     }
   };
-  const long long stride = (long long)gridDim.x * kXWaves;
-  long long l0 = (long long)blockIdx.x * kXWaves;
+  const long long stride = (long long)gridDim.x * LW;
+  long long l0 = (long long)blockIdx.x * LW;
   if (l0 < nlines) fetch(l0);
   for (; l0 < nlines; l0 += stride) {
     const bool active = pact;
@@ -318,11 +304,11 @@ __global__ __launch_bounds__(kXMaxThreads) void k_xlse_pass(
       }
     __syncthreads();
     // transposed store: the lines of one workgroup are consecutive, so each
-    // output position j gets up to kXWaves adjacent values.  Thread i serves
-    // line slot i % kXWaves throughout (blockDim is a multiple of kXWaves), so
+    // output position j gets up to LW adjacent values.  Thread i serves
+    // line slot i % LW throughout (blockDim is a multiple of LW), so
     // the line's offset and activity are worked out once per line group.
     {
-      const int ws = threadIdx.x % kXWaves;
+      const int ws = threadIdx.x % LW;
       const long long ln = l0 + ws;
       const bool act = ACT[ws] != 0;  // implies ln < nlines
       long long ob = 0;
@@ -331,7 +317,7 @@ __global__ __launch_bounds__(kXMaxThreads) void k_xlse_pass(
         ob = img * (long long)rows * len + (ln - img * rows);
       }
       if (act)
-        for (int j = threadIdx.x / kXWaves; j < len; j += blockDim.x / kXWaves)
+        for (int j = threadIdx.x / LW; j < len; j += blockDim.x / LW)
           epi(ob + (long long)j * rows, O0[ws * len + j], SIGNED ? SO0[ws * len + j] : 1);
     }
     __syncthreads();
@@ -354,11 +340,11 @@ static std::atomic<int> g_xlse_cus{0};
 
 // Solvers may be created from several host threads (the server's included);
 // a repeated attribute call is harmless, so a relaxed atomic suffices.
-template <bool SIGNED, class Epi>
-static int xlse_set_attr(int lds) {
+template <bool SIGNED, class Epi, int LW>
+static int xlse_set_attr_lw(int lds) {
   static std::atomic<int> done{0};
   if (lds <= done.load(std::memory_order_relaxed)) return WDL_OK;
-  WDL_HIP(hipFuncSetAttribute((const void*)k_xlse_pass<SIGNED, Epi>,
+  WDL_HIP(hipFuncSetAttribute((const void*)k_xlse_pass<SIGNED, Epi, LW>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, lds));
   int cur = done.load(std::memory_order_relaxed);
   while (cur < lds && !done.compare_exchange_weak(cur, lds, std::memory_order_relaxed)) {
@@ -366,21 +352,37 @@ static int xlse_set_attr(int lds) {
   return WDL_OK;
 }
 
+
+// maxlen: the longest line the passes will run (the LDS attribute to allow).
+template <bool SIGNED, class Epi>
+static int xlse_set_attr(int maxlen) {
+  int e;
+  if ((e = xlse_set_attr_lw<SIGNED, Epi, 4>(xlse_lds_bytes(maxlen, 4)))) return e;
+  const int l8 = xlse_lds_bytes(maxlen, 8);
+  return l8 <= kXLdsMax ? xlse_set_attr_lw<SIGNED, Epi, 8>(l8) : WDL_OK;
+}
+
 // One pass over nimg images of rows lines of length len (see k_xlse_pass).
 template <bool SIGNED, class Epi>
 static void xlse_launch(const double* in, const unsigned char* sin, Epi epi, int rows, int len,
                         long long nimg, double ig, const int* Lp, int p0, int ipp, int thr,
-                        int wpl, hipStream_t st) {
+                        int wpl, int lw, hipStream_t st) {
   const long long nl = nimg * rows;
-  const int lds = xlse_lds_bytes(len);
+  const int w = xlse_waves_per_line(len, wpl);
+  lw = xlse_lines(len, w, lw);
+  const int lds = xlse_lds_bytes(len, lw);
   // twice the workgroups the CUs hold at once, each looping over line groups
   const int perCU = std::max(1, std::min(160 * 1024 / lds, 8));
   const int cus = g_xlse_cus.load(std::memory_order_relaxed);
   const long long cap = 2LL * (cus > 0 ? cus : 256) * perCU;
-  const int grid = (int)std::min<long long>((nl + kXWaves - 1) / kXWaves, cap);
-  const int block = kXWaves * 64 * xlse_waves_per_line(len, wpl);
-  hipLaunchKernelGGL((k_xlse_pass<SIGNED, Epi>), dim3(grid), dim3(block), lds, st, in, sin, epi,
-                     rows, len, nl, ig, Lp, p0, ipp, thr);
+  const int grid = (int)std::min<long long>((nl + lw - 1) / lw, cap);
+  const int block = lw * 64 * w;
+  if (lw == 8)
+    hipLaunchKernelGGL((k_xlse_pass<SIGNED, Epi, 8>), dim3(grid), dim3(block), lds, st, in, sin,
+                       epi, rows, len, nl, ig, Lp, p0, ipp, thr);
+  else
+    hipLaunchKernelGGL((k_xlse_pass<SIGNED, Epi, 4>), dim3(grid), dim3(block), lds, st, in, sin,
+                       epi, rows, len, nl, ig, Lp, p0, ipp, thr);
 }
 
 // Two passes (along x, then along y) over nimg W x H images; tmp holds the
@@ -389,16 +391,17 @@ static void xlse_launch(const double* in, const unsigned char* sin, Epi epi, int
 template <class Epi>
 static int xlse_conv_e(const double* in, const unsigned char* sin, Epi epi, double* tmp,
                        unsigned char* stmp, int W, int H, long long nimg, double gamma,
-                       const int* Lp, int p0, int ipp, int thr, hipStream_t st, int wpl = 0) {
+                       const int* Lp, int p0, int ipp, int thr, hipStream_t st, int wpl = 0,
+                       int lw = kXWaves) {
   const double ig = 1.0 / gamma;
   const int thr_ = Lp ? thr : 0;
   if (sin) {
-    xlse_launch<true>(in, sin, XStore{tmp, stmp}, H, W, nimg, ig, Lp, p0, ipp, thr_, wpl, st);
-    xlse_launch<true>(tmp, stmp, epi, W, H, nimg, ig, Lp, p0, ipp, thr_, wpl, st);  // columns
+    xlse_launch<true>(in, sin, XStore{tmp, stmp}, H, W, nimg, ig, Lp, p0, ipp, thr_, wpl, lw, st);
+    xlse_launch<true>(tmp, stmp, epi, W, H, nimg, ig, Lp, p0, ipp, thr_, wpl, lw, st);  // columns
   } else {
     xlse_launch<false>(in, nullptr, XStore{tmp, nullptr}, H, W, nimg, ig, Lp, p0, ipp, thr_, wpl,
-                       st);
-    xlse_launch<false>(tmp, nullptr, epi, W, H, nimg, ig, Lp, p0, ipp, thr_, wpl, st);
+                       lw, st);
+    xlse_launch<false>(tmp, nullptr, epi, W, H, nimg, ig, Lp, p0, ipp, thr_, wpl, lw, st);
   }
   WDL_HIP(hipGetLastError());
   return WDL_OK;
@@ -406,9 +409,10 @@ static int xlse_conv_e(const double* in, const unsigned char* sin, Epi epi, doub
 
 static int xlse_conv(const double* in, const unsigned char* sin, double* out, unsigned char* sout,
                      double* tmp, unsigned char* stmp, int W, int H, long long nimg, double gamma,
-                     const int* Lp, int p0, int ipp, int thr, hipStream_t st, int wpl = 0) {
+                     const int* Lp, int p0, int ipp, int thr, hipStream_t st, int wpl = 0,
+                     int lw = kXWaves) {
   return xlse_conv_e(in, sin, XStore{out, sin ? sout : nullptr}, tmp, stmp, W, H, nimg, gamma, Lp,
-                     p0, ipp, thr, st, wpl);
+                     p0, ipp, thr, st, wpl, lw);
 }
 
 // Solver output stages fused into the second pass (the image layout of the
@@ -482,7 +486,6 @@ struct XEpiBwd2 {
 };
 
 int xlse_set_lds_attr(int maxlen) {
-  const int b = xlse_lds_bytes(maxlen);
   int e;
   if (g_xlse_cus.load(std::memory_order_relaxed) == 0) {
     int dev = 0, cus = 0;
@@ -490,9 +493,9 @@ int xlse_set_lds_attr(int maxlen) {
     WDL_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     g_xlse_cus.store(std::max(cus, 1), std::memory_order_relaxed);
   }
-  if ((e = xlse_set_attr<true, XStore>(b)) || (e = xlse_set_attr<false, XStore>(b)) ||
-      (e = xlse_set_attr<false, XEpiFwd1>(b)) || (e = xlse_set_attr<true, XEpiBwd1>(b)) ||
-      (e = xlse_set_attr<true, XEpiBwd2>(b)))
+  if ((e = xlse_set_attr<true, XStore>(maxlen)) || (e = xlse_set_attr<false, XStore>(maxlen)) ||
+      (e = xlse_set_attr<false, XEpiFwd1>(maxlen)) || (e = xlse_set_attr<true, XEpiBwd1>(maxlen)) ||
+      (e = xlse_set_attr<true, XEpiBwd2>(maxlen)))
     return e;
   return WDL_OK;
 }
@@ -691,10 +694,10 @@ int exact_forward(const SolveArgs& a, hipStream_t st, int s) {
                      a.N};
   const int thr = std::max(s, a.rmin);  // lines of histograms that run iteration s
   if ((e = xlse_conv_e(xa, nullptr, epi, xt, nullptr, W, H, nimg, g, a.Lp, a.p0,
-                       a.K, thr, st, a.xwpl)))
+                       a.K, thr, st, a.xwpl, a.xlw)))
     return e;
   if ((e = xlse_conv(xa, nullptr, (double*)a.Fh + hslot(a, s - 1) * a.stage + off,
-                     nullptr, xt, nullptr, W, H, nimg, g, a.Lp, a.p0, a.K, thr, st, a.xwpl)))
+                     nullptr, xt, nullptr, W, H, nimg, g, a.Lp, a.p0, a.K, thr, st, a.xwpl, a.xlw)))
     return e;
   WDL_HIP(hipGetLastError());
   return WDL_OK;
@@ -717,11 +720,11 @@ int exact_backward(const SolveArgs& a, hipStream_t st, int t) {
   hipLaunchKernelGGL(k_xdot_fin, ew_grid((long long)a.np * a.K), dim3(256), 0, st, a, t);
   // histogram p runs step t when L_p >= t (the second convolution for t >= 2)
   if ((e = xlse_conv_e(a.xa + off, a.sga + off, XEpiBwd1{a, t, nch, off}, a.xt + off, a.sgt + off,
-                       W, H, nimg, g, a.Lp, a.p0, nch * a.K, t, st, a.xwpl)))
+                       W, H, nimg, g, a.Lp, a.p0, nch * a.K, t, st, a.xwpl, a.xlw)))
     return e;
   if (t >= 2 && (e = xlse_conv_e(a.xa + off, a.sga + off, XEpiBwd2{a, t, nch}, a.xt + off,
                                  a.sgt + off, W, H, nimg, g, a.Lp, a.p0, nch * a.K, t, st,
-                                 a.xwpl)))
+                                 a.xwpl, a.xlw)))
     return e;
   WDL_HIP(hipGetLastError());
   return WDL_OK;
