@@ -616,6 +616,8 @@ def main():
     ap.add_argument("--no-line", action="store_true", help="skip the 1-D / 3-D kernel leg")
     ap.add_argument("--projection-requests", type=int, default=4096,
                     help="how many of the 4096 projection requests to run")
+    ap.add_argument("--no-one-launch", action="store_true",
+                    help="skip the projection leg's single checkpointed launch of all requests")
     ap.add_argument("--served-seconds", type=float, default=20.0,
                     help="arrival window of the served leg")
     ap.add_argument("--copy-calib", action="store_true",
@@ -773,7 +775,8 @@ def main():
         line["line_kernels"] = line_leg(ctx)
         log("1-D / 3-D kernels: %s" % json.dumps(line["line_kernels"]))
     if not args.no_projection and args.projection_requests > 0:
-        line["projection"] = projection_leg(ctx, args.projection_requests)
+        line["projection"] = projection_leg(ctx, args.projection_requests,
+                                            one_launch=not args.no_one_launch)
         log("projection: %s iters/s" % line["projection"]["value"])
     if not args.no_log:
         line["log_small_eps"] = log_leg(ctx)
