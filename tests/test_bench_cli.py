@@ -26,12 +26,16 @@ def test_traffic_summary_parses_and_matches_its_counters():
     assert src == os.path.join("profiles", "k_backward_batched_std_traffic.md")
     text = open(os.path.join(ROOT, src)).read()
 
-    def avg(counter):
-        m = re.search(r"\| k_backward<4, false> \| 640 \| %s \| \d+ \| ([0-9.]+) \|" % counter, text)
+    def avg(kernel, counter):
+        m = re.search(r"\| %s \| \d+ \| %s \| \d+ \| ([0-9.]+) \|" % (kernel, counter), text)
         return float(m.group(1))
 
-    # bytes = FETCH_SIZE x 2 + WRITE_SIZE x 1 (KiB), the copy calibration stated there
-    expect = (avg("FETCH_SIZE") * 2.0 + avg("WRITE_SIZE")) * 1024
+    kb, copy = r"k_backward<4, false, \d+>", "__amd_rocclr_copyBuffer"
+    # bytes = FETCH_SIZE x f + WRITE_SIZE x w (KiB), with f and w the factors
+    # that make the 256 MiB copy's counters read 262144 KiB each way
+    f = 262144.0 / avg(copy, "FETCH_SIZE")
+    w = 262144.0 / avg(copy, "WRITE_SIZE")
+    expect = (avg(kb, "FETCH_SIZE") * f + avg(kb, "WRITE_SIZE") * w) * 1024
     assert abs(traffic - expect) <= 1.0
     # at least the algorithmic bytes of the launch: (9 + 2/5) images x P K N x 4 B
     assert traffic >= (9 + 2 / 5) * 64 * 5 * 16384 * 4
