@@ -236,6 +236,36 @@ int wdl_solver_projection_round(wdl_solver* s);
 /* Histogram `slot`'s current weights (K) and the loss of its last run.  Synchronous. */
 int wdl_solver_get_slot(wdl_solver* s, int slot, double* lambdas, double* loss);
 
+/* ---- groups: solvers of uneven grid sizes in one launch per stage ----------
+ * No counterpart in the reference (its solves are one serial loop over
+ * histograms of one size, inverseWasserstein.h:1052-1113).  A group runs the
+ * batched solves of several solvers -- each with its own W x H, K, P, L, gamma,
+ * loss and per-histogram iteration counts -- as one launch per stage kernel:
+ * every launch is the concatenation of the members' grids, and each
+ * workgroup finds its member in a device-side block table.  Members must be
+ * exact-shift log-domain solvers (WDL_FLAG_LOG_DOMAIN | WDL_FLAG_EXACT_LSE) on
+ * the group's context, without WDL_FLAG_CHECKPOINT or the Wasserstein loss,
+ * and either all or none WDL_FLAG_WEIGHTS_ONLY.  Launch s of the forward runs
+ * the members (and histograms) with at least s iterations; the backward is
+ * start-aligned the same way.  Results land in each member, exactly as
+ * wdl_solver_run (WDL_RUN_BATCHED) of that member alone would leave them, and
+ * are read with the member's own getters.  Destroy a group before its
+ * members. */
+typedef struct wdl_group wdl_group;
+#define WDL_GROUP_LAUNCH_KINDS 12
+int wdl_group_create(wdl_ctx* ctx, wdl_solver* const* solvers, int n, wdl_group** out);
+int wdl_group_destroy(wdl_group* g);
+/* compute_gradient_both_log_signArray (inverseWasserstein.h:678-814), or its
+ * weight chain alone, for every histogram of every member.  Asynchronous on
+ * the context stream (a captured graph, replayed on later calls). */
+int wdl_group_run(wdl_group* g);
+/* wdl_solver_projection_round for every member at once (members need
+ * WDL_FLAG_WEIGHTS_ONLY | WDL_FLAG_WARM_RESTART).  Asynchronous. */
+int wdl_group_projection_round(wdl_group* g);
+/* The launch shape: workgroups of each of the WDL_GROUP_LAUNCH_KINDS launch
+ * kinds (grids may be NULL), the passes' waves per line and their LDS bytes. */
+int wdl_group_shape(const wdl_group* g, int* grids, int* waves_per_line, int* lds_bytes);
+
 /* ---- served projection loop ----------------------------------------------
  * No counterpart in the reference.  A scheduler thread serves requests that
  * project histograms onto registered dictionaries (weights fitting by

@@ -105,6 +105,11 @@ _SIGS = {
                                ctypes.POINTER(ctypes.c_longlong)]),
     "wdl_server_wait": (_i, [_vp, ctypes.c_longlong, ctypes.c_double, _d, _d, _d, _d]),
     "wdl_server_stats": (_i, [_vp, ctypes.POINTER(ctypes.c_longlong)]),
+    "wdl_group_create": (_i, [_vp, ctypes.POINTER(_vp), _i, ctypes.POINTER(_vp)]),
+    "wdl_group_destroy": (_i, [_vp]),
+    "wdl_group_run": (_i, [_vp]),
+    "wdl_group_projection_round": (_i, [_vp]),
+    "wdl_group_shape": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)]),
 }
 
 
@@ -247,6 +252,7 @@ class HipGaussian2DKernel:
                "wdl_gauss2d_create")
         self.handle, self.ctx, self.gamma, self.W, self.H = h, ctx, float(gamma), W, H
         self.N = W * H
+        self._groups = weakref.WeakSet()  # groups that hold this solver
         ctx._children.add(self)
 
     def kernel1d(self):
@@ -313,6 +319,7 @@ class SinkhornSolver:
         self.handle, self.ctx = h, ctx
         self.W, self.H, self.K, self.P, self.L = W, H, K, P, L
         self.N = W * H
+        self._groups = weakref.WeakSet()  # groups that hold this solver
         ctx._children.add(self)
 
     @staticmethod
@@ -445,7 +452,55 @@ class SinkhornSolver:
 
     def close(self):
         if self.handle:
+            for g in list(self._groups):
+                g.close()
             lib().wdl_solver_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        if not sys.is_finalizing():
+            try:
+                self.close()
+            except Exception:
+                pass
+
+
+GROUP_LAUNCH_KINDS = 12  # WDL_GROUP_LAUNCH_KINDS
+GROUP_KIND_NAMES = ("xf1", "fwd_x", "fwd_y", "final", "dot", "xb1", "dot_fin", "bwd_x", "bwd_y",
+                    "project", "warm_commit", "reduce_gd")
+
+
+class SolverGroup:
+    """Exact-shift log-domain solvers of uneven sizes run as one launch per
+    stage kernel (wdl_group).  Results are read from the member solvers."""
+
+    def __init__(self, ctx: Context, solvers):
+        arr = (_vp * len(solvers))(*[s.handle for s in solvers])
+        h = _vp()
+        _check(lib().wdl_group_create(ctx.handle, arr, len(solvers), ctypes.byref(h)),
+               "wdl_group_create")
+        self.handle, self.ctx, self.members = h, ctx, list(solvers)
+        for s in solvers:
+            s._groups.add(self)
+        ctx._children.add(self)
+
+    def run(self):
+        _check(lib().wdl_group_run(self.handle), "wdl_group_run")
+
+    def projection_round(self):
+        _check(lib().wdl_group_projection_round(self.handle), "wdl_group_projection_round")
+
+    def shape(self) -> dict:
+        grids = (_i * GROUP_LAUNCH_KINDS)()
+        w, lds = _i(), _i()
+        _check(lib().wdl_group_shape(self.handle, grids, ctypes.byref(w), ctypes.byref(lds)),
+               "wdl_group_shape")
+        return dict(grids=dict(zip(GROUP_KIND_NAMES, list(grids))), waves_per_line=w.value,
+                    lds_bytes=lds.value)
+
+    def close(self):
+        if self.handle:
+            lib().wdl_group_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -203,6 +203,80 @@ int tiled_reduce_gl(const SolveArgs& a, hipStream_t st);
 int xlse_set_lds_attr(int maxlen);
 int exact_forward(const SolveArgs& a, hipStream_t st, int s);
 int exact_backward(const SolveArgs& a, hipStream_t st, int t);
+
+// ---- groups: one launch per stage kernel over several solvers -----------------
+// The members of a group are exact-shift log-domain solvers of any grid size,
+// atom count, histogram count, iteration count and gamma.  Every stage kernel
+// of a group launch is the concatenation of the members' grids: for each kind
+// of launch, starts[kind][m] is member m's first workgroup (starts[kind][nm] the
+// total), and a workgroup runs the single-solver kernel's body for its member
+// with its block index inside that member's share.
+struct GroupMember {
+  SolveArgs a;          // the member's arguments over all its histograms (p0 = 0)
+  double ig;            // 1 / gamma, as the single-solver launches compute it (host)
+  const float* eta;     // mirror-descent steps per histogram (projection rounds)
+  double* GDsum;        // dL/datoms summed over the histograms (K x N)
+};
+
+enum GroupKind {
+  GK_XF1 = 0,   // forward: log b^{s-1}, the first convolution's input
+  GK_FA,        // forward passes along x (nimg = P K images of H lines)
+  GK_FB,        // forward passes along y (P K images of W lines)
+  GK_FINAL,     // k_final_log: kLossBlocks workgroups per histogram
+  GK_DOT,       // weight-gradient dot chunks
+  GK_XB1,       // both chains: the first backward convolution's input
+  GK_DOTFIN,    // the dots' in-order finish
+  GK_BA,        // backward passes along x (P nch K images)
+  GK_BB,        // backward passes along y
+  GK_PROJ,      // projection update (64 histograms per workgroup)
+  GK_WARM,      // per-histogram warm commit
+  GK_RGD,       // summed atom gradient
+  GK_COUNT
+};
+
+struct GroupArgs {
+  const GroupMember* m;  // device, nm entries
+  const int* starts;     // device, GK_COUNT x (nm + 1)
+  int nm;
+};
+
+// Member of workgroup b in a launch of `kind`, with its block index (lb) and
+// block count (nb) inside the member's share.  starts[kind] is non-decreasing
+// with starts[kind][nm] > b, so the search ends on a member with lb < nb.
+__device__ inline int group_member(const GroupArgs& g, int kind, int b, int& lb, int& nb) {
+  const int* st = g.starts + kind * (g.nm + 1);
+  int lo = 0, hi = g.nm;  // st[lo] <= b < st[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (st[mid] <= b)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  lb = b - st[lo];
+  nb = st[lo + 1] - st[lo];
+  return lo;
+}
+
+// Host side of a group launch: the grid of every kind, the passes' waves per
+// line (the most any member's lines need) and their LDS (the longest line's).
+struct GroupShape {
+  int grid[GK_COUNT];
+  int wpl;
+  int lds;
+  int maxlen;
+  int weightsOnly;
+  int Lmax;
+};
+
+// Per-member block counts of every kind (host): what the single-solver
+// launches use, with the passes' workgroups shared out over the members by
+// their work when the group holds more than the device runs at once.
+void group_blocks(const std::vector<SolveArgs>& ms, int lds, int cus,
+                  std::vector<int> (&counts)[GK_COUNT]);
+int xlse_group_set_attr(int lds);
+int exact_group_forward(const GroupArgs& g, const GroupShape& sh, hipStream_t st, int s);
+int exact_group_backward(const GroupArgs& g, const GroupShape& sh, hipStream_t st, int t);
 // WassersteinLoss at the barycenters: fills G0, lossPart, gsumPart (wdl_xlse.hip).
 int wasserstein_loss(const SolveArgs& a, hipStream_t st);
 constexpr int kXMaxLen = 1024;  // longest line of the exact-shift passes

@@ -475,16 +475,18 @@ __global__ __launch_bounds__(kThreads, 4) void k_forward_log(SolveArgs a, int s)
 
 // After iteration L: log q = sum_i lam_i log phi_i^L, b^L = log q - log phi^L,
 // q = exp(log q) (:711-728), loss and its gradient at q.
-__global__ __launch_bounds__(kThreads) void k_final_log(SolveArgs a) {
-  const int p = a.p0 + blockIdx.y;
+// bx / nbx: the workgroup's pixel chunk and the chunks per histogram; by: the
+// histogram (from p0).
+__device__ inline void final_log_body(const SolveArgs& a, int bx, int nbx, int by) {
+  const int p = a.p0 + by;
   const long long N = a.N;
   const int K = a.K;
   const long long hbase = (long long)p * K * N;
   const int Lp = steps_of(a, p);
   if (Lp == 0) {
     if (threadIdx.x == 0) {
-      a.lossPart[(long long)p * kLossBlocks + blockIdx.x] = 0.0;
-      a.gsumPart[(long long)p * kLossBlocks + blockIdx.x] = 0.0;
+      a.lossPart[(long long)p * kLossBlocks + bx] = 0.0;
+      a.gsumPart[(long long)p * kLossBlocks + bx] = 0.0;
     }
     return;
   }
@@ -492,8 +494,8 @@ __global__ __launch_bounds__(kThreads) void k_final_log(SolveArgs a) {
   double* B = (double*)a.Bh + hslot(a, Lp - 1) * a.stage + hbase;
   const float* lam = a.lam + (long long)p * K;
   double acc = 0, gs = 0;
-  for (long long x = (long long)blockIdx.x * kThreads + threadIdx.x; x < N;
-       x += (long long)gridDim.x * kThreads) {
+  for (long long x = (long long)bx * kThreads + threadIdx.x; x < N;
+       x += (long long)nbx * kThreads) {
     double lq = 0.0;
     for (int j = 0; j < K; ++j) lq += (double)lam[j] * F[j * N + x];
     for (int j = 0; j < K; ++j) {
@@ -512,9 +514,20 @@ __global__ __launch_bounds__(kThreads) void k_final_log(SolveArgs a) {
   }
   const double t = block_sum(acc), tg = block_sum(gs);
   if (threadIdx.x == 0 && a.loss != WDL_LOSS_WASSERSTEIN) {
-    a.lossPart[(long long)p * kLossBlocks + blockIdx.x] = t;
-    a.gsumPart[(long long)p * kLossBlocks + blockIdx.x] = tg;
+    a.lossPart[(long long)p * kLossBlocks + bx] = t;
+    a.gsumPart[(long long)p * kLossBlocks + bx] = tg;
   }
+}
+
+__global__ __launch_bounds__(kThreads) void k_final_log(SolveArgs a) {
+  final_log_body(a, blockIdx.x, gridDim.x, blockIdx.y);
+}
+
+// A group's final stage: kLossBlocks workgroups per histogram of each member.
+__global__ __launch_bounds__(kThreads) void k_final_log_group(GroupArgs g) {
+  int lb, nb;
+  const SolveArgs a = g.m[group_member(g, GK_FINAL, blockIdx.x, lb, nb)].a;
+  final_log_body(a, lb % kLossBlocks, kLossBlocks, lb / kLossBlocks);
 }
 
 template <int VW>
@@ -670,10 +683,9 @@ __global__ __launch_bounds__(kThreads, 4) void k_backward_log(SolveArgs a, int t
 //   lambda_i <- lambda_i exp(-eta g_i / max_j |g_j|) / Z,
 // the exponentiated-gradient update with a step normalised by the largest
 // component (scale-free across losses and grid sizes).
-__global__ void k_project_update(float* __restrict__ lam, const double* __restrict__ GL,
-                                 const int* __restrict__ Lp, const float* __restrict__ eta, int P,
-                                 int K) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void project_update_one(float* __restrict__ lam, const double* __restrict__ GL,
+                                          const int* __restrict__ Lp,
+                                          const float* __restrict__ eta, int p, int P, int K) {
   if (p >= P || Lp[p] <= 0) return;
   float* l = lam + (long long)p * K;
   const double* g = GL + (long long)p * K;
@@ -688,17 +700,64 @@ __global__ void k_project_update(float* __restrict__ lam, const double* __restri
   for (int i = 0; i < K; ++i) l[i] = (float)((double)l[i] * exp(-step * (g[i] - d)) / z);
 }
 
+__global__ void k_project_update(float* __restrict__ lam, const double* __restrict__ GL,
+                                 const int* __restrict__ Lp, const float* __restrict__ eta, int P,
+                                 int K) {
+  project_update_one(lam, GL, Lp, eta, blockIdx.x * blockDim.x + threadIdx.x, P, K);
+}
+
 // b_storage <- b_temp for the active histograms (the warm restart of :418-422,
-// per histogram).
+// per histogram); bx / nbx: the workgroup's share of histogram p's K N values.
+template <class T>
+__device__ inline void warm_commit_one(T* __restrict__ store, const T* __restrict__ temp,
+                                       const int* __restrict__ Lp, long long KN, int p, int bx,
+                                       int nbx) {
+  if (Lp[p] <= 0) return;
+  const long long o = (long long)p * KN;
+  for (long long e = (long long)bx * blockDim.x + threadIdx.x; e < KN;
+       e += (long long)nbx * blockDim.x)
+    store[o + e] = temp[o + e];
+}
+
 template <class T>
 __global__ void k_warm_commit_active(T* __restrict__ store, const T* __restrict__ temp,
                                      const int* __restrict__ Lp, long long KN) {
-  const int p = blockIdx.y;
-  if (Lp[p] <= 0) return;
-  const long long o = (long long)p * KN;
-  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < KN;
-       e += (long long)gridDim.x * blockDim.x)
-    store[o + e] = temp[o + e];
+  warm_commit_one(store, temp, Lp, KN, blockIdx.y, blockIdx.x, gridDim.x);
+}
+
+// A group's projection round after its solve: the update (64 histograms per
+// workgroup) and the warm commit (min(64, K N / 256) workgroups per histogram)
+// of every member; members are log-domain, so the store holds log b in FP64.
+__global__ void k_project_update_group(GroupArgs g) {
+  int lb, nb;
+  const GroupMember& gm = g.m[group_member(g, GK_PROJ, blockIdx.x, lb, nb)];
+  project_update_one(const_cast<float*>(gm.a.lam), gm.a.GL, gm.a.Lp, gm.eta,
+                     lb * blockDim.x + threadIdx.x, gm.a.P, gm.a.K);
+}
+
+__global__ void k_warm_commit_group(GroupArgs g) {
+  int lb, nb;
+  const SolveArgs a = g.m[group_member(g, GK_WARM, blockIdx.x, lb, nb)].a;
+  const long long KN = (long long)a.K * a.N;
+  const int per = nb / a.P;  // workgroups per histogram
+  warm_commit_one(const_cast<double*>(a.lbInit), (const double*)a.lbSave, a.Lp, KN, lb / per,
+                  lb % per, per);
+}
+
+// A group's summed atom gradients (full-gradient members), as k_reduce_gd.
+__global__ void k_reduce_gd_group(GroupArgs g) {
+  int lb, nb;
+  const GroupMember& gm = g.m[group_member(g, GK_RGD, blockIdx.x, lb, nb)];
+  const float* GD = gm.a.GD;
+  double* out = gm.GDsum;
+  const int P = gm.a.P;
+  const long long KN = (long long)gm.a.K * gm.a.N;
+  for (long long e = (long long)lb * blockDim.x + threadIdx.x; e < KN;
+       e += (long long)nb * blockDim.x) {
+    double acc = 0;
+    for (int p = 0; p < P; ++p) acc += GD[(long long)p * KN + e];
+    out[e] = acc;
+  }
 }
 
 // Vector width 4 when rows hold whole float4 groups; the workgroup shape of
@@ -1854,6 +1913,201 @@ double wdl_evaluate_both(void* instance, const double* x, double* g, int n, doub
     return nan;
   }
   return f;
+}
+
+}  // extern "C"
+
+// ---- groups: uneven grid sizes in one launch per stage kernel ----------------
+// The members' own buffers hold their problems and results; the group keeps
+// their arguments and the block tables in device memory and replays one
+// captured sequence: forward launches s = 1..Lmax, the final stage, backward
+// launches t = Lmax..1 (start-aligned in both directions, as within one solver,
+// so a member with fewer iterations sits out the launches past its own L),
+// then, for projection rounds, the update and the warm commit.
+struct wdl_group {
+  wdl_ctx* ctx = nullptr;
+  std::vector<wdl_solver*> ms;
+  DevBuf dev;  // GroupMember[nm], then int starts[GK_COUNT][nm + 1]
+  GroupArgs ga{};
+  GroupShape sh{};
+  bool projection = false;  // every member weights-only with a warm-restart store
+  hipGraph_t graph[2] = {nullptr, nullptr};  // [0] run, [1] projection round
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+
+  int enqueue(bool proj) {
+    hipStream_t st = ctx->stream;
+    int e;
+    for (int s = 1; s <= sh.Lmax; ++s)
+      if ((e = exact_group_forward(ga, sh, st, s))) return e;
+    hipLaunchKernelGGL(k_final_log_group, dim3(sh.grid[GK_FINAL]), dim3(kThreads), 0, st, ga);
+    for (int t = sh.Lmax; t >= 1; --t)
+      if ((e = exact_group_backward(ga, sh, st, t))) return e;
+    if (!sh.weightsOnly)
+      hipLaunchKernelGGL(k_reduce_gd_group, dim3(sh.grid[GK_RGD]), dim3(256), 0, st, ga);
+    if (proj) {
+      hipLaunchKernelGGL(k_project_update_group, dim3(sh.grid[GK_PROJ]), dim3(64), 0, st, ga);
+      hipLaunchKernelGGL(k_warm_commit_group, dim3(sh.grid[GK_WARM]), dim3(256), 0, st, ga);
+    }
+    WDL_HIP(hipGetLastError());
+    return WDL_OK;
+  }
+
+  int launch(bool proj) {
+    const int k = proj ? 1 : 0;
+    if (!exec[k]) {
+      WDL_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      const int st = enqueue(proj);
+      hipGraph_t gr = nullptr;
+      const hipError_t e = hipStreamEndCapture(ctx->stream, &gr);
+      if (st) {
+        if (gr) (void)hipGraphDestroy(gr);
+        return st;
+      }
+      if (e != hipSuccess) {
+        set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        return WDL_ERR_HIP;
+      }
+      graph[k] = gr;
+      WDL_HIP(hipGraphInstantiate(&exec[k], graph[k], nullptr, nullptr, 0));
+    }
+    WDL_HIP(hipGraphLaunch(exec[k], ctx->stream));
+    for (wdl_solver* m : ms) m->ran = true;
+    return WDL_OK;
+  }
+};
+
+extern "C" {
+
+int wdl_group_create(wdl_ctx* ctx, wdl_solver* const* solvers, int n, wdl_group** out) {
+  WDL_CHECK(ctx && solvers && out, WDL_ERR_ARG, "wdl_group_create: NULL argument");
+  WDL_CHECK(n >= 1 && n <= 4096, WDL_ERR_ARG, "wdl_group_create: n must be in 1..4096");
+  int wo = -1, maxlen = 0, wpl = 1, Lmax = 0;
+  bool proj = true;
+  for (int m = 0; m < n; ++m) {
+    const wdl_solver* s = solvers[m];
+    WDL_CHECK(s, WDL_ERR_ARG, "wdl_group_create: NULL solver");
+    WDL_CHECK(s->ctx == ctx, WDL_ERR_ARG,
+              "wdl_group_create: every member must be created on the group's context");
+    WDL_CHECK(s->exact, WDL_ERR_UNSUPPORTED,
+              "wdl_group_create: members must be exact-shift log-domain solvers "
+              "(WDL_FLAG_LOG_DOMAIN | WDL_FLAG_EXACT_LSE)");
+    WDL_CHECK(!s->ring, WDL_ERR_UNSUPPORTED,
+              "wdl_group_create: members with checkpointed histories are not supported");
+    WDL_CHECK(s->loss != WDL_LOSS_WASSERSTEIN, WDL_ERR_UNSUPPORTED,
+              "wdl_group_create: the Wasserstein loss runs its own inner solve per solver");
+    const int w = (s->flags & WDL_FLAG_WEIGHTS_ONLY) ? 1 : 0;
+    WDL_CHECK(wo < 0 || wo == w, WDL_ERR_ARG,
+              "wdl_group_create: members must all have WDL_FLAG_WEIGHTS_ONLY or none");
+    wo = w;
+    for (int k = 0; k < m; ++k)
+      WDL_CHECK(solvers[k] != s, WDL_ERR_ARG, "wdl_group_create: a solver is listed twice");
+    const int len = std::max(s->W, s->H);
+    maxlen = std::max(maxlen, len);
+    wpl = std::max(wpl, xlse_waves_per_line(len));
+    Lmax = std::max(Lmax, s->L);
+    proj = proj && w && (s->flags & WDL_FLAG_WARM_RESTART);
+  }
+  WDL_HIP(hipSetDevice(ctx->device));
+  const int lds = xlse_lds_bytes(maxlen, kXWaves);
+  int st;
+  if ((st = xlse_group_set_attr(lds))) return st;
+  std::vector<SolveArgs> as;
+  std::vector<GroupMember> gm((size_t)n);
+  for (int m = 0; m < n; ++m) {
+    wdl_solver* s = solvers[m];
+    as.push_back(s->args(0, s->P));
+    gm[(size_t)m].a = as.back();
+    gm[(size_t)m].ig = 1.0 / s->gamma;
+    gm[(size_t)m].eta = s->eta.as<float>();
+    gm[(size_t)m].GDsum = s->GDsum.as<double>();
+  }
+  std::vector<int> counts[GK_COUNT];
+  group_blocks(as, lds, ctx->num_cus, counts);
+  std::vector<int> starts((size_t)GK_COUNT * (n + 1));
+  GroupShape sh{};
+  for (int k = 0; k < GK_COUNT; ++k) {
+    long long acc = 0;
+    for (int m = 0; m < n; ++m) {
+      starts[(size_t)k * (n + 1) + m] = (int)acc;
+      acc += counts[k][(size_t)m];
+    }
+    WDL_CHECK(acc < (1LL << 31), WDL_ERR_ARG, "wdl_group_create: too many workgroups");
+    starts[(size_t)k * (n + 1) + n] = (int)acc;
+    sh.grid[k] = (int)acc;
+  }
+  sh.wpl = wpl;
+  sh.lds = lds;
+  sh.maxlen = maxlen;
+  sh.weightsOnly = wo;
+  sh.Lmax = Lmax;
+  wdl_group* g = new wdl_group();
+  g->ctx = ctx;
+  g->ms.assign(solvers, solvers + n);
+  g->sh = sh;
+  g->projection = proj;
+  const size_t mbytes = (size_t)n * sizeof(GroupMember);
+  if ((st = g->dev.ensure(mbytes + starts.size() * sizeof(int)))) {
+    delete g;
+    return st;
+  }
+  hipError_t e = hipMemcpy(g->dev.p, gm.data(), mbytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy((char*)g->dev.p + mbytes, starts.data(), starts.size() * sizeof(int),
+                  hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    g->dev.release();
+    delete g;
+    set_error(std::string("wdl_group_create: ") + hipGetErrorString(e));
+    return WDL_ERR_HIP;
+  }
+  g->ga.m = g->dev.as<GroupMember>();
+  g->ga.starts = reinterpret_cast<const int*>((char*)g->dev.p + mbytes);
+  g->ga.nm = n;
+  *out = g;
+  return WDL_OK;
+}
+
+int wdl_group_destroy(wdl_group* g) {
+  if (!g) return WDL_OK;
+  (void)hipSetDevice(g->ctx->device);
+  (void)hipStreamSynchronize(g->ctx->stream);
+  for (int k = 0; k < 2; ++k) {
+    if (g->exec[k]) (void)hipGraphExecDestroy(g->exec[k]);
+    if (g->graph[k]) (void)hipGraphDestroy(g->graph[k]);
+  }
+  g->dev.release();
+  delete g;
+  return WDL_OK;
+}
+
+int wdl_group_run(wdl_group* g) {
+  WDL_CHECK(g, WDL_ERR_ARG, "wdl_group_run: NULL argument");
+  for (wdl_solver* s : g->ms)
+    WDL_CHECK(s->observed && s->problem, WDL_ERR_STATE,
+              "wdl_group_run: set_observed and set_problem must be called on every member");
+  WDL_HIP(hipSetDevice(g->ctx->device));
+  return g->launch(false);
+}
+
+int wdl_group_projection_round(wdl_group* g) {
+  WDL_CHECK(g, WDL_ERR_ARG, "wdl_group_projection_round: NULL argument");
+  WDL_CHECK(g->projection, WDL_ERR_STATE,
+            "wdl_group_projection_round: every member needs WDL_FLAG_WEIGHTS_ONLY | "
+            "WDL_FLAG_WARM_RESTART");
+  for (wdl_solver* s : g->ms)
+    WDL_CHECK(s->observed && s->problem, WDL_ERR_STATE,
+              "wdl_group_projection_round: set_observed and set_problem must be called on "
+              "every member");
+  WDL_HIP(hipSetDevice(g->ctx->device));
+  return g->launch(true);
+}
+
+int wdl_group_shape(const wdl_group* g, int* grids, int* waves_per_line, int* lds_bytes) {
+  WDL_CHECK(g, WDL_ERR_ARG, "wdl_group_shape: NULL argument");
+  if (grids) std::copy(g->sh.grid, g->sh.grid + GK_COUNT, grids);
+  if (waves_per_line) *waves_per_line = g->sh.wpl;
+  if (lds_bytes) *lds_bytes = g->sh.lds;
+  return WDL_OK;
 }
 
 }  // extern "C"

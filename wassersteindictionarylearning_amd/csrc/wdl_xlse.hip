@@ -164,11 +164,16 @@ constexpr int kXMaxThreads = 1024;
 // (any shift near the maximum serves: the output adds back the shift the terms
 // were formed with); exponents are formed in FP64 in log2 units and
 // exponentiated on the hardware exp2.
+//
+// blk / nblk: this workgroup's index and the number of workgroups sharing the
+// lines (the grid, or a member's share of a group launch).
 template <bool SIGNED, class Epi, int LW>
-__global__ __launch_bounds__(kXMaxThreads) void k_xlse_pass(
-    const double* __restrict__ in, const unsigned char* __restrict__ sin, Epi epi, int rows,
-    int len, long long nlines, double inv_gamma, const int* __restrict__ Lp, int p0, int ipp,
-    int active_thr) {
+__device__ __forceinline__ void xlse_pass_body(const double* __restrict__ in,
+                                               const unsigned char* __restrict__ sin, Epi epi,
+                                               int rows, int len, long long nlines,
+                                               double inv_gamma, const int* __restrict__ Lp,
+                                               int p0, int ipp, int active_thr, int blk,
+                                               int nblk) {
   extern __shared__ __align__(16) double xs[];
   const int lp = len + 2 * kXPad;
   const int wpl = blockDim.x / (LW * 64);  // waves per line
@@ -232,8 +237,8 @@ __global__ __launch_bounds__(kXMaxThreads) void k_xlse_pass(
       }
     }
   };
-  const long long stride = (long long)gridDim.x * LW;
-  long long l0 = (long long)blockIdx.x * LW;
+  const long long stride = (long long)nblk * LW;
+  long long l0 = (long long)blk * LW;
   if (l0 < nlines) fetch(l0);
   for (; l0 < nlines; l0 += stride) {
     const bool active = pact;
@@ -322,6 +327,15 @@ __global__ __launch_bounds__(kXMaxThreads) void k_xlse_pass(
     }
     __syncthreads();
   }
+}
+
+template <bool SIGNED, class Epi, int LW>
+__global__ __launch_bounds__(kXMaxThreads) void k_xlse_pass(
+    const double* __restrict__ in, const unsigned char* __restrict__ sin, Epi epi, int rows,
+    int len, long long nlines, double inv_gamma, const int* __restrict__ Lp, int p0, int ipp,
+    int active_thr) {
+  xlse_pass_body<SIGNED, Epi, LW>(in, sin, epi, rows, len, nlines, inv_gamma, Lp, p0, ipp,
+                                  active_thr, blockIdx.x, gridDim.x);
 }
 
 // Output stage of k_xlse_pass, called once per output (index oi into the
@@ -535,12 +549,16 @@ __device__ inline long long xoff(const SolveArgs& a, int nch, const XEl& r) {
   return (((long long)r.p * nch + r.cs) * a.K + r.i) * a.N + r.x;
 }
 
+// Grid-stride loop over a kernel's elements; blk_ / nblk_ are the workgroup's
+// index and the workgroups sharing the elements (the grid, or a member's share
+// of a group launch).
 #define XLOOP(total)                                                                     \
-  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < (total);      \
-       e += (long long)gridDim.x * blockDim.x)
+  for (long long e = (long long)blk_ * blockDim.x + threadIdx.x; e < (total);            \
+       e += (long long)nblk_ * blockDim.x)
+#define XGRID const int blk_ = blockIdx.x, nblk_ = gridDim.x
 
 // forward, input of the first convolution: log b^{s-1} (:711-724; b^0: 0 or the store)
-__global__ void k_xf1(SolveArgs a, int s) {
+__device__ inline void xf1_body(const SolveArgs& a, int s, int blk_, int nblk_) {
   // one thread per (histogram, pixel): the geometric mean of the K atoms'
   // phi^{s-1} is formed once and gives all K images' b
   const long long N = a.N, tot = (long long)a.np * N;
@@ -569,9 +587,11 @@ __global__ void k_xf1(SolveArgs a, int s) {
   }
 }
 
+__global__ void k_xf1(SolveArgs a, int s) { xf1_body(a, s, blockIdx.x, gridDim.x); }
+
 // backward step, first convolution's input: chain 0 (lam_i n + v_i) b^s, chain 1
 // (lam_i g + r_i) / phi^s, as log magnitude and sign (:740-749, :778-793)
-__global__ void k_xb1(SolveArgs a, int t, int nch) {
+__device__ inline void xb1_body(const SolveArgs& a, int t, int nch, int blk_, int nblk_) {
   const long long tot = (long long)a.np * nch * a.K * a.N;
   XLOOP(tot) {
     XEl r = xel(a, nch, e);
@@ -603,6 +623,8 @@ __global__ void k_xb1(SolveArgs a, int t, int nch) {
   }
 }
 
+__global__ void k_xb1(SolveArgs a, int t, int nch) { xb1_body(a, t, nch, blockIdx.x, gridDim.x); }
+
 // Weight-gradient dot of backward step s, <log phi_i^s, g_s> (:786), centred as
 // in the other solvers (exact_gsum), in two kernels.  k_xdot_chunk: one
 // workgroup per (histogram, atom, chunk of kXDotChunk pixels) writes the
@@ -612,10 +634,10 @@ __global__ void k_xb1(SolveArgs a, int t, int nch) {
 // the K atoms' r and of log phi_i^s.  k_xdot_fin adds the chunks in order, so
 // both forms give the same dots to the bit.
 template <bool FUSED>
-__global__ __launch_bounds__(kThreads) void k_xdot_chunk(SolveArgs a, int t) {
+__device__ inline void xdot_chunk_body(const SolveArgs& a, int t, int blk) {
   const long long N = a.N, nc = xdot_chunks(N);
-  const long long ic = blockIdx.x / nc;
-  const int c = (int)(blockIdx.x - ic * nc);
+  const long long ic = blk / nc;
+  const int c = (int)(blk - ic * nc);
   const int p = a.p0 + (int)(ic / a.K), i = (int)(ic % a.K);
   const int s = bwd_step(a, p, t);
   if (s < 1) return;
@@ -657,7 +679,12 @@ __global__ __launch_bounds__(kThreads) void k_xdot_chunk(SolveArgs a, int t) {
   }
 }
 
-__global__ void k_xdot_fin(SolveArgs a, int t) {
+template <bool FUSED>
+__global__ __launch_bounds__(kThreads) void k_xdot_chunk(SolveArgs a, int t) {
+  xdot_chunk_body<FUSED>(a, t, blockIdx.x);
+}
+
+__device__ inline void xdot_fin_body(const SolveArgs& a, int t, int blk_, int nblk_) {
   const long long N = a.N, nc = xdot_chunks(N);
   const long long tot = (long long)a.np * a.K;
   XLOOP(tot) {
@@ -675,6 +702,8 @@ __global__ void k_xdot_fin(SolveArgs a, int t) {
     *gl = s == steps_of(a, p) ? tfg : *gl + (tfg - tf / (double)N * (tg - exact_gsum(a, p, s)));
   }
 }
+
+__global__ void k_xdot_fin(SolveArgs a, int t) { xdot_fin_body(a, t, blockIdx.x, gridDim.x); }
 
 static dim3 ew_grid(long long n) {
   return dim3((unsigned)std::min<long long>((n + 255) / 256, 8192));
@@ -730,6 +759,193 @@ int exact_backward(const SolveArgs& a, hipStream_t st, int t) {
   return WDL_OK;
 }
 
+// ---- group launches -------------------------------------------------------------
+// Each kernel below finds its workgroup's member (group_member), copies the
+// member's arguments and runs the single-solver kernel's body on the member's
+// share of the grid, with the arguments exact_forward / exact_backward pass,
+// formed here from the member's SolveArgs (p0 = 0: a member runs all its
+// histograms).  A member with no histogram in the current step returns at
+// once, as all its lines and elements would be skipped.
+
+__global__ void k_xf1_group(GroupArgs g, int s) {
+  int lb, nb;
+  const SolveArgs a = g.m[group_member(g, GK_XF1, blockIdx.x, lb, nb)].a;
+  if (a.L < s) return;
+  xf1_body(a, s, lb, nb);
+}
+
+// Forward passes of iteration s: STAGE 0 / 1 the first convolution along x /
+// y (output stage XEpiFwd1), 2 / 3 the second (into the history Fh).
+template <int STAGE>
+__global__ __launch_bounds__(kXMaxThreads) void k_xlse_group_fwd(GroupArgs g, int s) {
+  int lb, nb;
+  const GroupMember& gm =
+      g.m[group_member(g, (STAGE & 1) ? GK_FB : GK_FA, blockIdx.x, lb, nb)];
+  const SolveArgs a = gm.a;
+  const double ig = gm.ig;
+  const int thr = max(s, a.rmin);
+  if (a.L < thr) return;
+  const int W = a.G.W, H = a.G.H;
+  const long long nimg = (long long)a.np * a.K, off = (long long)a.p0 * a.K * a.N;
+  double* xa = a.xa + a.xfwd + off;
+  double* xt = a.xt + a.xfwd + off;
+  if constexpr (STAGE == 0 || STAGE == 2) {
+    xlse_pass_body<false, XStore, kXWaves>(xa, nullptr, XStore{xt, nullptr}, H, W, nimg * H, ig,
+                                           a.Lp, a.p0, a.K, thr, lb, nb);
+  } else if constexpr (STAGE == 1) {
+    const XEpiFwd1 epi{(double*)a.Ch + hslot(a, s - 1) * a.stage + off, xa, a.logA, a.K, a.N};
+    xlse_pass_body<false, XEpiFwd1, kXWaves>(xt, nullptr, epi, W, H, nimg * W, ig, a.Lp, a.p0,
+                                             a.K, thr, lb, nb);
+  } else {
+    const XStore epi{(double*)a.Fh + hslot(a, s - 1) * a.stage + off, nullptr};
+    xlse_pass_body<false, XStore, kXWaves>(xt, nullptr, epi, W, H, nimg * W, ig, a.Lp, a.p0, a.K,
+                                           thr, lb, nb);
+  }
+}
+
+// Backward passes of step t: STAGE 0 / 1 the first convolution along x / y
+// (output stage XEpiBwd1), 2 / 3 the second (XEpiBwd2; launched for t >= 2).
+template <int STAGE>
+__global__ __launch_bounds__(kXMaxThreads) void k_xlse_group_bwd(GroupArgs g, int t) {
+  int lb, nb;
+  const GroupMember& gm =
+      g.m[group_member(g, (STAGE & 1) ? GK_BB : GK_BA, blockIdx.x, lb, nb)];
+  const SolveArgs a = gm.a;
+  const double ig = gm.ig;
+  if (a.L < t) return;
+  const int W = a.G.W, H = a.G.H, nch = a.weightsOnly ? 1 : 2;
+  const long long nimg = (long long)a.np * nch * a.K, off = (long long)a.p0 * nch * a.K * a.N;
+  if constexpr (STAGE == 0 || STAGE == 2) {
+    xlse_pass_body<true, XStore, kXWaves>(a.xa + off, a.sga + off, XStore{a.xt + off, a.sgt + off},
+                                          H, W, nimg * H, ig, a.Lp, a.p0, nch * a.K, t, lb, nb);
+  } else if constexpr (STAGE == 1) {
+    xlse_pass_body<true, XEpiBwd1, kXWaves>(a.xt + off, a.sgt + off, XEpiBwd1{a, t, nch, off}, W,
+                                            H, nimg * W, ig, a.Lp, a.p0, nch * a.K, t, lb, nb);
+  } else {
+    xlse_pass_body<true, XEpiBwd2, kXWaves>(a.xt + off, a.sgt + off, XEpiBwd2{a, t, nch}, W, H,
+                                            nimg * W, ig, a.Lp, a.p0, nch * a.K, t, lb, nb);
+  }
+}
+
+template <bool FUSED>
+__global__ __launch_bounds__(kThreads) void k_xdot_chunk_group(GroupArgs g, int t) {
+  int lb, nb;
+  const SolveArgs a = g.m[group_member(g, GK_DOT, blockIdx.x, lb, nb)].a;
+  if (a.L < t) return;
+  xdot_chunk_body<FUSED>(a, t, lb);
+}
+
+__global__ void k_xb1_group(GroupArgs g, int t) {
+  int lb, nb;
+  const SolveArgs a = g.m[group_member(g, GK_XB1, blockIdx.x, lb, nb)].a;
+  if (a.L < t) return;
+  xb1_body(a, t, a.weightsOnly ? 1 : 2, lb, nb);
+}
+
+__global__ void k_xdot_fin_group(GroupArgs g, int t) {
+  int lb, nb;
+  const SolveArgs a = g.m[group_member(g, GK_DOTFIN, blockIdx.x, lb, nb)].a;
+  if (a.L < t) return;
+  xdot_fin_body(a, t, lb, nb);
+}
+
+int xlse_group_set_attr(int lds) {
+  static std::atomic<int> done{0};  // groups may be created from several host threads
+  if (lds <= done.load(std::memory_order_relaxed)) return WDL_OK;
+  const void* fns[] = {(const void*)k_xlse_group_fwd<0>, (const void*)k_xlse_group_fwd<1>,
+                       (const void*)k_xlse_group_fwd<2>, (const void*)k_xlse_group_fwd<3>,
+                       (const void*)k_xlse_group_bwd<0>, (const void*)k_xlse_group_bwd<1>,
+                       (const void*)k_xlse_group_bwd<2>, (const void*)k_xlse_group_bwd<3>};
+  for (const void* f : fns)
+    WDL_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  int cur = done.load(std::memory_order_relaxed);
+  while (cur < lds && !done.compare_exchange_weak(cur, lds, std::memory_order_relaxed)) {
+  }
+  return WDL_OK;
+}
+
+void group_blocks(const std::vector<SolveArgs>& ms, int lds, int cus,
+                  std::vector<int> (&counts)[GK_COUNT]) {
+  const size_t nm = ms.size();
+  for (auto& c : counts) c.assign(nm, 0);
+  auto ew = [](long long n) { return (int)std::min<long long>((n + 255) / 256, 8192); };
+  // the passes: each member's lines in groups of kXWaves; when the members
+  // need more workgroups than twice what the device holds at once (the cap a
+  // single-solver pass takes), the cap is shared out by each member's work
+  // (lines x length), at least one workgroup each
+  const int perCU = std::max(1, std::min(160 * 1024 / lds, 8));
+  const long long cap = 2LL * (cus > 0 ? cus : 256) * perCU;
+  auto passes = [&](int kind, bool bwd, bool alongX) {
+    std::vector<long long> need(nm), work(nm);
+    long long tneed = 0, twork = 0;
+    for (size_t m = 0; m < nm; ++m) {
+      const SolveArgs& a = ms[m];
+      const int nch = bwd && !a.weightsOnly ? 2 : 1;
+      const long long nimg = (long long)a.np * nch * a.K;
+      const int rows = alongX ? a.G.H : a.G.W, len = alongX ? a.G.W : a.G.H;
+      const long long lines = nimg * rows;
+      need[m] = (lines + kXWaves - 1) / kXWaves;
+      work[m] = lines * len;
+      tneed += need[m];
+      twork += work[m];
+    }
+    for (size_t m = 0; m < nm; ++m) {
+      long long c = need[m];
+      if (tneed > cap)
+        c = std::min(need[m], std::max(1LL, (long long)((double)cap * work[m] / twork)));
+      counts[kind][m] = (int)c;
+    }
+  };
+  passes(GK_FA, false, true);
+  passes(GK_FB, false, false);
+  passes(GK_BA, true, true);
+  passes(GK_BB, true, false);
+  for (size_t m = 0; m < nm; ++m) {
+    const SolveArgs& a = ms[m];
+    const long long N = a.N, KN = (long long)a.K * N;
+    counts[GK_XF1][m] = ew((long long)a.np * N);
+    counts[GK_FINAL][m] = kLossBlocks * a.np;
+    counts[GK_DOT][m] = (int)((long long)a.np * a.K * xdot_chunks(N));
+    counts[GK_XB1][m] = a.weightsOnly ? 0 : ew((long long)a.np * 2 * KN);
+    counts[GK_DOTFIN][m] = ew((long long)a.np * a.K);
+    counts[GK_PROJ][m] = (a.P + 63) / 64;
+    counts[GK_WARM][m] = (int)std::min<long long>(64, (KN + 255) / 256) * a.P;
+    counts[GK_RGD][m] = a.weightsOnly ? 0 : (int)std::min<long long>(1024, (KN + 255) / 256);
+  }
+}
+
+int exact_group_forward(const GroupArgs& g, const GroupShape& sh, hipStream_t st, int s) {
+  const dim3 pb(kXWaves * 64 * sh.wpl);
+  hipLaunchKernelGGL(k_xf1_group, dim3(sh.grid[GK_XF1]), dim3(256), 0, st, g, s);
+  hipLaunchKernelGGL(k_xlse_group_fwd<0>, dim3(sh.grid[GK_FA]), pb, sh.lds, st, g, s);
+  hipLaunchKernelGGL(k_xlse_group_fwd<1>, dim3(sh.grid[GK_FB]), pb, sh.lds, st, g, s);
+  hipLaunchKernelGGL(k_xlse_group_fwd<2>, dim3(sh.grid[GK_FA]), pb, sh.lds, st, g, s);
+  hipLaunchKernelGGL(k_xlse_group_fwd<3>, dim3(sh.grid[GK_FB]), pb, sh.lds, st, g, s);
+  WDL_HIP(hipGetLastError());
+  return WDL_OK;
+}
+
+int exact_group_backward(const GroupArgs& g, const GroupShape& sh, hipStream_t st, int t) {
+  const dim3 pb(kXWaves * 64 * sh.wpl);
+  if (sh.weightsOnly) {
+    hipLaunchKernelGGL(k_xdot_chunk_group<true>, dim3(sh.grid[GK_DOT]), dim3(kThreads), 0, st, g,
+                       t);
+  } else {
+    hipLaunchKernelGGL(k_xdot_chunk_group<false>, dim3(sh.grid[GK_DOT]), dim3(kThreads), 0, st, g,
+                       t);
+    hipLaunchKernelGGL(k_xb1_group, dim3(sh.grid[GK_XB1]), dim3(256), 0, st, g, t);
+  }
+  hipLaunchKernelGGL(k_xdot_fin_group, dim3(sh.grid[GK_DOTFIN]), dim3(256), 0, st, g, t);
+  hipLaunchKernelGGL(k_xlse_group_bwd<0>, dim3(sh.grid[GK_BA]), pb, sh.lds, st, g, t);
+  hipLaunchKernelGGL(k_xlse_group_bwd<1>, dim3(sh.grid[GK_BB]), pb, sh.lds, st, g, t);
+  if (t >= 2) {
+    hipLaunchKernelGGL(k_xlse_group_bwd<2>, dim3(sh.grid[GK_BA]), pb, sh.lds, st, g, t);
+    hipLaunchKernelGGL(k_xlse_group_bwd<3>, dim3(sh.grid[GK_BB]), pb, sh.lds, st, g, t);
+  }
+  WDL_HIP(hipGetLastError());
+  return WDL_OK;
+}
+
 // ---- the Wasserstein loss (loss.h:92-154) ----------------------------------------
 // Its Bregman projections between the barycenter q and the observed o, 50
 // iterations from b = 1: c = K b, a = q / c, c = K a, b = o / c (:106-117), run
@@ -738,12 +954,14 @@ int exact_backward(const SolveArgs& a, hipStream_t st, int t) {
 // and gradient gamma * safe_log(a) (:148-150).
 
 __global__ void k_wl_init(SolveArgs a) {
+  XGRID;
   const long long tot = (long long)a.np * a.N, off = (long long)a.p0 * a.N;
   XLOOP(tot) a.wlb[off + e] = 0.0;
 }
 
 // log a = log q - conv (second = false) or log b = log o - conv (second = true)
 __global__ void k_wl_update(SolveArgs a, int second) {
+  XGRID;
   const long long tot = (long long)a.np * a.N, off = (long long)a.p0 * a.N;
   XLOOP(tot) {
     const long long i = off + e;
@@ -805,5 +1023,6 @@ int wasserstein_loss(const SolveArgs& a, hipStream_t st) {
 }
 
 #undef XLOOP
+#undef XGRID
 
 }  // namespace wdl
