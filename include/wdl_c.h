@@ -278,6 +278,17 @@ typedef struct wdl_server wdl_server;
 int wdl_server_create(int device, long long hbm_budget_bytes, int max_slots_per_pool,
                       wdl_server** out);
 int wdl_server_destroy(wdl_server* sv);
+/* How rounds are batched.  WDL_SERVER_POOLS (the default): each pool (one
+ * dictionary and iteration class) runs its rounds on its own stream.
+ * WDL_SERVER_MIXED: the pools of one iteration class share a stream and each
+ * round runs all of them -- dictionaries of different grid sizes, atom counts
+ * and gamma -- as one launch per stage kernel (wdl_group_projection_round);
+ * requests join and leave a class's pools between its rounds.  Results are
+ * the same in both modes.  Call before the first wdl_server_submit
+ * (WDL_ERR_STATE afterwards). */
+#define WDL_SERVER_POOLS 0
+#define WDL_SERVER_MIXED 1
+int wdl_server_set_batching(wdl_server* sv, int mode);
 /* A dictionary: K atoms (K x W x H, normalised per atom by the solver), gamma,
  * loss, the largest iteration count a request may ask for, flags 0,
  * WDL_FLAG_LOG_DOMAIN or WDL_FLAG_LOG_DOMAIN | WDL_FLAG_EXACT_LSE.  Returns its id. */

@@ -74,11 +74,11 @@ def test_projection_rounds_match_oracle(ctx, logd):
     s.close()
 
 
-def serve(requests, dicts, budget, max_slots=8, max_iterations=64):
+def serve(requests, dicts, budget, max_slots=8, max_iterations=64, mixed=False):
     """Runs the requests [(dict index, histogram, iterations, rounds, step)] through a
     fresh server; returns [(weights, loss)] and its stats.  A dictionary is
     (atoms, W, H, gamma[, flags])."""
-    sv = wdl.ProjectionServer(0, budget, max_slots)
+    sv = wdl.ProjectionServer(0, budget, max_slots, mixed=mixed)
     ids = [sv.add_dictionary(d[0], d[1], d[2], d[3], 2, max_iterations,
                              d[4] if len(d) > 4 else 0) for d in dicts]
     handles = [sv.submit(ids[d], h, it, r, st) for (d, h, it, r, st) in requests]
@@ -160,3 +160,36 @@ def test_server_rejects_bad_requests():
     with pytest.raises(wdl.WdlError):
         sv.submit(d + 1, np.ones(256) / 256, 5, 1)  # unknown dictionary
     sv.close()
+
+
+def test_server_mixed_batching_same_as_pools():
+    # Mixed batching: the exact-shift pools of one iteration class (here 32x32,
+    # 48x40, 136x136 and 64x64 dictionaries, K 2..4, gamma 0.5..2) run each
+    # round as one group launch; a standard-domain dictionary keeps its own
+    # stream.  Every request's weights and loss equal the pooled mode's to the
+    # bit, and one request matches the oracle's projection.
+    X = wdl.FLAG_LOG_DOMAIN | wdl.FLAG_EXACT_LSE
+    dicts = [(dictionary(32, 32, 4, 50, blobs=True), 32, 32, 1.0, X),
+             (dictionary(48, 40, 3, 51), 48, 40, 2.0, X),
+             (dictionary(136, 136, 2, 52, blobs=True), 136, 136, 0.5, X),
+             (dictionary(64, 64, 3, 53, blobs=True), 64, 64, 1.0, X),
+             (dictionary(24, 24, 3, 54), 24, 24, 1.0)]
+    rng = np.random.default_rng(55)
+    reqs = []
+    for k in range(14):
+        d = [0, 1, 2, 3, 0, 4, 1, 3, 2, 0, 4, 1, 3, 0][k]
+        W, H = dicts[d][1], dicts[d][2]
+        h = observed(W, H, 1, 200 + k)[0]
+        reqs.append((d, h, int(rng.integers(5, 60)), int(rng.integers(1, 4)), 0.4))
+    print("server: %d requests, pooled and mixed batching" % len(reqs), flush=True)
+    pooled, _ = serve(reqs, dicts, 8 << 30)
+    mixed, stats = serve(reqs, dicts, 8 << 30, mixed=True)
+    assert stats["completed"] == len(reqs)
+    for k in range(len(reqs)):
+        np.testing.assert_array_equal(mixed[k][0], pooled[k][0], err_msg="request %d" % k)
+        assert mixed[k][1] == pooled[k][1], k
+    d, h, it, r, st = reqs[1]
+    a, W, H, g = dicts[d][:4]
+    ow, ol = O.project_weights(a, h, W, H, g, it, r, st, 2, log_domain=True)
+    assert rel(mixed[1][0], ow) < 5e-5
+    assert abs(mixed[1][1] - ol) < TOL * abs(ol)

@@ -27,6 +27,7 @@ FLAG_BF16X3 = 128
 FLAG_CHECKPOINT = 256
 NOALLOC_MAX_VALUES = 2097152  # WDL_NOALLOC_MAX_VALUES
 RUN_BATCHED, RUN_PLAIN = 0, 1
+SERVER_POOLS, SERVER_MIXED = 0, 1
 STAGE_FORWARD, STAGE_BACKWARD = 0, 1
 
 _lib = None
@@ -105,6 +106,7 @@ _SIGS = {
                                ctypes.POINTER(ctypes.c_longlong)]),
     "wdl_server_wait": (_i, [_vp, ctypes.c_longlong, ctypes.c_double, _d, _d, _d, _d]),
     "wdl_server_stats": (_i, [_vp, ctypes.POINTER(ctypes.c_longlong)]),
+    "wdl_server_set_batching": (_i, [_vp, _i]),
     "wdl_group_create": (_i, [_vp, ctypes.POINTER(_vp), _i, ctypes.POINTER(_vp)]),
     "wdl_group_destroy": (_i, [_vp]),
     "wdl_group_run": (_i, [_vp]),
@@ -520,12 +522,20 @@ class ProjectionServer:
              "rounds")
 
     def __init__(self, device: int = 0, hbm_budget_bytes: int = 64 << 30,
-                 max_slots_per_pool: int = 16):
+                 max_slots_per_pool: int = 16, mixed: bool = False):
+        """mixed: the pools of one iteration class run each round as one group
+        launch (WDL_SERVER_MIXED) instead of on their own streams."""
         h = _vp()
         _check(lib().wdl_server_create(device, int(hbm_budget_bytes), int(max_slots_per_pool),
                                        ctypes.byref(h)), "wdl_server_create")
         self.handle = h
         self.K = {}
+        if mixed:
+            try:
+                _check(lib().wdl_server_set_batching(h, SERVER_MIXED), "wdl_server_set_batching")
+            except Exception:
+                self.close()
+                raise
 
     def add_dictionary(self, atoms, W, H, gamma, loss=LOSS_QUADRATIC, max_iterations=500,
                        flags=0) -> int:

@@ -26,6 +26,12 @@
 // others run is admitted as soon as a slot and memory are free.  Every slot's
 // arithmetic is independent of the others', so a request's result does not
 // depend on what it was batched with.
+//
+// Mixed batching (wdl_server_set_batching): the pools of one iteration class
+// share one stream, and each round of the class runs all of them -- every
+// dictionary and grid size of that class -- as one group launch per stage
+// kernel (wdl_group_projection_round), with a class's pools joined and left
+// between its rounds.  Classes run concurrently on their own streams.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -78,13 +84,24 @@ struct Request {
 
 struct Pool {
   int dict, lclass;
-  wdl_ctx* ctx = nullptr;
+  wdl_ctx* ctx = nullptr;  // the pool's own, or its class's when grouped
+  bool grouped = false;    // mixed batching of an exact-shift dictionary: rounds run
+                           // with the other pools of its class as one group
   wdl_solver* s = nullptr;
   std::vector<long long> slots;  // request id or -1
   int active = 0;
   bool busy = false;
   double idleSince = 0;
   long long bytes = 0;
+};
+
+// Mixed batching: the pools of one iteration class, their shared stream and
+// the group their rounds run as.
+struct ClassRun {
+  wdl_ctx* ctx = nullptr;
+  wdl_group* group = nullptr;
+  std::vector<Pool*> members;  // the pools `group` was made of, in pool order
+  bool busy = false;
 };
 
 }  // namespace
@@ -109,6 +126,26 @@ struct wdl_server {
   std::string error;
   std::thread worker;
   std::atomic<bool> stop{false};
+  std::atomic<int> mixed{0};                  // batching mode (set before the first submit)
+  std::unordered_map<int, ClassRun> classes;  // worker-owned (mixed batching)
+
+  // Pools of a dictionary run as groups in mixed batching when the dictionary
+  // is exact-shift log domain (the forms a group takes); others keep streams
+  // of their own.
+  bool grouped(const Dict& D) const {
+    return mixed.load() && (D.flags & WDL_FLAG_LOG_DOMAIN) && (D.flags & WDL_FLAG_EXACT_LSE);
+  }
+
+  ClassRun* class_of(int lclass) {
+    auto it = classes.find(lclass);
+    return it == classes.end() ? nullptr : &it->second;
+  }
+
+  void drop_group(ClassRun& c) {
+    if (c.group) wdl_group_destroy(c.group);
+    c.group = nullptr;
+    c.members.clear();
+  }
 
   int fail(const std::string& where) {
     std::lock_guard<std::mutex> g(mu);
@@ -119,8 +156,13 @@ struct wdl_server {
 
   void release_pool(size_t k) {
     Pool& p = *pools[k];
+    if (ClassRun* c = p.grouped ? class_of(p.lclass) : nullptr) {
+      // the class's group holds the pool's solver: it goes first (the class is
+      // idle -- a pool of a running class is busy and never released)
+      if (std::find(c->members.begin(), c->members.end(), &p) != c->members.end()) drop_group(*c);
+    }
     if (p.s) wdl_solver_destroy(p.s);
-    if (p.ctx) wdl_ctx_destroy(p.ctx);
+    if (p.ctx && !p.grouped) wdl_ctx_destroy(p.ctx);
     used -= p.bytes;
     pools.erase(pools.begin() + (long)k);
     std::lock_guard<std::mutex> g(mu);
@@ -173,12 +215,21 @@ struct wdl_server {
     p->lclass = lclass;
     p->slots.assign((size_t)cap, -1);
     const int flags = D.flags | kPoolFlags;
-    if (wdl_ctx_create(device, &p->ctx)) {
+    const bool mix = grouped(D);
+    p->grouped = mix;
+    if (mix) {  // the class's shared stream, created with its first pool
+      ClassRun& c = classes[lclass];
+      if (!c.ctx && wdl_ctx_create(device, &c.ctx)) {
+        fail("wdl_ctx_create");
+        return false;
+      }
+      p->ctx = c.ctx;
+    } else if (wdl_ctx_create(device, &p->ctx)) {
       fail("wdl_ctx_create");
       return false;
     }
     if (wdl_solver_create(p->ctx, D.W, D.H, D.K, cap, lclass, D.gamma, D.loss, flags, &p->s)) {
-      wdl_ctx_destroy(p->ctx);
+      if (!mix) wdl_ctx_destroy(p->ctx);
       fail("wdl_solver_create");
       return false;
     }
@@ -187,7 +238,7 @@ struct wdl_server {
     if (wdl_solver_set_problem(p->s, D.atoms.data(), lam.data()) ||
         wdl_solver_set_iterations(p->s, zero.data())) {
       wdl_solver_destroy(p->s);
-      wdl_ctx_destroy(p->ctx);
+      if (!mix) wdl_ctx_destroy(p->ctx);
       fail("pool setup");
       return false;
     }
@@ -198,6 +249,30 @@ struct wdl_server {
     std::lock_guard<std::mutex> g(mu);
     ++stats[1];
     stats[4] = used;
+    return true;
+  }
+
+  // A pool's round has ended: count it for each request and retire the
+  // requests that have run all their rounds.  False on failure (reported).
+  bool round_done(Pool& p) {
+    p.busy = false;
+    p.idleSince = now_s();
+    for (size_t k = 0; k < p.slots.size(); ++k) {
+      if (p.slots[k] < 0) continue;
+      Request& r = running.at(p.slots[k]);
+      if (++r.doneRounds < r.rounds) continue;
+      r.weights.resize((size_t)r.K);
+      if (wdl_solver_get_slot(p.s, (int)k, r.weights.data(), &r.loss) ||
+          wdl_solver_set_slot(p.s, (int)k, nullptr, nullptr, 0, 0.0)) {
+        fail("retire");
+        return false;
+      }
+      p.slots[k] = -1;
+      --p.active;
+      Request fin = std::move(r);
+      running.erase(fin.id);
+      finish(fin);
+    }
     return true;
   }
 
@@ -221,38 +296,43 @@ struct wdl_server {
         incoming.clear();
       }
       // 1. finished rounds
-      for (auto& pp : pools) {
-        Pool& p = *pp;
-        if (!p.busy) continue;
-        const int q = wdl_ctx_query(p.ctx);
-        if (q < 0) {
-          fail("wdl_ctx_query");
-          return;
-        }
-        if (q == 1) continue;
-        p.busy = false;
-        p.idleSince = now_s();
-        progress = true;
-        for (size_t k = 0; k < p.slots.size(); ++k) {
-          if (p.slots[k] < 0) continue;
-          Request& r = running.at(p.slots[k]);
-          if (++r.doneRounds < r.rounds) continue;
-          r.weights.resize((size_t)r.K);
-          if (wdl_solver_get_slot(p.s, (int)k, r.weights.data(), &r.loss) ||
-              wdl_solver_set_slot(p.s, (int)k, nullptr, nullptr, 0, 0.0)) {
-            fail("retire");
+      {
+        for (auto& pp : pools) {
+          Pool& p = *pp;
+          if (!p.busy || p.grouped) continue;
+          const int q = wdl_ctx_query(p.ctx);
+          if (q < 0) {
+            fail("wdl_ctx_query");
             return;
           }
-          p.slots[k] = -1;
-          --p.active;
-          Request fin = std::move(r);
-          running.erase(fin.id);
-          finish(fin);
+          if (q == 1) continue;
+          progress = true;
+          if (!round_done(p)) return;
+        }
+        for (auto& kv : classes) {
+          ClassRun& c = kv.second;
+          if (!c.busy) continue;
+          const int q = wdl_ctx_query(c.ctx);
+          if (q < 0) {
+            fail("wdl_ctx_query");
+            return;
+          }
+          if (q == 1) continue;
+          c.busy = false;
+          progress = true;
+          for (Pool* p : c.members)
+            if (!round_done(*p)) return;
         }
       }
-      // 2. admission, oldest first
+      // 2. admission, oldest first (mixed batching: into pools of a class
+      // between its rounds, since a class's pools share its stream)
       for (auto it = pending.begin(); it != pending.end();) {
         Request& r = running.at(*it);
+        const ClassRun* cr = grouped(*dict(r.dict)) ? class_of(r.lclass) : nullptr;
+        if (cr && cr->busy) {
+          ++it;
+          continue;
+        }
         Pool* target = nullptr;
         for (auto& pp : pools)
           if (pp->dict == r.dict && pp->lclass == r.lclass && !pp->busy &&
@@ -297,17 +377,53 @@ struct wdl_server {
         progress = true;
       }
       // 3. next rounds
-      for (auto& pp : pools) {
-        Pool& p = *pp;
-        if (p.busy || p.active == 0) continue;
-        if (wdl_solver_projection_round(p.s)) {
-          fail("wdl_solver_projection_round");
-          return;
+      {
+        for (auto& pp : pools) {
+          Pool& p = *pp;
+          if (p.busy || p.active == 0 || p.grouped) continue;
+          if (wdl_solver_projection_round(p.s)) {
+            fail("wdl_solver_projection_round");
+            return;
+          }
+          p.busy = true;
+          progress = true;
+          std::lock_guard<std::mutex> g(mu);
+          ++stats[5];
         }
-        p.busy = true;
-        progress = true;
-        std::lock_guard<std::mutex> g(mu);
-        ++stats[5];
+        // every pool of an idle class with an active slot runs a round: the
+        // class's pools as one group (made again when its pools changed;
+        // pools with no active slot sit out its launches at no cost)
+        for (auto& kv : classes) {
+          ClassRun& c = kv.second;
+          if (c.busy) continue;
+          std::vector<Pool*> ms;
+          bool any = false;
+          for (auto& pp : pools)
+            if (pp->grouped && pp->lclass == kv.first) {
+              ms.push_back(pp.get());
+              any = any || pp->active > 0;
+            }
+          if (!any) continue;
+          if (ms != c.members) {
+            drop_group(c);
+            std::vector<wdl_solver*> ss;
+            for (Pool* p : ms) ss.push_back(p->s);
+            if (wdl_group_create(c.ctx, ss.data(), (int)ss.size(), &c.group)) {
+              fail("wdl_group_create");
+              return;
+            }
+            c.members = ms;
+          }
+          if (wdl_group_projection_round(c.group)) {
+            fail("wdl_group_projection_round");
+            return;
+          }
+          c.busy = true;
+          for (Pool* p : ms) p->busy = true;
+          progress = true;
+          std::lock_guard<std::mutex> g(mu);
+          ++stats[5];
+        }
       }
       if (!progress) std::this_thread::sleep_for(std::chrono::microseconds(20));
     }
@@ -345,7 +461,22 @@ int wdl_server_destroy(wdl_server* sv) {
   if (sv->worker.joinable()) sv->worker.join();
   (void)hipSetDevice(sv->device);
   while (!sv->pools.empty()) sv->release_pool(sv->pools.size() - 1);
+  for (auto& kv : sv->classes) {
+    sv->drop_group(kv.second);
+    if (kv.second.ctx) wdl_ctx_destroy(kv.second.ctx);
+  }
   delete sv;
+  return WDL_OK;
+}
+
+int wdl_server_set_batching(wdl_server* sv, int mixed) {
+  WDL_CHECK(sv, WDL_ERR_ARG, "wdl_server_set_batching: NULL argument");
+  WDL_CHECK(mixed == WDL_SERVER_POOLS || mixed == WDL_SERVER_MIXED, WDL_ERR_ARG,
+            "wdl_server_set_batching: mode must be WDL_SERVER_POOLS or WDL_SERVER_MIXED");
+  std::lock_guard<std::mutex> g(sv->mu);
+  WDL_CHECK(sv->nextId == 1, WDL_ERR_STATE,
+            "wdl_server_set_batching: must be called before the first request");
+  sv->mixed.store(mixed);
   return WDL_OK;
 }
 

@@ -216,6 +216,8 @@ struct GroupMember {
   double ig;            // 1 / gamma, as the single-solver launches compute it (host)
   const float* eta;     // mirror-descent steps per histogram (projection rounds)
   double* GDsum;        // dL/datoms summed over the histograms (K x N)
+  const int* lmax;      // the largest per-histogram iteration count (device, kept by
+                        // the solver): a member sits out the launches past it
 };
 
 enum GroupKind {

@@ -826,7 +826,8 @@ struct wdl_solver {
   int D = 0;
   DevBuf tab, A, lam, obs, Bh, Qh, Ch, Fh, V0, V1, R0, R1, GD, GL, Q, G0, lossPart, bStore,
       bTemp, GDsum, logA, events, Lp, X0, X1, XS0, XS1, GLt, gsumPart, Mt, eta, xa, xt, xdp,
-      nj, gj, sga, sgt, wla, wlb, wlc, wlt, evX, evG, evW, evA, evF, evP, CK, LF0, LF1;
+      nj, gj, sga, sgt, wla, wlb, wlc, wlt, evX, evG, evW, evA, evF, evP, CK, LF0, LF1, lmax;
+  std::vector<int> hostLp;  // the per-histogram iteration counts; lmax holds their maximum
   int seg = 0, ring = 0;  // checkpointed histories: segment length, history slots (0: L slots)
   int fwdT = 1024, bwdT = 768, xwpl = 0, xlw = 4;  // launch shape (wdl_solver_set_tuning)
   bool tiled = false;  // 64 x 64 tiles with a halo (a side above kMaxSide)
@@ -1260,6 +1261,7 @@ int wdl_solver_create(wdl_ctx* ctx, int W, int H, int K, int P, int L, double ga
       {&s->logA, (flags & WDL_FLAG_LOG_DOMAIN) ? (size_t)K * N * f4 : 0},
       {&s->events, sizeof(unsigned)},
       {&s->Lp, (size_t)P * sizeof(int)},
+      {&s->lmax, sizeof(int)},
       {&s->X0, tiled ? (size_t)PKN * hsz : 0},
       {&s->X1, tiled ? (size_t)PKN * hsz : 0},
       {&s->XS0, tiled && (flags & WDL_FLAG_LOG_DOMAIN) ? (size_t)PKN : 0},
@@ -1294,8 +1296,9 @@ int wdl_solver_create(wdl_ctx* ctx, int W, int H, int K, int P, int L, double ga
   }
   hipError_t e = hipMemcpy(s->tab.p, tab.data(), tab.size() * f4, hipMemcpyHostToDevice);
   if (e == hipSuccess) {
-    const std::vector<int> lp((size_t)P, L);
-    e = hipMemcpy(s->Lp.p, lp.data(), lp.size() * sizeof(int), hipMemcpyHostToDevice);
+    s->hostLp.assign((size_t)P, L);
+    e = hipMemcpy(s->Lp.p, s->hostLp.data(), (size_t)P * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(s->lmax.p, &L, sizeof(int), hipMemcpyHostToDevice);
   }
   // Outputs start at zero, so a histogram left idle (0 iterations) reads 0
   // until it first runs, and adds 0 to the summed atom gradient.
@@ -1349,7 +1352,7 @@ int wdl_solver_destroy(wdl_solver* s) {
                     &s->logA, &s->events, &s->Lp, &s->X0, &s->X1, &s->XS0, &s->XS1, &s->GLt, &s->Mt, &s->gsumPart, &s->eta, &s->xa, &s->xt, &s->xdp, &s->nj, &s->gj, &s->sga,
                     &s->sgt, &s->wla, &s->wlb, &s->wlc,
                     &s->wlt, &s->evX, &s->evG, &s->evW, &s->evA, &s->evF, &s->evP,
-                    &s->CK, &s->LF0, &s->LF1};
+                    &s->CK, &s->LF0, &s->LF1, &s->lmax};
   for (DevBuf* b : bufs) b->release();
   delete s;
   return WDL_OK;
@@ -1595,9 +1598,12 @@ int wdl_solver_set_iterations(wdl_solver* s, const int* iterations) {
       lp[p] = iterations[p];
     }
   WDL_HIP(hipSetDevice(s->ctx->device));
+  const int mx = *std::max_element(lp.begin(), lp.end());
   WDL_HIP(hipMemcpyAsync(s->Lp.p, lp.data(), lp.size() * sizeof(int), hipMemcpyHostToDevice,
                          s->ctx->stream));
+  WDL_HIP(hipMemcpyAsync(s->lmax.p, &mx, sizeof(int), hipMemcpyHostToDevice, s->ctx->stream));
   WDL_HIP(hipStreamSynchronize(s->ctx->stream));
+  s->hostLp = lp;
   return WDL_OK;
 }
 
@@ -1771,6 +1777,9 @@ int wdl_solver_set_slot(wdl_solver* s, int slot, const double* observed, const d
   }
   const float et = (float)step;
   WDL_HIP(hipMemcpyAsync(s->Lp.as<int>() + slot, &iterations, sizeof(int), hipMemcpyHostToDevice, st));
+  s->hostLp[(size_t)slot] = iterations;
+  const int mx = *std::max_element(s->hostLp.begin(), s->hostLp.end());
+  WDL_HIP(hipMemcpyAsync(s->lmax.p, &mx, sizeof(int), hipMemcpyHostToDevice, st));
   WDL_HIP(hipMemcpyAsync(s->eta.as<float>() + slot, &et, sizeof(float), hipMemcpyHostToDevice, st));
   if (s->flags & WDL_FLAG_WARM_RESTART) {
     // the start value of b_storage, 1 in both builds of the reference (:380-384)
@@ -2020,6 +2029,7 @@ int wdl_group_create(wdl_ctx* ctx, wdl_solver* const* solvers, int n, wdl_group*
     gm[(size_t)m].ig = 1.0 / s->gamma;
     gm[(size_t)m].eta = s->eta.as<float>();
     gm[(size_t)m].GDsum = s->GDsum.as<double>();
+    gm[(size_t)m].lmax = s->lmax.as<int>();
   }
   std::vector<int> counts[GK_COUNT];
   group_blocks(as, lds, ctx->num_cus, counts);

@@ -764,13 +764,15 @@ int exact_backward(const SolveArgs& a, hipStream_t st, int t) {
 // member's arguments and runs the single-solver kernel's body on the member's
 // share of the grid, with the arguments exact_forward / exact_backward pass,
 // formed here from the member's SolveArgs (p0 = 0: a member runs all its
-// histograms).  A member with no histogram in the current step returns at
+// histograms).  A member whose histograms have fewer iterations than the
+// current step (lmax: the largest of its per-histogram counts) returns at
 // once, as all its lines and elements would be skipped.
 
 __global__ void k_xf1_group(GroupArgs g, int s) {
   int lb, nb;
-  const SolveArgs a = g.m[group_member(g, GK_XF1, blockIdx.x, lb, nb)].a;
-  if (a.L < s) return;
+  const GroupMember& gm = g.m[group_member(g, GK_XF1, blockIdx.x, lb, nb)];
+  if (*gm.lmax < s) return;
+  const SolveArgs a = gm.a;
   xf1_body(a, s, lb, nb);
 }
 
@@ -784,7 +786,7 @@ __global__ __launch_bounds__(kXMaxThreads) void k_xlse_group_fwd(GroupArgs g, in
   const SolveArgs a = gm.a;
   const double ig = gm.ig;
   const int thr = max(s, a.rmin);
-  if (a.L < thr) return;
+  if (*gm.lmax < thr) return;
   const int W = a.G.W, H = a.G.H;
   const long long nimg = (long long)a.np * a.K, off = (long long)a.p0 * a.K * a.N;
   double* xa = a.xa + a.xfwd + off;
@@ -812,7 +814,7 @@ __global__ __launch_bounds__(kXMaxThreads) void k_xlse_group_bwd(GroupArgs g, in
       g.m[group_member(g, (STAGE & 1) ? GK_BB : GK_BA, blockIdx.x, lb, nb)];
   const SolveArgs a = gm.a;
   const double ig = gm.ig;
-  if (a.L < t) return;
+  if (*gm.lmax < t) return;
   const int W = a.G.W, H = a.G.H, nch = a.weightsOnly ? 1 : 2;
   const long long nimg = (long long)a.np * nch * a.K, off = (long long)a.p0 * nch * a.K * a.N;
   if constexpr (STAGE == 0 || STAGE == 2) {
@@ -830,22 +832,25 @@ __global__ __launch_bounds__(kXMaxThreads) void k_xlse_group_bwd(GroupArgs g, in
 template <bool FUSED>
 __global__ __launch_bounds__(kThreads) void k_xdot_chunk_group(GroupArgs g, int t) {
   int lb, nb;
-  const SolveArgs a = g.m[group_member(g, GK_DOT, blockIdx.x, lb, nb)].a;
-  if (a.L < t) return;
+  const GroupMember& gm = g.m[group_member(g, GK_DOT, blockIdx.x, lb, nb)];
+  if (*gm.lmax < t) return;
+  const SolveArgs a = gm.a;
   xdot_chunk_body<FUSED>(a, t, lb);
 }
 
 __global__ void k_xb1_group(GroupArgs g, int t) {
   int lb, nb;
-  const SolveArgs a = g.m[group_member(g, GK_XB1, blockIdx.x, lb, nb)].a;
-  if (a.L < t) return;
+  const GroupMember& gm = g.m[group_member(g, GK_XB1, blockIdx.x, lb, nb)];
+  if (*gm.lmax < t) return;
+  const SolveArgs a = gm.a;
   xb1_body(a, t, a.weightsOnly ? 1 : 2, lb, nb);
 }
 
 __global__ void k_xdot_fin_group(GroupArgs g, int t) {
   int lb, nb;
-  const SolveArgs a = g.m[group_member(g, GK_DOTFIN, blockIdx.x, lb, nb)].a;
-  if (a.L < t) return;
+  const GroupMember& gm = g.m[group_member(g, GK_DOTFIN, blockIdx.x, lb, nb)];
+  if (*gm.lmax < t) return;
+  const SolveArgs a = gm.a;
   xdot_fin_body(a, t, lb, nb);
 }
 
