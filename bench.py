@@ -182,16 +182,19 @@ def projection_leg(ctx, requests=4096, chunk=512, one_launch=True):
                                        "tests/golden/large_projection.npz)"}}
 
 
-def served_leg(duration_s=20.0, rate=2.0, rounds=3, step=0.5, budget=200 << 30, seed=11):
+def served_leg(duration_s=20.0, rate=2.0, rounds=3, step=0.5, budget=200 << 30, seed=11,
+               mixed=False):
     """Served projection loop: Poisson arrivals at `rate` requests/s for
     `duration_s`, grid sizes 64/128/256/512 (probabilities .4/.3/.2/.1), one
     random-blob dictionary per size (K uniform 4..10, gamma from {0.5, 1, 2}),
     L uniform 20..500; each request fits its weights with `rounds` projection
     rounds (log domain, exact shifts: blob dictionaries make the scalings
     diverge).  The native server batches requests continuously within a
-    device-memory budget; latency is submit to completion."""
+    device-memory budget; latency is submit to completion.  mixed: the pools
+    of one iteration class (all grid sizes) run each round as one group
+    launch instead of on their own streams."""
     flags = wdl.FLAG_LOG_DOMAIN | wdl.FLAG_EXACT_LSE
-    sv = wdl.ProjectionServer(0, budget, 16)
+    sv = wdl.ProjectionServer(0, budget, 16, mixed=mixed)
     wl, arrivals, rng = synthetic.served_workload(seed, rate, duration_s)
     sizes = sorted(wl)
     dicts = {}
@@ -236,6 +239,8 @@ def served_leg(duration_s=20.0, rate=2.0, rounds=3, step=0.5, budget=200 << 30, 
     return {"workload": "served projection: Poisson arrivals %.1f/s for %.0f s, sizes 64/128/256/512 "
                         "(p .4/.3/.2/.1), blob dictionaries K 4..10, gamma {0.5,1,2}, L 20..500, "
                         "%d rounds per request, log domain exact shifts" % (rate, duration_s, rounds),
+            "batching": "mixed: one group launch per iteration class and round" if mixed else
+                        "pools: one stream per dictionary and iteration class",
             "requests": len(res), "finite_results": finite, "wall_s": round(wall, 3),
             "solves_per_s": round(len(res) / wall, 3),
             "value": round(iters / wall, 1), "unit": "Sinkhorn-iters/s",
@@ -249,6 +254,90 @@ def served_leg(duration_s=20.0, rate=2.0, rounds=3, step=0.5, budget=200 << 30, 
             "pools_created": st["pools_created"], "pools_released": st["pools_released"],
             "rounds_launched": st["rounds"] - st0["rounds"], "hbm_budget_gb": budget / 1e9,
             "err_vs_fp64": err}
+
+
+def uneven_leg(reps=3, seed=11):
+    """Uneven grid sizes in one launch: projection rounds (weights only, warm
+    start, exact shifts) over the served workload's four dictionaries at once
+    -- 64x64, 128x128, 256x256, 512x512 with their own K and gamma -- with 32,
+    16, 8 and 4 requests whose Sinkhorn iterations are uniform in 20..100.  A
+    round is timed as one solver group (one launch per stage kernel over all
+    sizes) and as the four solvers on streams of their own (the server's
+    pooled form), same inputs; the results are compared bit for bit."""
+    wl, _, _ = synthetic.served_workload(seed)
+    counts = {64: 32, 128: 16, 256: 8, 512: 4}
+    flags = (wdl.FLAG_WEIGHTS_ONLY | wdl.FLAG_WARM_RESTART | wdl.FLAG_LOG_DOMAIN |
+             wdl.FLAG_EXACT_LSE)
+    rng = np.random.default_rng(seed + 1)
+    probs = {}
+    for n, P in counts.items():
+        K, g, atoms = wl[n]
+        probs[n] = (K, g, atoms, synthetic.blob_histograms(rng, n, P, (1, 5), (3, 20)),
+                    rng.integers(20, 101, P).astype(np.int32))
+    iters = int(sum(int(v[4].sum()) for v in probs.values()))
+
+    def build(ctxs):
+        ss = []
+        for (n, P), c in zip(counts.items(), ctxs):
+            K, g, atoms, obs, it = probs[n]
+            s = wdl.SinkhornSolver(c, n, n, K, P, 100, g, wdl.LOSS_QUADRATIC, flags)
+            s.set_observed(obs)
+            s.set_problem(atoms, np.full((P, K), 1.0 / K))
+            s.set_iterations(it)
+            s.set_streams(1)
+            ss.append(s)
+        return ss
+
+    def timed(run, sync):
+        run()  # warm-up: graph capture
+        sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            run()
+        sync()
+        return (time.perf_counter() - t0) / reps
+
+    def state(ss):
+        return [np.concatenate([s.get_slot(p)[0] for p in range(s.P)]) for s in ss]
+
+    # one group on one context
+    ctx = wdl.Context(0)
+    ms = build([ctx] * 4)
+    grp = wdl.SolverGroup(ctx, ms)
+    tg = timed(grp.projection_round, ctx.synchronize)
+    shape = grp.shape()
+    wg = state(ms)
+    grp.close()
+    ctx.close()
+    # the same solvers, each on its own context (stream)
+    ctxs = [wdl.Context(0) for _ in counts]
+    ss = build(ctxs)
+
+    def run_all():
+        for s in ss:
+            s.projection_round()
+
+    def sync_all():
+        for c in ctxs:
+            c.synchronize()
+
+    tp = timed(run_all, sync_all)
+    wp = state(ss)
+    for c in ctxs:
+        c.close()
+    same = all(np.array_equal(a, b) for a, b in zip(wg, wp))
+    return {"workload": "uneven grid sizes: projection rounds over 64^2 x32, 128^2 x16, 256^2 x8, "
+                        "512^2 x4 requests (the served workload's dictionaries, K %s, gamma %s), "
+                        "L_p uniform 20..100, weights only, warm start, exact shifts"
+                        % ([probs[n][0] for n in counts], [probs[n][1] for n in counts]),
+            "sinkhorn_iters_per_round": iters,
+            "group": {"ms_per_round": round(tg * 1e3, 3), "value": round(iters / tg, 1),
+                      "unit": "Sinkhorn-iters/s", "waves_per_line": shape["waves_per_line"],
+                      "form": "one solver group: one launch per stage kernel over all sizes"},
+            "streams": {"ms_per_round": round(tp * 1e3, 3), "value": round(iters / tp, 1),
+                        "unit": "Sinkhorn-iters/s",
+                        "form": "one solver per size, each on its own stream"},
+            "results_bit_equal": bool(same)}
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_backward_batched_std_traffic.md")
@@ -612,6 +701,8 @@ def main():
     ap.add_argument("--no-projection", action="store_true", help="skip the projection leg")
     ap.add_argument("--no-log", action="store_true", help="skip the 256x256 log-domain leg")
     ap.add_argument("--no-served", action="store_true", help="skip the served projection leg")
+    ap.add_argument("--no-uneven", action="store_true",
+                    help="skip the uneven-grid-sizes (solver group) leg")
     ap.add_argument("--no-bf16x3", action="store_true", help="skip the split-BF16x3 leg")
     ap.add_argument("--no-line", action="store_true", help="skip the 1-D / 3-D kernel leg")
     ap.add_argument("--projection-requests", type=int, default=4096,
@@ -784,6 +875,13 @@ def main():
     if not args.no_served:
         line["served"] = served_leg(args.served_seconds)
         log("served: %s" % json.dumps(line["served"]["latency_ms"]))
+        line["served_mixed"] = served_leg(args.served_seconds, mixed=True)
+        log("served, mixed batching: %s" % json.dumps(line["served_mixed"]["latency_ms"]))
+    if not args.no_uneven:
+        line["uneven_sizes"] = uneven_leg()
+        log("uneven sizes: group %.1f ms, streams %.1f ms per round" %
+            (line["uneven_sizes"]["group"]["ms_per_round"],
+             line["uneven_sizes"]["streams"]["ms_per_round"]))
     bres = None
     if not args.no_bf16x3:
         line["bf16x3"], bres = bf16x3_leg(ctx, c, args.steps)
