@@ -80,7 +80,10 @@ int make_tile_geom(int W, int H, int D, double gamma, bool logdom, TileGeom& t) 
   t.H = H;
   t.HB = hb;
   t.R = kTile + 2 * hb;
-  t.S = round_up(t.R, 64) + 4;  // congruent to 4 mod 64 floats, as make_geom's strides
+  // R is a multiple of 32, so S / 4 = R / 4 + 1 is odd: the 16 rows of a
+  // quarter-wave's block start on 16 distinct groups of 4 banks (as the
+  // resident strides, which are congruent to 4 mod 64), with the least LDS
+  t.S = t.R + 4;
   t.D = D;
   t.nx = (W + kTile - 1) / kTile;
   t.ny = (H + kTile - 1) / kTile;

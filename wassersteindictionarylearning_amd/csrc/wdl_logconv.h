@@ -49,6 +49,17 @@ __device__ inline floatx4 vf(doublex4 x) { return __builtin_convertvector(x, flo
 // log of a linear FP32 value as an FP64 log value: the binary exponent goes
 // in exactly, only log of the mantissa (in [0.5, 1), |log| < 0.7) is
 // rounded, so the absolute error stays ~4e-8 however large |log x| is.
+// exp(x - m) for an FP64 log value x and an FP32 shift m: the difference and
+// its scaling to log2 units in FP64, rounded once to FP32, then the hardware
+// exp2 -- the same single rounding of the argument as expf(float(x - m)) has,
+// without expf's range reduction.
+__device__ inline float vexp_shift(double x, float m) {
+  return __builtin_amdgcn_exp2f((float)((x - (double)m) * 1.4426950408889634));
+}
+__device__ inline floatx4 vexp_shift(doublex4 x, float m) {
+  return floatx4{vexp_shift(x.x, m), vexp_shift(x.y, m), vexp_shift(x.z, m), vexp_shift(x.w, m)};
+}
+
 __device__ inline double hlog(float x) {
   int e;
   const float m = frexpf(x, &e);

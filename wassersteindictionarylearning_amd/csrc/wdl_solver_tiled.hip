@@ -85,14 +85,21 @@ struct TileSmem {
   LogLds L;
 };
 
+// Log domain: the row keys (R region rows), the column keys (64 inner
+// columns) and the finite minima of either, right after the image: a
+// 96-row region then takes 40 KB of LDS in all, four workgroups per CU.
+__host__ __device__ inline int tile_keys(const TileGeom& T) {
+  return T.R + kTile + (T.R > kTile ? T.R : kTile);
+}
+
 __device__ inline TileSmem tile_smem(float* smem, const TileGeom& T) {
   TileSmem m;
   m.tabs = smem;
   m.buf = smem + kTabFloats;
   int* keys = reinterpret_cast<int*>(m.buf + T.R * T.S);
   m.L.mrow = keys;
-  m.L.mcol = keys + kMaxSide;
-  m.L.rmin = keys + 2 * kMaxSide;
+  m.L.mcol = keys + T.R;
+  m.L.rmin = keys + T.R + kTile;
   return m;
 }
 
@@ -727,7 +734,7 @@ __global__ __launch_bounds__(kThreads, 6) void k_tbwd2_log(SolveArgs a, int t) {
 // ---- launchers -------------------------------------------------------------------
 
 int tiled_lds_bytes(const SolveArgs& a) {
-  return tile_lds_bytes(a.T) + (a.logA ? kLogKeysBytes : 0);
+  return tile_lds_bytes(a.T) + (a.logA ? tile_keys(a.T) * (int)sizeof(int) : 0);
 }
 
 int tiled_set_lds_attr() {

@@ -63,18 +63,6 @@ __device__ inline void tile_region_pass(const TileGeom& T, int tx, int ty, Load&
   }
 }
 
-// tile_region_pass with the group index e = r * (R / VW) + k / VW handed to
-// apply(e, li, gi, valid, value) as well.
-template <int VW, class Load, class Apply>
-__device__ inline void tile_region_pass_e(const TileGeom& T, int tx, int ty, Load&& load,
-                                          Apply&& apply) {
-  tile_region_pass<VW>(T, tx, ty, load, [&](int li, int gi, bool ok, const auto& v) {
-    const int q = T.R / VW;
-    const int r = li / T.S;
-    apply(r * q + (li - r * T.S) / VW, li, gi, ok, v);
-  });
-}
-
 // Visits the 64 x 64 inner tile as laid out by tile_conv's result (stride S).
 template <int VW, class Load, class Apply>
 __device__ inline void tile_inner_pass(const TileGeom& T, int tx, int ty, Load&& load,
@@ -126,49 +114,67 @@ __device__ inline bool tile_inner(const TileGeom& T, int li) {
 
 // Log-domain tile convolution (the shifted scheme of log_conv_lds on the
 // tile's shapes).  The input of region pixel group li / gi is
-// make(li, gi, v, second) -> SLog<VW> with v = load(gi); make is evaluated
-// twice, for the row maxima (second = false) and for the shifted values
-// (second = true), so side outputs belong in the second call.  make must not
-// read buf: the first evaluation stages its row keys there.  On return buf
-// holds S2 (64 x 64 at stride S) and L.mcol the shifts of the inner columns.
+// make(li, gi, v, second) -> SLog<VW> with v = load(gi).  With groups of 4
+// pixels make is evaluated once per group (second = true); single pixels
+// evaluate it twice, for the row maxima (second = false) and for the shifted
+// values (second = true), so side outputs belong in the second = true call.
+// make must not read buf.  On return buf holds S2 (64 x 64 at stride S) and
+// L.mcol the shifts of the inner columns.
 template <int VW, class Load, class Make>
 __device__ inline void tile_log_conv(float* buf, const float* tabs, const LogLds& L,
                                      const TileGeom& T, int tx, int ty, unsigned* events,
                                      Load&& load, Make&& make) {
   typedef typename Vec<VW>::T V;
-  for (int r = threadIdx.x; r < kMaxSide; r += kThreads) {
-    L.mrow[r] = fkey(-INFINITY);
-    L.rmin[r] = fkey(INFINITY);
-  }
-  __syncthreads();
   if constexpr (VW == 4) {
-    // Row maxima (and finite minima) without contended LDS atomics: each
-    // group's keys go to buf (free until the shifted values are written),
-    // then one thread per region row reduces its R / 4 groups.
-    const int q = T.R / 4, ng = T.R * q;
-    int* gmax = reinterpret_cast<int*>(buf);
-    int* gmin = gmax + ng;
-    tile_region_pass_e<VW>(T, tx, ty, load, [&](int e, int li, int gi, bool ok, const auto& v) {
-      int kx = fkey(-INFINITY), kn = fkey(INFINITY);
+    // One evaluation of the region: a half-wave (32 lanes) per region row, one
+    // group of 4 pixels per lane (R / 4 <= 32 groups), so the row's maximum
+    // and finite minimum are half-wave DPP reductions and each value is
+    // shifted by its row's maximum as soon as it is made -- one round of
+    // loads and one make() per pixel.  R is a multiple of 32, so every wave
+    // runs the same R / 16 rows' worth of steps with all lanes active.
+    const int q = T.R / 4;
+    const int lane = threadIdx.x & 63, l32 = lane & 31;
+    const int y0 = ty * kTile - T.HB, x0 = tx * kTile - T.HB;
+    for (int r = threadIdx.x >> 5; r < T.R; r += kThreads / 32) {
+      const int y = y0 + r, x = x0 + l32 * 4;
+      const int li = r * T.S + l32 * 4;
+      const bool ok = l32 < q && y >= 0 && y < T.H && x >= 0 && x < T.W;
+      const int gi = ok ? y * T.W + x : 0;
+      SLog<VW> z{};
+      float mx = -INFINITY, mn = INFINITY;
       if (ok) {
-        const V xf = vf(make(li, gi, v, false).x);
-        kx = fkey(vmax_all(xf));
-        kn = fkey(vmin_fin(xf));
+        z = make(li, gi, load(gi), true);
+        const V xf = vf(z.x);
+        mx = vmax_all(xf);
+        mn = vmin_fin(xf);
       }
-      gmax[e] = kx;
-      gmin[e] = kn;
-    });
-    __syncthreads();
-    for (int r = threadIdx.x; r < T.R; r += kThreads) {
-      int kx = fkey(-INFINITY), kn = fkey(INFINITY);
-      for (int g = 0; g < q; ++g) {
-        kx = max(kx, gmax[r * q + g]);
-        kn = min(kn, gmin[r * q + g]);
+      const float hx = half_scan(mx, MaxOp());
+      const float m = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(hx), 31));
+      const float m2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(hx), 63));
+      const float rm = (lane & 32) ? m2 : m;
+      // the range check: a finite value more than kLogRange under the row's
+      // maximum (one row per half of the ballot)
+      const unsigned long long bad = __ballot(rm - mn > kLogRange);
+      if (l32 < q) {
+        V e = vsplat<VW>(0.f);
+        if (ok && rm > -INFINITY) e = z.s * vexp_shift(z.x, rm);
+        vst<VW>(buf + li, e);
       }
-      L.mrow[r] = kx;
-      L.rmin[r] = kn;
+      if (l32 == 0) {
+        L.mrow[r] = fkey(rm);  // rows outside the image stay at -inf
+        if (events && ((bad >> (lane & 32)) & 0xffffffffull)) atomicAdd(events, 1u);
+      }
     }
   } else {
+    for (int r = threadIdx.x; r < T.R; r += kThreads) {
+      L.mrow[r] = fkey(-INFINITY);
+      L.rmin[r] = fkey(INFINITY);
+    }
+    __syncthreads();
+  }
+  if constexpr (VW != 4) {
+    // single pixels (widths not a multiple of 4): row keys by LDS atomics
+    // from a first evaluation, then the shifted values from a second
     tile_region_pass<VW>(T, tx, ty, load, [&](int li, int gi, bool ok, const auto& v) {
       if (ok) {
         const SLog<VW> z = make(li, gi, v, false);
@@ -178,18 +184,18 @@ __device__ inline void tile_log_conv(float* buf, const float* tabs, const LogLds
         atomicMin(L.rmin + r, fkey(vmin_fin(xf)));
       }
     });
+    __syncthreads();
+    check_range(L.mrow, L.rmin, T.R, events);  // region rows outside the image stay at -inf
+    tile_region_pass<VW>(T, tx, ty, load, [&](int li, int gi, bool ok, const auto& v) {
+      V e = vsplat<VW>(0.f);
+      if (ok) {
+        const SLog<VW> z = make(li, gi, v, true);
+        const float m = funkey(L.mrow[li / T.S]);
+        if (m > -INFINITY) e = z.s * vexpf(vf(z.x - (double)m));
+      }
+      vst<VW>(buf + li, e);
+    });
   }
-  __syncthreads();
-  check_range(L.mrow, L.rmin, T.R, events);  // region rows outside the image stay at -inf
-  tile_region_pass<VW>(T, tx, ty, load, [&](int li, int gi, bool ok, const auto& v) {
-    V e = vsplat<VW>(0.f);
-    if (ok) {
-      const SLog<VW> z = make(li, gi, v, true);
-      const float m = funkey(L.mrow[li / T.S]);
-      if (m > -INFINITY) e = z.s * vexpf(vf(z.x - (double)m));
-    }
-    vst<VW>(buf + li, e);
-  });
   floatx4 acc[kMaxUnits][4];
   __syncthreads();
   rows_gemm(buf, T.S, T.R, T.R, kTile, T.HB, tabs, T.D, acc);
@@ -197,43 +203,38 @@ __device__ inline void tile_log_conv(float* buf, const float* tabs, const LogLds
   store_transposed(buf, T.S, T.R, kTile, acc);  // line c (inner column), position r (region row)
   __syncthreads();
   const int ncol = min(kTile, T.W - tx * kTile);  // inner columns inside the image
-  // Column maxima (and finite minima) of T = M_r + log|S1|: one wave per
-  // inner column, reduced with DPP scans (no LDS atomics, no index division).
+  // Per inner column (one wave each, R <= 128 positions in two per lane, kept
+  // in registers): the maximum of T = M_r + log|S1| by a DPP scan, the range
+  // check by a ballot, then E2 = S1 exp(M_r - M'_c) written back in place.
   const int lane = threadIdx.x & 63, wave = wave_id();
+  const bool has1 = lane + 64 < T.R;
+  const float mr0 = funkey(L.mrow[lane]);
+  const float mr1 = has1 ? funkey(L.mrow[lane + 64]) : -INFINITY;
   for (int c = wave; c < kTile; c += kWaves) {
-    float mx = -INFINITY, mn = INFINITY;
-    if (c < ncol)
-      for (int r = lane; r < T.R; r += 64) {
-        const float m = funkey(L.mrow[r]);
-        if (m > -INFINITY) {
-          const float t = m + fast_logf(fabsf(buf[c * T.S + r]));
-          mx = fmaxf(mx, t);
-          mn = fminf(mn, vmin_fin(t));
-        }
-      }
-    mx = wave_max(mx);
-    mn = wave_min(mn);
+    float* col = buf + c * T.S;
+    const float v0 = col[lane], v1 = has1 ? col[lane + 64] : 0.f;
+    float t0 = -INFINITY, t1 = -INFINITY;
+    if (c < ncol) {
+      if (mr0 > -INFINITY) t0 = mr0 + fast_logf(fabsf(v0));
+      if (mr1 > -INFINITY) t1 = mr1 + fast_logf(fabsf(v1));
+    }
+    const float m = wave_max(fmaxf(t0, t1));
+    const float lo = fminf(vmin_fin(t0), vmin_fin(t1));
+    const unsigned long long bad = __ballot(m - lo > kLogRange);
     if (lane == 0) {
-      L.mcol[c] = fkey(mx);
-      L.rmin[c] = fkey(mn);
+      L.mcol[c] = fkey(m);
+      if (events && bad) atomicAdd(events, 1u);
     }
-  }
-  __syncthreads();
-  check_range(L.mcol, L.rmin, kTile, events);
-  for (int c = wave; c < kTile; c += kWaves) {
-    const float m = funkey(L.mcol[c]);
-    for (int r = lane; r < T.R; r += 64) {
-      float* p = buf + c * T.S + r;
-      float v = 0.f;
-      const float mr = funkey(L.mrow[r]);
-      if (c < ncol && m > -INFINITY && mr > -INFINITY) {
-        // E2 = S1 exp(M_r - M'_c), the shift difference in FP64; a factor
-        // beyond FP32 range only occurs for |S1| < e^-80 and goes through logs.
-        const float d = (float)((double)mr - (double)m);
-        v = d < 80.f ? *p * expf(d) : vsign(*p) * expf(logf(fabsf(*p)) + d);
-      }
-      *p = v;
-    }
+    // a factor beyond FP32 range only occurs for |S1| < e^-80 and goes
+    // through logs
+    auto e2 = [&](float v, float mr) {
+      if (!(c < ncol && m > -INFINITY && mr > -INFINITY)) return 0.f;
+      const float d = mr - m;
+      return d < 80.f ? v * vexp_shift((double)mr, m)
+                      : vsign(v) * expf(logf(fabsf(v)) + d);
+    };
+    col[lane] = e2(v0, mr0);
+    if (has1) col[lane + 64] = e2(v1, mr1);
   }
   __syncthreads();
   rows_gemm(buf, T.S, kTile, T.R, kTile, T.HB, tabs, T.D, acc);
