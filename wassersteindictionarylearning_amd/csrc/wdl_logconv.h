@@ -60,10 +60,20 @@ __device__ inline floatx4 vexp_shift(doublex4 x, float m) {
   return floatx4{vexp_shift(x.x, m), vexp_shift(x.y, m), vexp_shift(x.z, m), vexp_shift(x.w, m)};
 }
 
+// exp(d) of an FP64 exponent, by the same route as vexp_shift.
+__device__ inline float vexp_d(double d) {
+  return __builtin_amdgcn_exp2f((float)(d * 1.4426950408889634));
+}
+__device__ inline floatx4 vexp_d(doublex4 d) {
+  return floatx4{vexp_d(d.x), vexp_d(d.y), vexp_d(d.z), vexp_d(d.w)};
+}
+
+// The mantissa's log2 on the hardware log (within an ulp of a value in
+// [-1, 0), i.e. ~6e-8 absolute), the exponent added exactly in FP64.
 __device__ inline double hlog(float x) {
   int e;
   const float m = frexpf(x, &e);
-  return (double)e * 0.69314718055994530942 + (double)logf(m);
+  return ((double)e + (double)__builtin_amdgcn_logf(m)) * 0.69314718055994530942;
 }
 __device__ inline double vhlog(float x) { return hlog(x); }
 __device__ inline doublex4 vhlog(floatx4 x) {

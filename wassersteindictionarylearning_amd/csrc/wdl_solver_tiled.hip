@@ -624,7 +624,7 @@ __global__ __launch_bounds__(kThreads, 6) void k_tbwd1_log(SolveArgs a, int t) {
           } else {
             c = tile_log_result<VW>(sm.buf, sm.L, T, li);
           }
-          vst<VW>(GDp + gi, r.gd + c.s * vexpf(vf(c.x - r.c)));  // :752
+          vst<VW>(GDp + gi, r.gd + c.s * vexp_d(c.x - r.c));  // :752
         });
   } else {
     const float* Rp = (s & 1) ? a.R0 : a.R1;
@@ -716,7 +716,7 @@ __global__ __launch_bounds__(kThreads, 6) void k_tbwd2_log(SolveArgs a, int t) {
         [&](int li, int gi, bool ok, VD f) {
           if (!ok) return;
           const SLog<VW> v = tile_log_result<VW>(sm.buf, sm.L, T, li);
-          vst<VW>(Vc + gi, v.s * vexpf(vf(v.x - f)));  // -v (:762)
+          vst<VW>(Vc + gi, v.s * vexp_d(v.x - f));  // -v (:762)
         });
   } else {
     const double* Bm = (const double*)a.Bh + hslot(a, s - 2) * a.stage + img;  // log b^{s-1}
@@ -726,7 +726,7 @@ __global__ __launch_bounds__(kThreads, 6) void k_tbwd2_log(SolveArgs a, int t) {
         [&](int li, int gi, bool ok, VD b) {
           if (!ok) return;
           const SLog<VW> r = tile_log_result<VW>(sm.buf, sm.L, T, li);
-          vst<VW>(Rc + gi, r.s * vexpf(vf(r.x + b)));  // -r (:805)
+          vst<VW>(Rc + gi, r.s * vexp_d(r.x + b));  // -r (:805)
         });
   }
 }
