@@ -6,6 +6,8 @@ full-size workloads, where the numpy oracle would take hours:
   large_c3          the 256x256 curve workload (32 histograms, K=6, gamma=0.5,
                     KL loss) at its full L=200, histogram 0: dL/dlambda, the
                     barycenter and dL/datoms.
+  large_c3_more     histograms 1-3 of the same workload at L=200: dL/dlambda and
+                    the barycenter.
   large_served_<n>  one served-size request at 512x512 (L=20), 256x256 (L=300)
                     and 128x128 (L=400) against a blob dictionary, warm-started
                     from a seeded log b^0: dL/dlambda.
@@ -44,6 +46,7 @@ from wassersteindictionarylearning_amd import synthetic  # noqa: E402
 SERVED_CASES = {512: (512, 5, 1.0, 20, 512), 256: (256, 4, 2.0, 300, 256),
                 128: (128, 8, 1.0, 400, 128)}
 PROJECTION_SAMPLE = (0, 1365, 2730, 4095)   # requests of the projection workload
+C3_MORE = (1, 2, 3)                          # further histograms of the 256x256 workload
 SERVED_SAMPLE = (16, 23)                     # arrivals of the served workload (both 64x64, K=4)
 SERVED_ROUNDS, SERVED_STEP = 3, 0.5          # as the served benchmark leg runs them
 
@@ -95,6 +98,20 @@ def make_c3(ref):
     _save("large_c3", L=L, grad_lambda=o["grad_lambda"], bary=o["bary"].astype(np.float32),
           grad_dic=o["grad_dic"].astype(np.float32),
           check=checksums(c["obs"][0], c["atoms"], c["weights"][0]))
+
+
+def make_c3_more(ref):
+    c = synthetic.make_problem("log_small_eps")
+    n, K, g, L, loss = c["n"], c["K"], c["gamma"], c["L"], c["loss"]
+    gl, bary, chk = [], [], []
+    for p in C3_MORE:
+        o = ref.gradient_both(c["atoms"], c["weights"][p], c["obs"][p], n, n, g, L, loss,
+                              variant="log")
+        gl.append(o["grad_lambda"])
+        bary.append(o["bary"].astype(np.float32))
+        chk.append(checksums(c["obs"][p], c["atoms"], c["weights"][p]))
+    _save("large_c3_more", L=L, requests=np.array(C3_MORE), grad_lambda=np.array(gl),
+          bary=np.array(bary), check=np.array(chk))
 
 
 def make_served(ref, n):
@@ -163,7 +180,8 @@ def make_served_rounds(ref):
 def main(names):
     from oracle import ref
     assert ref.available(), "build oracle/_ref first: make -C oracle"
-    jobs = {"c3": lambda: make_c3(ref), "projection": lambda: make_projection(ref),
+    jobs = {"c3": lambda: make_c3(ref), "c3_more": lambda: make_c3_more(ref),
+            "projection": lambda: make_projection(ref),
             "served_rounds": lambda: make_served_rounds(ref)}
     for n in SERVED_CASES:
         jobs["served_%d" % n] = (lambda n=n: make_served(ref, n))

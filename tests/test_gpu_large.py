@@ -52,12 +52,17 @@ def test_log_config_full_length_against_reference(ctx):
     same_inputs(z, c["obs"][0], c["atoms"], c["weights"][0])
     n, K, g, L, loss = c["n"], c["K"], c["gamma"], int(z["L"]), c["loss"]
     print("log configuration %dx%d K=%d L=%d against the reference" % (n, n, K, L), flush=True)
+    zm = fixture("large_c3_more")  # histograms 1-3: dL/dlambda and the barycenter
+    P = 1 + len(zm["requests"])
+    for j, p in enumerate(zm["requests"]):
+        np.testing.assert_allclose(ML.checksums(c["obs"][p], c["atoms"], c["weights"][p]),
+                                   zm["check"][j], rtol=1e-12, atol=0)
     out = {}
     for name, f in (("full", wdl.FLAG_LOG_DOMAIN),
                     ("checkpointed", wdl.FLAG_LOG_DOMAIN | wdl.FLAG_CHECKPOINT)):
-        s = wdl.SinkhornSolver(ctx, n, n, K, 2, L, g, loss, f)
-        s.set_observed(c["obs"][:2])
-        s.set_problem(c["atoms"], c["weights"][:2])
+        s = wdl.SinkhornSolver(ctx, n, n, K, P, L, g, loss, f)
+        s.set_observed(c["obs"][:P])
+        s.set_problem(c["atoms"], c["weights"][:P])
         s.run()
         out[name] = s.results()
         out[name]["grad_dic"] = s.grad_dic()
@@ -73,6 +78,11 @@ def test_log_config_full_length_against_reference(ctx):
         e = rel(got, z[k])
         print("%s: %.2e" % (k, e), flush=True)
         assert e < TOL, k
+    for j, p in enumerate(zm["requests"]):
+        for k in ("bary", "grad_lambda"):
+            e = rel(r[k][p], zm[k][j])
+            print("histogram %d %s: %.2e" % (p, k, e), flush=True)
+            assert e < TOL, (p, k)
 
 
 @pytest.mark.parametrize("n", [128, 256, 512])

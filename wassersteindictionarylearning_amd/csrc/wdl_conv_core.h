@@ -465,7 +465,25 @@ __device__ inline float wave_min(float x) {
 // safe_log (loss.h:3-5) in FP32: log(max(1e-250, x)); 1e-250 is below the
 // FP32 range, so every x <= 0 (and NaN, as std::max returns its first
 // argument) maps to log(1e-250).
-__device__ inline float safe_logf(float x) { return x > 0.f ? logf(x) : -575.64627324851142f; }
+// log and exp of the elementwise steps on the hardware log2 / exp2, without the
+// library's range reductions.  flog adds the exact binary exponent to the
+// mantissa's log2 (in [-1, 0): within ~1e-7 absolute of log x, plus the final
+// rounding).  fexp multiplies the hardware 2^h, h = log2(e) x rounded, by
+// 1 + r ln 2, r the rounding's remainder (exact by FMA) plus the constant's own
+// error: within about 2 ulp of exp x (below 2^-126 the hardware may flush).
+__device__ inline float flog(float x) {
+  int e;
+  const float m = frexpf(x, &e);
+  return fmaf((float)e, 0.693147182f, __builtin_amdgcn_logf(m) * 0.693147182f);
+}
+__device__ inline float fexp(float x) {
+  const float h = x * 1.44269502f;
+  const float r = fmaf(x, 1.44269502f, -h) + x * 1.92596299e-8f;
+  const float e = __builtin_amdgcn_exp2f(h);
+  // at the ends of the range (2^h beyond 2^+-126, inf, 0, NaN) 2^h alone
+  return fabsf(h) < 126.f ? fmaf(e, r * 0.693147182f, e) : e;
+}
+__device__ inline float safe_logf(float x) { return x > 0.f ? flog(x) : -575.64627324851142f; }
 __device__ inline floatx4 safe_logf(floatx4 x) {
   return floatx4{safe_logf(x.x), safe_logf(x.y), safe_logf(x.z), safe_logf(x.w)};
 }
@@ -497,9 +515,9 @@ __device__ inline float fast_logf(float x) {
   const float y = small ? x * 18446744073709551616.f : x;
   return (__builtin_amdgcn_logf(y) - (small ? 64.f : 0.f)) * 0.693147180559945309f;
 }
-__device__ inline float vexpf(float x) { return expf(x); }
-__device__ inline floatx4 vexpf(floatx4 x) { return floatx4{expf(x.x), expf(x.y), expf(x.z), expf(x.w)}; }
-__device__ inline float vlogf(float x) { return logf(x); }
-__device__ inline floatx4 vlogf(floatx4 x) { return floatx4{logf(x.x), logf(x.y), logf(x.z), logf(x.w)}; }
+__device__ inline float vexpf(float x) { return fexp(x); }
+__device__ inline floatx4 vexpf(floatx4 x) { return floatx4{fexp(x.x), fexp(x.y), fexp(x.z), fexp(x.w)}; }
+__device__ inline float vlogf(float x) { return flog(x); }
+__device__ inline floatx4 vlogf(floatx4 x) { return floatx4{flog(x.x), flog(x.y), flog(x.z), flog(x.w)}; }
 
 }  // namespace wdl
