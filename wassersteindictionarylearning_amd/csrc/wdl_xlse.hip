@@ -217,17 +217,30 @@ __device__ __forceinline__ void xlse_pass_body(const double* __restrict__ in,
   // Per line slot: whether its line runs (1) or is skipped, for the store.
   unsigned char* ACT = SO0 + LW * len;
   // The next line group's values (at most kXPer per lane: lines up to 1024
-  // over 4 waves), sign bytes and activity, loaded one group ahead so the
-  // loads' latency passes under the current group's work.
+  // over 4 waves), sign bytes and iteration counts, loaded one group ahead so
+  // the loads' latency passes under the current group's work.  Lane i loads
+  // the count of line slot i % LW and the wave reads its slot's lane when the
+  // group starts: a load at the wave's own (uniform) address would be a
+  // scalar load, waited for at the next LDS access (both count on the same
+  // counter), i.e. at once.
   constexpr int kXPer = 4;
   double pv[kXPer];
   unsigned char ps[kXPer];
-  bool pact = false;
+  bool pinb = false;
+  int plp = 0;
+  // image of line ln: in 32-bit arithmetic when every line index fits
+  const bool lines32 = nlines <= 0xffffffffLL;
+  auto image_of = [&](long long ln) -> long long {
+    return lines32 ? (long long)((unsigned)ln / (unsigned)rows) : ln / rows;
+  };
   auto fetch = [&](long long l0n) {
     const long long ln = l0n + w;
     const bool inb = ln < nlines;
-    pact = inb;
-    if (inb && active_thr > 0) pact = Lp[p0 + (int)((ln / rows) / ipp)] >= active_thr;
+    pinb = inb;
+    if (active_thr > 0) {
+      const long long li = l0n + lane % LW;
+      if (li < nlines) plp = Lp[p0 + (int)image_of(li) / ipp];
+    }
 #pragma unroll
     for (int u = 0; u < kXPer; ++u) {
       const int k = sub * 64 + lane + u * wpl * 64;
@@ -241,7 +254,8 @@ __device__ __forceinline__ void xlse_pass_body(const double* __restrict__ in,
   long long l0 = (long long)blk * LW;
   if (l0 < nlines) fetch(l0);
   for (; l0 < nlines; l0 += stride) {
-    const bool active = pact;
+    const int lpw = __builtin_amdgcn_readlane(plp, w);
+    const bool active = pinb && (active_thr <= 0 || lpw >= active_thr);
     if (active) {  // stage the line: 64 values per wave and step
 #pragma unroll
       for (int u = 0; u < kXPer; ++u) {
@@ -310,20 +324,19 @@ __device__ __forceinline__ void xlse_pass_body(const double* __restrict__ in,
     __syncthreads();
     // transposed store: the lines of one workgroup are consecutive, so each
     // output position j gets up to LW adjacent values.  Thread i serves
-    // line slot i % LW throughout (blockDim is a multiple of LW), so
-    // the line's offset and activity are worked out once per line group.
+    // line slot i % LW throughout (blockDim is a multiple of LW), so the
+    // line's image, offset and output stage (epi.line: its pointers and
+    // per-image values) are worked out once per line group.
     {
       const int ws = threadIdx.x % LW;
       const long long ln = l0 + ws;
-      const bool act = ACT[ws] != 0;  // implies ln < nlines
-      long long ob = 0;
-      if (act) {
-        const long long img = ln / rows;
-        ob = img * (long long)rows * len + (ln - img * rows);
-      }
-      if (act)
+      if (ACT[ws] != 0) {  // implies ln < nlines
+        const long long img = image_of(ln);
+        const int r = (int)(ln - img * rows);
+        const auto out = epi.line(img, (long long)rows * len);
         for (int j = threadIdx.x / LW; j < len; j += blockDim.x / LW)
-          epi(ob + (long long)j * rows, O0[ws * len + j], SIGNED ? SO0[ws * len + j] : 1);
+          out(r + j * rows, O0[ws * len + j], SIGNED ? SO0[ws * len + j] : 1);
+      }
     }
     __syncthreads();
   }
@@ -338,14 +351,22 @@ __global__ __launch_bounds__(kXMaxThreads) void k_xlse_pass(
                                   active_thr, blockIdx.x, gridDim.x);
 }
 
-// Output stage of k_xlse_pass, called once per output (index oi into the
-// image layout of the pass, value, sign byte): the plain stage stores them.
+// Output stage of k_xlse_pass: line(img, N) gives, for image img (of N
+// values) of the pass, the stage of that image's outputs, called once per
+// output (pixel x, value, sign byte).  The plain stage stores them.
 struct XStore {
   double* out;
   unsigned char* sout;  // nullptr without signs
-  __device__ void operator()(long long oi, double v, unsigned char sg) const {
-    out[oi] = v;
-    if (sout) sout[oi] = sg;
+  struct Line {
+    double* o;
+    unsigned char* so;
+    __device__ void operator()(int x, double v, unsigned char sg) const {
+      o[x] = v;
+      if (so) so[x] = sg;
+    }
+  };
+  __device__ Line line(long long img, long long N) const {
+    return Line{out + img * N, sout ? sout + img * N : nullptr};
   }
 };
 
@@ -440,10 +461,17 @@ struct XEpiFwd1 {
   const float* logA;
   int K;
   long long N;
-  __device__ void operator()(long long oi, double c, unsigned char) const {
-    const long long img = oi / N, x = oi - img * N;
-    ch[oi] = c;
-    xa[oi] = (double)logA[(img % K) * N + x] - c;
+  struct Line {
+    double* ch;
+    double* xa;
+    const float* la;
+    __device__ void operator()(int x, double c, unsigned char) const {
+      ch[x] = c;
+      xa[x] = (double)la[x] - c;
+    }
+  };
+  __device__ Line line(long long img, long long) const {
+    return Line{ch + img * N, xa + img * N, logA + (long long)((int)img % K) * N};
   }
 };
 
@@ -454,24 +482,42 @@ struct XEpiBwd1 {
   SolveArgs a;
   int t, nch;
   long long off;  // scratch offset of the first image
-  __device__ void operator()(long long oi, double c, unsigned char sgb) const {
-    const long long N = a.N, img = oi / N, x = oi - img * N;
-    const int i = (int)(img % a.K);
-    const long long pc = img / a.K;
-    const int chain = nch == 1 ? 1 : (int)(pc % nch);
-    const int p = a.p0 + (int)(pc / nch);
+  struct Line {
+    const double* cs;  // log c^{s-1} of the image
+    float* gd;         // resultDic of the image (chain 0), else nullptr
+    double* xa;
+    unsigned char* sga;
+    const float* la;   // log p of the atom
+    bool first;        // the histogram's last step: resultDic starts at 0
+    bool more;         // s >= 2: the second convolution follows
+    __device__ void operator()(int x, double c, unsigned char sgb) const {
+      const double csx = cs[x];
+      if (gd) {
+        const float prev = first ? 0.f : gd[x];
+        gd[x] = prev + (sgb ? 1.f : -1.f) * expf((float)(c - csx));
+      }
+      if (more) {
+        xa[x] = (double)la[x] + c - 2.0 * csx;
+        sga[x] = sgb;
+      }
+    }
+  };
+  __device__ Line line(long long img, long long) const {
+    const long long N = a.N;
+    const int im = (int)img, i = im % a.K, pc = im / a.K;  // images: fewer than 2^31
+    const int chain = nch == 1 ? 1 : pc % nch;
+    const int p = a.p0 + pc / nch;
     const int s = bwd_step(a, p, t);
     const long long pimg = ((long long)p * a.K + i) * N;
-    const double cs = ((const double*)a.Ch)[hslot(a, s - 1) * a.stage + pimg + x];
-    if (chain == 0) {
-      float* gd = a.GD + pimg + x;
-      const float prev = s == steps_of(a, p) ? 0.f : *gd;
-      *gd = prev + (sgb ? 1.f : -1.f) * expf((float)(c - cs));
-    }
-    if (s > 1) {
-      a.xa[off + oi] = (double)a.logA[(long long)i * N + x] + c - 2.0 * cs;
-      a.sga[off + oi] = sgb;
-    }
+    Line l;
+    l.cs = (const double*)a.Ch + hslot(a, s - 1) * a.stage + pimg;
+    l.gd = chain == 0 ? a.GD + pimg : nullptr;
+    l.first = s == steps_of(a, p);
+    l.more = s > 1;
+    l.xa = a.xa + off + img * N;
+    l.sga = a.sga + off + img * N;
+    l.la = a.logA + (long long)i * N;
+    return l;
   }
 };
 
@@ -480,22 +526,24 @@ struct XEpiBwd1 {
 struct XEpiBwd2 {
   SolveArgs a;
   int t, nch;
-  __device__ void operator()(long long oi, double v, unsigned char sgb) const {
-    const long long N = a.N, img = oi / N, x = oi - img * N;
-    const int i = (int)(img % a.K);
-    const long long pc = img / a.K;
-    const int chain = nch == 1 ? 1 : (int)(pc % nch);
-    const int p = a.p0 + (int)(pc / nch);
+  struct Line {
+    const double* src;  // log phi^{s-1} (chain 0) or log b^{s-1} (chain 1) of the image
+    float* dst;         // -v or -r of the image
+    double sgn;         // -1 (chain 0: res - log phi) or +1 (chain 1: res + log b)
+    __device__ void operator()(int x, double v, unsigned char sgb) const {
+      dst[x] = (sgb ? 1.f : -1.f) * expf((float)(v + sgn * src[x]));
+    }
+  };
+  __device__ Line line(long long img, long long) const {
+    const long long N = a.N;
+    const int im = (int)img, i = im % a.K, pc = im / a.K;  // images: fewer than 2^31
+    const int chain = nch == 1 ? 1 : pc % nch;
+    const int p = a.p0 + pc / nch;
     const int s = bwd_step(a, p, t);
     const long long pimg = ((long long)p * a.K + i) * N;
-    const float sg = sgb ? 1.f : -1.f;
-    if (chain == 0) {
-      const double f = ((const double*)a.Fh)[hslot(a, s - 2) * a.stage + pimg + x];
-      ((s & 1) ? a.V1 : a.V0)[pimg + x] = sg * expf((float)(v - f));
-    } else {
-      const double b = ((const double*)a.Bh)[hslot(a, s - 2) * a.stage + pimg + x];
-      ((s & 1) ? a.R1 : a.R0)[pimg + x] = sg * expf((float)(v + b));
-    }
+    const long long h = hslot(a, s - 2) * a.stage + pimg;
+    if (chain == 0) return Line{(const double*)a.Fh + h, ((s & 1) ? a.V1 : a.V0) + pimg, -1.0};
+    return Line{(const double*)a.Bh + h, ((s & 1) ? a.R1 : a.R0) + pimg, 1.0};
   }
 };
 
