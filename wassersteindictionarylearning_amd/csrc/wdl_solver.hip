@@ -833,8 +833,15 @@ struct wdl_solver {
   bool tiled = false;  // 64 x 64 tiles with a halo (a side above kMaxSide)
   bool exact = false;  // log domain with the exact-shift line passes (WDL_FLAG_EXACT_LSE)
   TileGeom tg{};
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  // The batched run as two captured graphs: [0] the first forward steps
+  // (none when L < 8), [1] the rest.
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  bool captured = false;
+  // Steps enqueue() emits: its launch sequence is numbered (forward steps,
+  // the loss, backward steps or segments, the reductions) and only numbers in
+  // [op_lo, op_hi) are launched.
+  int op_lo = 0, op_hi = std::numeric_limits<int>::max();
   bool observed = false, problem = false, ran = false;
   double scaleDic = 1.0;
   int expWeight = 1;
@@ -931,15 +938,17 @@ struct wdl_solver {
   // The captured graph holds the launch shape; a new shape captures again.
   int drop_graph() {
     WDL_HIP(hipStreamSynchronize(ctx->stream));
-    if (exec) {
-      (void)hipGraphExecDestroy(exec);
-      exec = nullptr;
-    }
-    if (graph) {
-      (void)hipGraphDestroy(graph);
-      graph = nullptr;
-    }
+    release_graphs();
     return WDL_OK;
+  }
+  void release_graphs() {
+    for (int k = 0; k < 2; ++k) {
+      if (exec[k]) (void)hipGraphExecDestroy(exec[k]);
+      if (graph[k]) (void)hipGraphDestroy(graph[k]);
+      exec[k] = nullptr;
+      graph[k] = nullptr;
+    }
+    captured = false;
   }
 
   int set_groups(int n) {
@@ -974,7 +983,8 @@ struct wdl_solver {
   // All P solves; with several groups, fork onto the side streams and join
   // back on the context stream before the final reduction.
   int enqueue_batched() {
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, ctx->stream, events.as<unsigned>());
+    if (op_lo == 0)
+      hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, ctx->stream, events.as<unsigned>());
     if (ngroups <= 1) return enqueue(0, P, true, ctx->stream);
     WDL_HIP(hipEventRecord(fork, ctx->stream));
     int st;
@@ -985,7 +995,8 @@ struct wdl_solver {
       WDL_HIP(hipEventRecord(joins[g], side[g]));
       WDL_HIP(hipStreamWaitEvent(ctx->stream, joins[g], 0));
     }
-    return (flags & WDL_FLAG_WEIGHTS_ONLY) ? WDL_OK : reduce_gd(ctx->stream);
+    const bool last = op_hi == std::numeric_limits<int>::max();
+    return (!last || (flags & WDL_FLAG_WEIGHTS_ONLY)) ? WDL_OK : reduce_gd(ctx->stream);
   }
 
   int reduce_gd(hipStream_t st) {
@@ -1019,7 +1030,13 @@ struct wdl_solver {
     const size_t esz = logd ? sizeof(double) : sizeof(float);
     const long long gofs = (long long)p0 * K * N, gn = (long long)np * K * N;  // the group's values
     const dim3 cgrid((unsigned)std::min<long long>((gn + 255) / 256, 4096));
+    int op = 0;
+    auto take = [&]() {
+      const int o = op++;
+      return o >= op_lo && o < op_hi;
+    };
     for (int s = 1; s <= L; ++s) {
+      if (!take()) continue;
       if ((e = forward(a, s))) return e;
       if (ring && s % seg == 0 && s < L) {  // checkpoint phi^s
         char* src = (char*)Fh.p + (size_t)(hslot(a, s - 1) * a.stage + gofs) * esz;
@@ -1034,14 +1051,16 @@ struct wdl_solver {
     }
     // kLossBlocks blocks per histogram: every lossPart slot is written
     // (blocks past the last pixel write 0).
-    if (a.logA)
-      hipLaunchKernelGGL(k_final_log, dim3(kLossBlocks, np), blk, 0, st, a);
-    else
-      hipLaunchKernelGGL(k_final, dim3(kLossBlocks, np), blk, 0, st, a);
-    if (loss == WDL_LOSS_WASSERSTEIN && (e = wasserstein_loss(a, st))) return e;
+    if (take()) {
+      if (a.logA)
+        hipLaunchKernelGGL(k_final_log, dim3(kLossBlocks, np), blk, 0, st, a);
+      else
+        hipLaunchKernelGGL(k_final, dim3(kLossBlocks, np), blk, 0, st, a);
+      if (loss == WDL_LOSS_WASSERSTEIN && (e = wasserstein_loss(a, st))) return e;
+    }
     if (!ring) {
       for (int s = L; s >= 1; --s)
-        if ((e = backward(s))) return e;
+        if (take() && (e = backward(s))) return e;
     } else {
       // Segments (c, c + seg] from the last: the ring still holds the last
       // segment of every histogram after the forward (a histogram's last
@@ -1051,7 +1070,7 @@ struct wdl_solver {
       const int nseg = (L + seg - 1) / seg;
       for (int k = nseg - 1; k >= 0; --k) {
         const int c = k * seg, end = std::min(c + seg, L);
-        if (end < L) {
+        if (end < L && take()) {
           SolveArgs r = a;
           r.rmin = end + 1;
           r.events = nullptr;  // the first pass counted these lines
@@ -1070,9 +1089,10 @@ struct wdl_solver {
             if ((e = forward(r, s))) return e;
         }
         for (int s = end; s > c; --s)
-          if ((e = backward(s))) return e;
+          if (take() && (e = backward(s))) return e;
       }
     }
+    if (!take()) return WDL_OK;
     if (tiled && !exact && (e = tiled_reduce_gl(a, st))) return e;
     WDL_HIP(hipGetLastError());
     return (reduce && !(flags & WDL_FLAG_WEIGHTS_ONLY)) ? reduce_gd(st) : WDL_OK;
@@ -1343,8 +1363,7 @@ int wdl_solver_destroy(wdl_solver* s) {
   if (!s) return WDL_OK;
   (void)hipSetDevice(s->ctx->device);
   (void)hipStreamSynchronize(s->ctx->stream);
-  if (s->exec) (void)hipGraphExecDestroy(s->exec);
-  if (s->graph) (void)hipGraphDestroy(s->graph);
+  s->release_graphs();
   s->release_streams();
   DevBuf* bufs[] = {&s->tab, &s->A,  &s->lam, &s->obs, &s->Bh, &s->Qh, &s->Ch,  &s->Fh,
                     &s->V0,  &s->V1, &s->R0,  &s->R1,  &s->GD, &s->GL,       &s->Q,
@@ -1499,20 +1518,43 @@ int wdl_solver_run(wdl_solver* s, int mode) {
   } else if (s->flags & WDL_FLAG_NO_GRAPH) {
     if ((st = s->enqueue_batched())) return st;
   } else {
-    if (!s->exec) {
-      WDL_HIP(hipStreamBeginCapture(s->ctx->stream, hipStreamCaptureModeThreadLocal));
-      st = s->enqueue_batched();
-      hipGraph_t g = nullptr;
-      hipError_t e = hipStreamEndCapture(s->ctx->stream, &g);
-      if (st) return st;
-      if (e != hipSuccess) {
-        set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-        return WDL_ERR_HIP;
+    if (!s->captured) {
+      // Two graphs, the first L/4 forward steps and the rest: a graph's first
+      // kernel starts only after the runtime has prepared much of the graph
+      // (at L = 100 with two histogram groups, ~0.6 ms of the ~1.2 ms the
+      // launch call takes), so a small first graph starts at once and the
+      // second is prepared while it runs.  A run waited for at once (an
+      // L-BFGS evaluation, a projection round) gains that time.
+      const int split = s->L >= 8 ? s->L / 4 : 0;
+      const int imax = std::numeric_limits<int>::max();
+      for (int k = split ? 0 : 1; k < 2; ++k) {
+        s->op_lo = k == 0 ? 0 : split;
+        s->op_hi = k == 0 ? split : imax;
+        WDL_HIP(hipStreamBeginCapture(s->ctx->stream, hipStreamCaptureModeThreadLocal));
+        st = s->enqueue_batched();
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(s->ctx->stream, &g);
+        s->op_lo = 0;
+        s->op_hi = imax;
+        if (st || e != hipSuccess) {
+          if (g) (void)hipGraphDestroy(g);
+          s->release_graphs();
+          if (st) return st;
+          set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+          return WDL_ERR_HIP;
+        }
+        s->graph[k] = g;
+        const hipError_t ie = hipGraphInstantiate(&s->exec[k], g, nullptr, nullptr, 0);
+        if (ie != hipSuccess) {
+          s->release_graphs();
+          set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+          return WDL_ERR_HIP;
+        }
       }
-      s->graph = g;
-      WDL_HIP(hipGraphInstantiate(&s->exec, s->graph, nullptr, nullptr, 0));
+      s->captured = true;
     }
-    WDL_HIP(hipGraphLaunch(s->exec, s->ctx->stream));
+    for (int k = 0; k < 2; ++k)
+      if (s->exec[k]) WDL_HIP(hipGraphLaunch(s->exec[k], s->ctx->stream));
   }
   s->ran = true;
   return WDL_OK;
