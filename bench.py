@@ -290,7 +290,8 @@ def uneven_leg(reps=3, seed=11):
             s = wdl.SinkhornSolver(c, n, n, K, P, 100, g, wdl.LOSS_QUADRATIC, flags)
             s.set_observed(obs)
             s.set_problem(atoms, np.full((P, K), 1.0 / K))
-            s.set_iterations(it)
+            for p in range(P):  # iterations and the served loop's mirror-descent step
+                s.set_slot(p, None, None, int(it[p]), 0.5)
             s.set_streams(1)
             ss.append(s)
         return ss

@@ -231,7 +231,9 @@ int wdl_solver_set_slot(wdl_solver* s, int slot, const double* observed, const d
  * WDL_FLAG_WARM_RESTART): the batched weights-only solve (:635-673) seeded from the
  * warm-restart store, then lambda_i <- lambda_i exp(-step g_i / max|g|) / Z with
  * g = resultLa - <lambda, resultLa> (the projection of :1095-1099), then the
- * per-histogram warm commit b_storage <- b_temp (:418-422).  Asynchronous. */
+ * per-histogram warm commit b_storage <- b_temp (:418-422).  A slot whose step
+ * was never set (wdl_solver_set_slot) steps 0: its weights are only renormalised.
+ * Asynchronous. */
 int wdl_solver_projection_round(wdl_solver* s);
 /* Histogram `slot`'s current weights (K) and the loss of its last run.  Synchronous. */
 int wdl_solver_get_slot(wdl_solver* s, int slot, double* lambdas, double* loss);

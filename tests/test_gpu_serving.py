@@ -74,6 +74,36 @@ def test_projection_rounds_match_oracle(ctx, logd):
     s.close()
 
 
+def test_projection_round_without_steps_renormalises_only(ctx):
+    """Slots whose mirror-descent step was never set (set_iterations only) step
+    0: a projection round only renormalises their weights, on every solver --
+    also one created where an earlier solver's steps lay in device memory."""
+    W = H = 24
+    K, P, L, g = 3, 4, 12, 1.0
+    print("projection round without steps", flush=True)
+    atoms = dictionary(W, H, K, 1)
+    obs = observed(W, H, P, 2)
+    lam0 = np.random.default_rng(5).uniform(0.2, 1, (P, K))
+    iters = np.array([12, 8, 10, 6], np.int32)
+    flags = POOL_FLAGS | wdl.FLAG_LOG_DOMAIN
+    out = []
+    for trial in range(3):
+        s = wdl.SinkhornSolver(ctx, W, H, K, P, L, g, 2, flags)
+        s.set_observed(obs)
+        s.set_problem(atoms, lam0)
+        if trial == 0:  # leaves steps of 0.5 in the memory the next solvers get
+            for p in range(P):
+                s.set_slot(p, None, None, int(iters[p]), 0.5)
+        else:
+            s.set_iterations(iters)
+        s.projection_round()
+        out.append(np.array([s.get_slot(p)[0] for p in range(P)]))
+        s.close()
+    assert not np.allclose(out[0], lam0 / lam0.sum(1, keepdims=True), rtol=1e-3)
+    np.testing.assert_array_equal(out[1], out[2])
+    np.testing.assert_allclose(out[1], lam0 / lam0.sum(1, keepdims=True), rtol=1e-6)
+
+
 def serve(requests, dicts, budget, max_slots=8, max_iterations=64, mixed=False):
     """Runs the requests [(dict index, histogram, iterations, rounds, step)] through a
     fresh server; returns [(weights, loss)] and its stats.  A dictionary is

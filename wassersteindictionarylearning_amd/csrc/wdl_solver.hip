@@ -1321,9 +1321,11 @@ int wdl_solver_create(wdl_ctx* ctx, int W, int H, int K, int P, int L, double ga
     if (e == hipSuccess) e = hipMemcpy(s->lmax.p, &L, sizeof(int), hipMemcpyHostToDevice);
   }
   // Outputs start at zero, so a histogram left idle (0 iterations) reads 0
-  // until it first runs, and adds 0 to the summed atom gradient.
+  // until it first runs, and adds 0 to the summed atom gradient; so do the
+  // mirror-descent steps, so a projection round over slots whose step was
+  // never set only renormalises their weights.
   for (DevBuf* b : {&s->GD, &s->GL, &s->Q, &s->G0, &s->lossPart, &s->gsumPart, &s->GDsum,
-                    &s->bTemp})
+                    &s->bTemp, &s->eta})
     if (e == hipSuccess && b->bytes) e = hipMemsetAsync(b->p, 0, b->bytes, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e == hipSuccess && (flags & WDL_FLAG_WARM_RESTART)) {
@@ -1982,17 +1984,29 @@ struct wdl_group {
   GroupArgs ga{};
   GroupShape sh{};
   bool projection = false;  // every member weights-only with a warm-restart store
-  hipGraph_t graph[2] = {nullptr, nullptr};  // [0] run, [1] projection round
-  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  // [kind][part]: kind 0 run, 1 projection round; part 0 the first Lmax/4
+  // forward launches (none when Lmax < 8), part 1 the rest -- a small first
+  // graph starts at once and the second is prepared while it runs.
+  hipGraph_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+  hipGraphExec_t exec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+  bool captured[2] = {false, false};
 
-  int enqueue(bool proj) {
+  // The launch sequence numbered (forward steps, the final stage, backward
+  // steps, the reductions / updates); only numbers in [lo, hi) are launched.
+  int enqueue(bool proj, int lo, int hi) {
     hipStream_t st = ctx->stream;
-    int e;
+    int e, op = 0;
+    auto take = [&]() {
+      const int o = op++;
+      return o >= lo && o < hi;
+    };
     for (int s = 1; s <= sh.Lmax; ++s)
-      if ((e = exact_group_forward(ga, sh, st, s))) return e;
-    hipLaunchKernelGGL(k_final_log_group, dim3(sh.grid[GK_FINAL]), dim3(kThreads), 0, st, ga);
+      if (take() && (e = exact_group_forward(ga, sh, st, s))) return e;
+    if (take())
+      hipLaunchKernelGGL(k_final_log_group, dim3(sh.grid[GK_FINAL]), dim3(kThreads), 0, st, ga);
     for (int t = sh.Lmax; t >= 1; --t)
-      if ((e = exact_group_backward(ga, sh, st, t))) return e;
+      if (take() && (e = exact_group_backward(ga, sh, st, t))) return e;
+    if (!take()) return WDL_OK;
     if (!sh.weightsOnly)
       hipLaunchKernelGGL(k_reduce_gd_group, dim3(sh.grid[GK_RGD]), dim3(256), 0, st, ga);
     if (proj) {
@@ -2003,25 +2017,45 @@ struct wdl_group {
     return WDL_OK;
   }
 
+  void release_graphs(int k) {
+    for (int h = 0; h < 2; ++h) {
+      if (exec[k][h]) (void)hipGraphExecDestroy(exec[k][h]);
+      if (graph[k][h]) (void)hipGraphDestroy(graph[k][h]);
+      exec[k][h] = nullptr;
+      graph[k][h] = nullptr;
+    }
+    captured[k] = false;
+  }
+
   int launch(bool proj) {
     const int k = proj ? 1 : 0;
-    if (!exec[k]) {
-      WDL_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-      const int st = enqueue(proj);
-      hipGraph_t gr = nullptr;
-      const hipError_t e = hipStreamEndCapture(ctx->stream, &gr);
-      if (st) {
-        if (gr) (void)hipGraphDestroy(gr);
-        return st;
+    if (!captured[k]) {
+      const int split = sh.Lmax >= 8 ? sh.Lmax / 4 : 0;
+      for (int h = split ? 0 : 1; h < 2; ++h) {
+        WDL_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        const int st = enqueue(proj, h == 0 ? 0 : split,
+                               h == 0 ? split : std::numeric_limits<int>::max());
+        hipGraph_t gr = nullptr;
+        const hipError_t e = hipStreamEndCapture(ctx->stream, &gr);
+        if (st || e != hipSuccess) {
+          if (gr) (void)hipGraphDestroy(gr);
+          release_graphs(k);
+          if (st) return st;
+          set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+          return WDL_ERR_HIP;
+        }
+        graph[k][h] = gr;
+        const hipError_t ie = hipGraphInstantiate(&exec[k][h], gr, nullptr, nullptr, 0);
+        if (ie != hipSuccess) {
+          release_graphs(k);
+          set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+          return WDL_ERR_HIP;
+        }
       }
-      if (e != hipSuccess) {
-        set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-        return WDL_ERR_HIP;
-      }
-      graph[k] = gr;
-      WDL_HIP(hipGraphInstantiate(&exec[k], graph[k], nullptr, nullptr, 0));
+      captured[k] = true;
     }
-    WDL_HIP(hipGraphLaunch(exec[k], ctx->stream));
+    for (int h = 0; h < 2; ++h)
+      if (exec[k][h]) WDL_HIP(hipGraphLaunch(exec[k][h], ctx->stream));
     for (wdl_solver* m : ms) m->ran = true;
     return WDL_OK;
   }
@@ -2123,10 +2157,7 @@ int wdl_group_destroy(wdl_group* g) {
   if (!g) return WDL_OK;
   (void)hipSetDevice(g->ctx->device);
   (void)hipStreamSynchronize(g->ctx->stream);
-  for (int k = 0; k < 2; ++k) {
-    if (g->exec[k]) (void)hipGraphExecDestroy(g->exec[k]);
-    if (g->graph[k]) (void)hipGraphDestroy(g->graph[k]);
-  }
+  for (int k = 0; k < 2; ++k) g->release_graphs(k);
   g->dev.release();
   delete g;
   return WDL_OK;
