@@ -62,6 +62,12 @@ typedef struct wdl_solver wdl_solver;
 
 const char* wdl_last_error(void);
 const char* wdl_version(void);
+/* Debug: while on (nonzero), every solver created fills the device buffers it
+ * does not initialise itself (histories, scratch images, steps not yet set ...)
+ * with 0xFF bytes (NaN in FP32 and FP64) before use, so a read of memory no
+ * launch has written shows in the results.  No counterpart in the reference;
+ * for tests.  Returns the previous setting. */
+int wdl_debug_fill(int on);
 
 /* ---- context: one device, one stream --------------------------------------- */
 int wdl_ctx_create(int device, wdl_ctx** out);

@@ -40,6 +40,7 @@ _i = ctypes.c_int
 _SIGS = {
     "wdl_last_error": (ctypes.c_char_p, []),
     "wdl_version": (ctypes.c_char_p, []),
+    "wdl_debug_fill": (_i, [_i]),
     "wdl_ctx_create": (_i, [_i, ctypes.POINTER(_vp)]),
     "wdl_ctx_destroy": (_i, [_vp]),
     "wdl_ctx_stream": (_vp, [_vp]),
@@ -583,6 +584,13 @@ class ProjectionServer:
                 self.close()
             except Exception:
                 pass
+
+
+def debug_fill(on: bool) -> bool:
+    """While on, solvers created fill the device buffers they do not initialise
+    with NaN bytes (wdl_debug_fill): a read of unwritten memory shows in the
+    results.  Returns the previous setting."""
+    return bool(lib().wdl_debug_fill(1 if on else 0))
 
 
 def export_histograms(folder, names, histograms, fmt="csv"):

@@ -1189,6 +1189,10 @@ long long wdl_solver_footprint(int W, int H, int K, int P, int L, int flags) {
   return footprint(W, H, K, P, L, flags);
 }
 
+static std::atomic<int> g_debug_fill{0};
+
+int wdl_debug_fill(int on) { return g_debug_fill.exchange(on ? 1 : 0); }
+
 int wdl_solver_create(wdl_ctx* ctx, int W, int H, int K, int P, int L, double gamma, int loss,
                       int flags, wdl_solver** out) {
   WDL_CHECK(ctx && out, WDL_ERR_ARG, "wdl_solver_create: NULL argument");
@@ -1313,6 +1317,14 @@ int wdl_solver_create(wdl_ctx* ctx, int W, int H, int K, int P, int L, double ga
       wdl_solver_destroy(s);
       return st;
     }
+  }
+  if (g_debug_fill.load()) {  // wdl_debug_fill: NaN bytes until written
+    for (auto& al : allocs)
+      if (al.b->bytes && hipMemset(al.b->p, 0xFF, al.b->bytes) != hipSuccess) {
+        wdl_solver_destroy(s);
+        set_error("wdl_solver_create: debug fill failed");
+        return WDL_ERR_HIP;
+      }
   }
   hipError_t e = hipMemcpy(s->tab.p, tab.data(), tab.size() * f4, hipMemcpyHostToDevice);
   if (e == hipSuccess) {
