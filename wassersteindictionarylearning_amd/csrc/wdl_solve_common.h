@@ -284,6 +284,7 @@ int wasserstein_loss(const SolveArgs& a, hipStream_t st);
 constexpr int kXMaxLen = 1024;  // longest line of the exact-shift passes
 constexpr int kXWaves = 4;  // lines per workgroup by default (4 or 8; see xlse_lines)
 constexpr int kXPad = 32;         // -inf positions on both sides of a staged line
+constexpr int kXBPad = 16;        // -inf positions on both sides of its prefix / suffix maxima
 
 // Dynamic LDS of k_xlse_pass: per line its values (FP64 scaled by log2 e, FP32),
 // input signs, prefix and suffix maxima, outputs and output signs; per
@@ -293,8 +294,8 @@ constexpr int kXPad = 32;         // -inf positions on both sides of a staged li
 constexpr int kXSym = 2 * kXPad + 1;
 __host__ __device__ inline int xlse_tab_len(int len) { return len + 3; }
 __host__ __device__ inline int xlse_lds_bytes(int len, int lw = kXWaves) {
-  const int lp = len + 2 * kXPad;
-  return lw * (lp * 8 + len * 8 + lp * 4 * 2 + len * 4 * 2 + len) +
+  const int lp = len + 2 * kXPad, lb = len + 2 * kXBPad;
+  return lw * (lp * 8 + len * 8 + lp * 4 * 2 + lb * 4 * 2 + len) +
          (xlse_tab_len(len) + kXSym) * 12 + 64;
 }
 

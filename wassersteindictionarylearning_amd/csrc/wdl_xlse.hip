@@ -113,34 +113,43 @@ __device__ inline float xlse_window_sum(const double* X2, const float* SG, const
 // so the distances 4, 8, 12, 16 that pass form a prefix: their count locates
 // the first failing multiple of 4, a test at 2 below it the first failing
 // even distance d, and one at d - 1 the first failing distance; the radius is
-// the distance before it.  Windows wider than 16 continue in steps of 2.
-__device__ inline int xlse_radius(const float* B, const float* CF, int jc, int dir, int lim,
-                                  float thr, float igf) {
-  // the four reads are independent (distances clamped to lim stay in the line
-  // and fail anyway), so they are issued together
-  float b[4];
+// the distance before it.  Windows wider than 16 continue in steps of 2.  B
+// holds -inf for kXBPad positions past either end of the line, so a distance
+// beyond it fails as a distance whose bound is too low does: no clamping to
+// lim below 18 (the penalties are formed as the table's are, (float)(d d) / gamma).
+__device__ inline int xlse_radius(const float* B, int jc, int dir, int lim, float thr, float igf) {
+  auto pen = [&](int d) { return (float)(d * d) * igf; };
+  float b[4];  // independent reads, issued together
 #pragma unroll
-  for (int i = 0; i < 4; ++i) b[i] = B[jc + dir * min(4 * i + 4, lim)];
+  for (int i = 0; i < 4; ++i) b[i] = B[jc + dir * (4 * i + 4)];
   int cnt = 0;
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int d = 4 * i + 4;
-    cnt += (int)((d <= lim) & (b[i] - (float)(d * d) * igf > thr));
-  }
+  for (int i = 0; i < 4; ++i) cnt += (int)(b[i] - pen(4 * i + 4) > thr);
   int d;
   if (cnt < 4) {
     d = 4 * cnt + 4;  // first failing multiple of 4
-    const int h = d - 2, hh = min(h, lim);
-    if (!((h <= lim) & (B[jc + dir * hh] - CF[hh] > thr))) d = h;
+    const int h = d - 2;
+    if (!(B[jc + dir * h] - pen(h) > thr)) d = h;
   } else {
     d = 18;
-    while (d <= lim && B[jc + dir * d] - CF[d] > thr) d += 2;
+    while (d <= lim && B[jc + dir * d] - pen(d) > thr) d += 2;
   }
-  // d fails and d - 2 passes (or is 0, the centre); d - 1 decides
-  const int o = d - 1, oo = min(max(o, 0), lim);
-  if (o >= 1 && !((o <= lim) & (B[jc + dir * oo] - CF[oo] > thr))) d = o;
+  // d fails and d - 2 passes (or is 0, the centre); d - 1 decides (d - 1 is at
+  // most lim + 1: within the padding)
+  const int o = d - 1;
+  if (!(B[jc + dir * o] - pen(o) > thr)) d = o;
   return min(d - 1, lim);
 }
+
+// Maximum of two pairs of 16-bit counts packed in an int (the window's left
+// and right radii, scanned together).
+struct PkMaxU16 {
+  typedef unsigned short us2 __attribute__((ext_vector_type(2)));
+  __device__ int operator()(int a, int b) const {
+    return __builtin_bit_cast(int, __builtin_elementwise_max(__builtin_bit_cast(us2, a),
+                                                             __builtin_bit_cast(us2, b)));
+  }
+};
 
 constexpr int kXMaxThreads = 1024;
 
@@ -182,7 +191,7 @@ __device__ __forceinline__ void xlse_pass_body(const double* __restrict__ in,
   const float igf = (float)inv_gamma;
   const double kLog2e = 1.4426950408889634;
   // layout: X2 [LW][lp], O0 [LW][len], C2 [nt], C2s [kXSym] (doubles);
-  // XF, SG [LW][lp], PM, SM [LW][len], CF [nt], CFs [kXSym] (floats);
+  // XF, SG [LW][lp], PM, SM [LW][lb], CF [nt], CFs [kXSym] (floats);
   // SO0 [LW][len]
   const int nt = xlse_tab_len(len);
   double* X2 = xs + w * lp + kXPad;
@@ -193,9 +202,10 @@ __device__ __forceinline__ void xlse_pass_body(const double* __restrict__ in,
   float* F0 = reinterpret_cast<float*>(C2 + nt + kXSym);
   float* XF = F0 + w * lp + kXPad;
   float* SG = F0 + LW * lp + w * lp + kXPad;
-  float* PM = F0 + 2 * LW * lp + w * len;
-  float* SM = F0 + 2 * LW * lp + LW * len + w * len;
-  float* CF = F0 + 2 * LW * lp + 2 * LW * len;
+  const int lb = len + 2 * kXBPad;
+  float* PM = F0 + 2 * LW * lp + w * lb + kXBPad;
+  float* SM = F0 + 2 * LW * lp + LW * lb + w * lb + kXBPad;
+  float* CF = F0 + 2 * LW * lp + 2 * LW * lb;
   float* CFs = CF + nt + kXPad;
   unsigned char* SO0 = reinterpret_cast<unsigned char*>(CF + nt + kXSym);
   unsigned char* SO = SO0 + w * len;
@@ -208,12 +218,14 @@ __device__ __forceinline__ void xlse_pass_body(const double* __restrict__ in,
     C2s[d] = (double)d * d * (inv_gamma * kLog2e);
     CFs[d] = (float)(d * d) * igf;
   }
-  // padding: the window loops read up to kXPad positions past either end
+  // padding: the window loops read up to kXPad positions past either end,
+  // the window search up to kXBPad
   if (sub == 0 && lane < kXPad) {
     X2[-1 - lane] = X2[len + lane] = -(double)INFINITY;
     XF[-1 - lane] = XF[len + lane] = -INFINITY;
     SG[-1 - lane] = SG[len + lane] = 0.f;
   }
+  if (sub == 0 && lane < kXBPad) PM[-1 - lane] = PM[len + lane] = SM[-1 - lane] = SM[len + lane] = -INFINITY;
   // Per line slot: whether its line runs (1) or is skipped, for the store.
   unsigned char* ACT = SO0 + LW * len;
   // The next line group's values (at most kXPer per lane: lines up to 1024
@@ -303,11 +315,13 @@ __device__ __forceinline__ void xlse_pass_body(const double* __restrict__ in,
 #pragma unroll
           for (int t = 1; t <= 2; ++t) m0 = fmaxf(m0, fmaxf(XF[jc - t], XF[jc + t]) - CF[t]);
           const float thr = m0 - kXMargin;
-          dr = xlse_radius(SM, CF, jc, 1, len - 1 - jc, thr, igf);
-          dl = xlse_radius(PM, CF, jc, -1, jc, thr, igf);
+          dr = xlse_radius(SM, jc, 1, len - 1 - jc, thr, igf);
+          dl = xlse_radius(PM, jc, -1, jc, thr, igf);
         }
-        const int DL = __builtin_amdgcn_readlane(wave_scan(dl, MaxOp()), 63);
-        const int DR = __builtin_amdgcn_readlane(wave_scan(dr, MaxOp()), 63);
+        // the wave's widest window: both radii (< 2^16) in one packed scan
+        const int dlr =
+            __builtin_amdgcn_readlane(wave_scan((dl << 16) | dr, PkMaxU16()), 63);
+        const int DL = dlr >> 16, DR = dlr & 0xffff;
         const bool clamp = DL > kXPad || DR > kXPad;
         const float m = clamp ? xlse_window_max<true>(XF, CF, CFs, jc, len, DL, DR)
                               : xlse_window_max<false>(XF, CF, CFs, jc, len, DL, DR);
