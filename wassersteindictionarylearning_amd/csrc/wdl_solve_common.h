@@ -289,14 +289,11 @@ constexpr int kXBPad = 16;        // -inf positions on both sides of its prefix 
 // Dynamic LDS of k_xlse_pass: per line its values (FP64 scaled by log2 e, FP32),
 // input signs, prefix and suffix maxima, outputs and output signs; per
 // workgroup the penalty tables t^2 / gamma (FP32) and t^2 log2(e) / gamma (FP64)
-// for t = 0..len+2 (distances up to len - 1, and 2 for lines shorter than 2)
-// and, centred, for t = -kXPad..kXPad.
+// for t = -kXPad..kXPad.
 constexpr int kXSym = 2 * kXPad + 1;
-__host__ __device__ inline int xlse_tab_len(int len) { return len + 3; }
 __host__ __device__ inline int xlse_lds_bytes(int len, int lw = kXWaves) {
   const int lp = len + 2 * kXPad, lb = len + 2 * kXBPad;
-  return lw * (lp * 8 + len * 8 + lp * 4 * 2 + lb * 4 * 2 + len) +
-         (xlse_tab_len(len) + kXSym) * 12 + 64;
+  return lw * (lp * 8 + len * 8 + lp * 4 * 2 + lb * 4 * 2 + len) + kXSym * 12 + 64;
 }
 
 // Lines per workgroup of a pass: 8 when asked for and the workgroup fits

@@ -37,16 +37,17 @@ constexpr float kXMargin = 25.f;
 // every k lies in the padded line and |t| <= kXPad, and the penalty comes from
 // the table centred on t = 0 (CFs[t], t = -kXPad..kXPad: one address for the
 // whole window, stepped by immediate offsets); with it, positions outside the
-// line count as -inf and the penalty is CF[|t|].
+// line count as -inf and the penalty is formed as the table's entries are,
+// (float)(t t) / gamma.
 template <bool CLAMP>
-__device__ inline float xlse_window_max(const float* XF, const float* CF, const float* CFs, int jc,
-                                        int len, int DL, int DR) {
+__device__ inline float xlse_window_max(const float* XF, const float* CFs, int jc, int len,
+                                        int DL, int DR, float igf) {
   auto ld = [&](int t) {
     const int k = jc + t;
     if (CLAMP) return (unsigned)k < (unsigned)len ? XF[k] : -INFINITY;
     return XF[k];
   };
-  auto pen = [&](int t) { return CLAMP ? CF[abs(t)] : CFs[t]; };
+  auto pen = [&](int t) { return CLAMP ? (float)(t * t) * igf : CFs[t]; };
   float m0 = -INFINITY, m1 = -INFINITY;
   int t = -DL;
   for (; t + 3 <= DR; t += 4) {
@@ -66,12 +67,12 @@ __device__ inline float xlse_window_max(const float* XF, const float* CF, const 
 }
 
 // Sum of s_k 2^((X2_k - t^2 log2(e)/gamma) - m2) over the same window (C2s:
-// the centred FP64 table, C2: by |t|).
+// the centred FP64 table; with CLAMP t t (log2(e)/gamma) formed as its entries
+// are, ig2 = log2(e)/gamma).
 template <bool CLAMP, bool SIGNED>
-__device__ inline float xlse_window_sum(const double* X2, const float* SG, const double* C2,
-                                        const double* C2s, int jc, int len, int DL, int DR,
-                                        double m2) {
-  auto pen = [&](int t) { return CLAMP ? C2[abs(t)] : C2s[t]; };
+__device__ inline float xlse_window_sum(const double* X2, const float* SG, const double* C2s,
+                                        int jc, int len, int DL, int DR, double m2, double ig2) {
+  auto pen = [&](int t) { return CLAMP ? (double)t * t * ig2 : C2s[t]; };
   auto term = [&](double x2, double c, float sg) {
     const float e = __builtin_amdgcn_exp2f((float)((x2 - c) - m2));
     return SIGNED ? sg * e : e;
@@ -190,32 +191,26 @@ __device__ __forceinline__ void xlse_pass_body(const double* __restrict__ in,
   const int w = wave % LW, sub = wave / LW;  // line slot, wave within the line
   const float igf = (float)inv_gamma;
   const double kLog2e = 1.4426950408889634;
-  // layout: X2 [LW][lp], O0 [LW][len], C2 [nt], C2s [kXSym] (doubles);
-  // XF, SG [LW][lp], PM, SM [LW][lb], CF [nt], CFs [kXSym] (floats);
+  // layout: X2 [LW][lp], O0 [LW][len], C2s [kXSym] (doubles);
+  // XF, SG [LW][lp], PM, SM [LW][lb], CFs [kXSym] (floats);
   // SO0 [LW][len]
-  const int nt = xlse_tab_len(len);
   double* X2 = xs + w * lp + kXPad;
   double* O0 = xs + LW * lp;
   double* O = O0 + w * len;
-  double* C2 = O0 + LW * len;
-  double* C2s = C2 + nt + kXPad;  // C2s[t], t = -kXPad..kXPad
-  float* F0 = reinterpret_cast<float*>(C2 + nt + kXSym);
+  double* C2s = O0 + LW * len + kXPad;  // C2s[t], t = -kXPad..kXPad
+  float* F0 = reinterpret_cast<float*>(C2s + kXPad + 1);
   float* XF = F0 + w * lp + kXPad;
   float* SG = F0 + LW * lp + w * lp + kXPad;
   const int lb = len + 2 * kXBPad;
   float* PM = F0 + 2 * LW * lp + w * lb + kXBPad;
   float* SM = F0 + 2 * LW * lp + LW * lb + w * lb + kXBPad;
-  float* CF = F0 + 2 * LW * lp + 2 * LW * lb;
-  float* CFs = CF + nt + kXPad;
-  unsigned char* SO0 = reinterpret_cast<unsigned char*>(CF + nt + kXSym);
+  float* CFs = F0 + 2 * LW * lp + 2 * LW * lb + kXPad;
+  unsigned char* SO0 = reinterpret_cast<unsigned char*>(CFs + kXPad + 1);
   unsigned char* SO = SO0 + w * len;
-  for (int t = threadIdx.x; t < nt; t += blockDim.x) {
-    C2[t] = (double)t * t * (inv_gamma * kLog2e);
-    CF[t] = (float)(t * t) * igf;
-  }
+  const double ig2 = inv_gamma * kLog2e;
   for (int t = threadIdx.x; t < kXSym; t += blockDim.x) {
     const int d = t - kXPad;
-    C2s[d] = (double)d * d * (inv_gamma * kLog2e);
+    C2s[d] = (double)d * d * ig2;
     CFs[d] = (float)(d * d) * igf;
   }
   // padding: the window loops read up to kXPad positions past either end,
@@ -313,7 +308,7 @@ __device__ __forceinline__ void xlse_pass_body(const double* __restrict__ in,
         if (valid) {
           float m0 = XF[jc];
 #pragma unroll
-          for (int t = 1; t <= 2; ++t) m0 = fmaxf(m0, fmaxf(XF[jc - t], XF[jc + t]) - CF[t]);
+          for (int t = 1; t <= 2; ++t) m0 = fmaxf(m0, fmaxf(XF[jc - t], XF[jc + t]) - CFs[t]);
           const float thr = m0 - kXMargin;
           dr = xlse_radius(SM, jc, 1, len - 1 - jc, thr, igf);
           dl = xlse_radius(PM, jc, -1, jc, thr, igf);
@@ -323,13 +318,13 @@ __device__ __forceinline__ void xlse_pass_body(const double* __restrict__ in,
             __builtin_amdgcn_readlane(wave_scan((dl << 16) | dr, PkMaxU16()), 63);
         const int DL = dlr >> 16, DR = dlr & 0xffff;
         const bool clamp = DL > kXPad || DR > kXPad;
-        const float m = clamp ? xlse_window_max<true>(XF, CF, CFs, jc, len, DL, DR)
-                              : xlse_window_max<false>(XF, CF, CFs, jc, len, DL, DR);
+        const float m = clamp ? xlse_window_max<true>(XF, CFs, jc, len, DL, DR, igf)
+                              : xlse_window_max<false>(XF, CFs, jc, len, DL, DR, igf);
         const double m64 = fmax((double)m, -1e99);
         const double m2 = m64 * kLog2e;
         const float sum =
-            clamp ? xlse_window_sum<true, SIGNED>(X2, SG, C2, C2s, jc, len, DL, DR, m2)
-                  : xlse_window_sum<false, SIGNED>(X2, SG, C2, C2s, jc, len, DL, DR, m2);
+            clamp ? xlse_window_sum<true, SIGNED>(X2, SG, C2s, jc, len, DL, DR, m2, ig2)
+                  : xlse_window_sum<false, SIGNED>(X2, SG, C2s, jc, len, DL, DR, m2, ig2);
         if (valid) {
           O[j] = m64 + hlog(fabsf(sum));
           if (SIGNED) SO[j] = sum >= 0.f ? 1 : 0;
