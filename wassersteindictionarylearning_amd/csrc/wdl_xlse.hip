@@ -351,8 +351,12 @@ __device__ __forceinline__ void xlse_pass_body(const double* __restrict__ in,
   }
 }
 
+// At least 5 waves per SIMD: the signed passes would take ~100 VGPRs and hold
+// 4; limited to 96 they spill a few prefetch addresses, once per line group
+// (5 waves: 59.1k against 57.8k Sinkhorn-iters/s on the 512-request
+// projection; 6 waves spill more and lose).
 template <bool SIGNED, class Epi, int LW>
-__global__ __launch_bounds__(kXMaxThreads) void k_xlse_pass(
+__global__ __launch_bounds__(kXMaxThreads) __attribute__((amdgpu_waves_per_eu(5))) void k_xlse_pass(
     const double* __restrict__ in, const unsigned char* __restrict__ sin, Epi epi, int rows,
     int len, long long nlines, double inv_gamma, const int* __restrict__ Lp, int p0, int ipp,
     int active_thr) {
