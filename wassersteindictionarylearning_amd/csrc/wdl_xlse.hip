@@ -177,7 +177,7 @@ constexpr int kXMaxThreads = 1024;
 //
 // blk / nblk: this workgroup's index and the number of workgroups sharing the
 // lines (the grid, or a member's share of a group launch).
-template <bool SIGNED, class Epi, int LW>
+template <bool SIGNED, class Epi, int LW, int KP = 4>
 __device__ __forceinline__ void xlse_pass_body(const double* __restrict__ in,
                                                const unsigned char* __restrict__ sin, Epi epi,
                                                int rows, int len, long long nlines,
@@ -223,14 +223,15 @@ __device__ __forceinline__ void xlse_pass_body(const double* __restrict__ in,
   if (sub == 0 && lane < kXBPad) PM[-1 - lane] = PM[len + lane] = SM[-1 - lane] = SM[len + lane] = -INFINITY;
   // Per line slot: whether its line runs (1) or is skipped, for the store.
   unsigned char* ACT = SO0 + LW * len;
-  // The next line group's values (at most kXPer per lane: lines up to 1024
-  // over 4 waves), sign bytes and iteration counts, loaded one group ahead so
+  // The next line group's values (at most KP per lane: 4 for lines up to 1024
+  // over 4 waves, 2 where a line has at most 128 positions per wave), sign
+  // bytes and iteration counts, loaded one group ahead so
   // the loads' latency passes under the current group's work.  Lane i loads
   // the count of line slot i % LW and the wave reads its slot's lane when the
   // group starts: a load at the wave's own (uniform) address would be a
   // scalar load, waited for at the next LDS access (both count on the same
   // counter), i.e. at once.
-  constexpr int kXPer = 4;
+  constexpr int kXPer = KP;
   double pv[kXPer];
   unsigned char ps[kXPer];
   bool pinb = false;
@@ -354,14 +355,15 @@ __device__ __forceinline__ void xlse_pass_body(const double* __restrict__ in,
 // At least 5 waves per SIMD: the signed passes would take ~100 VGPRs and hold
 // 4; limited to 96 they spill a few prefetch addresses, once per line group
 // (5 waves: 59.1k against 57.8k Sinkhorn-iters/s on the 512-request
-// projection; 6 waves spill more and lose).
-template <bool SIGNED, class Epi, int LW>
+// projection; 6 waves spill more and lose).  KP: values per lane prefetched
+// (2 when each wave of a line holds at most 128 positions).
+template <bool SIGNED, class Epi, int LW, int KP>
 __global__ __launch_bounds__(kXMaxThreads) __attribute__((amdgpu_waves_per_eu(5))) void k_xlse_pass(
     const double* __restrict__ in, const unsigned char* __restrict__ sin, Epi epi, int rows,
     int len, long long nlines, double inv_gamma, const int* __restrict__ Lp, int p0, int ipp,
     int active_thr) {
-  xlse_pass_body<SIGNED, Epi, LW>(in, sin, epi, rows, len, nlines, inv_gamma, Lp, p0, ipp,
-                                  active_thr, blockIdx.x, gridDim.x);
+  xlse_pass_body<SIGNED, Epi, LW, KP>(in, sin, epi, rows, len, nlines, inv_gamma, Lp, p0, ipp,
+                                      active_thr, blockIdx.x, gridDim.x);
 }
 
 // Output stage of k_xlse_pass: line(img, N) gives, for image img (of N
@@ -388,11 +390,11 @@ static std::atomic<int> g_xlse_cus{0};
 
 // Solvers may be created from several host threads (the server's included);
 // a repeated attribute call is harmless, so a relaxed atomic suffices.
-template <bool SIGNED, class Epi, int LW>
+template <bool SIGNED, class Epi, int LW, int KP>
 static int xlse_set_attr_lw(int lds) {
   static std::atomic<int> done{0};
   if (lds <= done.load(std::memory_order_relaxed)) return WDL_OK;
-  WDL_HIP(hipFuncSetAttribute((const void*)k_xlse_pass<SIGNED, Epi, LW>,
+  WDL_HIP(hipFuncSetAttribute((const void*)k_xlse_pass<SIGNED, Epi, LW, KP>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, lds));
   int cur = done.load(std::memory_order_relaxed);
   while (cur < lds && !done.compare_exchange_weak(cur, lds, std::memory_order_relaxed)) {
@@ -405,9 +407,12 @@ static int xlse_set_attr_lw(int lds) {
 template <bool SIGNED, class Epi>
 static int xlse_set_attr(int maxlen) {
   int e;
-  if ((e = xlse_set_attr_lw<SIGNED, Epi, 4>(xlse_lds_bytes(maxlen, 4)))) return e;
-  const int l8 = xlse_lds_bytes(maxlen, 8);
-  return l8 <= kXLdsMax ? xlse_set_attr_lw<SIGNED, Epi, 8>(l8) : WDL_OK;
+  const int l4 = xlse_lds_bytes(maxlen, 4), l8 = xlse_lds_bytes(maxlen, 8);
+  if ((e = xlse_set_attr_lw<SIGNED, Epi, 4, 4>(l4)) || (e = xlse_set_attr_lw<SIGNED, Epi, 4, 2>(l4)))
+    return e;
+  if (l8 > kXLdsMax) return WDL_OK;
+  if ((e = xlse_set_attr_lw<SIGNED, Epi, 8, 4>(l8))) return e;
+  return xlse_set_attr_lw<SIGNED, Epi, 8, 2>(l8);
 }
 
 // One pass over nimg images of rows lines of length len (see k_xlse_pass).
@@ -425,12 +430,16 @@ static void xlse_launch(const double* in, const unsigned char* sin, Epi epi, int
   const long long cap = 2LL * (cus > 0 ? cus : 256) * perCU;
   const int grid = (int)std::min<long long>((nl + lw - 1) / lw, cap);
   const int block = lw * 64 * w;
-  if (lw == 8)
-    hipLaunchKernelGGL((k_xlse_pass<SIGNED, Epi, 8>), dim3(grid), dim3(block), lds, st, in, sin,
-                       epi, rows, len, nl, ig, Lp, p0, ipp, thr);
-  else
-    hipLaunchKernelGGL((k_xlse_pass<SIGNED, Epi, 4>), dim3(grid), dim3(block), lds, st, in, sin,
-                       epi, rows, len, nl, ig, Lp, p0, ipp, thr);
+  const bool kp2 = len <= 2 * 64 * w;  // at most 2 values per lane to prefetch
+#define WDL_XLAUNCH(LW_, KP_)                                                                   \
+  hipLaunchKernelGGL((k_xlse_pass<SIGNED, Epi, LW_, KP_>), dim3(grid), dim3(block), lds, st, in, \
+                     sin, epi, rows, len, nl, ig, Lp, p0, ipp, thr)
+  if (lw == 8) {
+    if (kp2) WDL_XLAUNCH(8, 2); else WDL_XLAUNCH(8, 4);
+  } else {
+    if (kp2) WDL_XLAUNCH(4, 2); else WDL_XLAUNCH(4, 4);
+  }
+#undef WDL_XLAUNCH
 }
 
 // Two passes (along x, then along y) over nimg W x H images; tmp holds the
