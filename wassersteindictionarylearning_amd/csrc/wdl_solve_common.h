@@ -106,7 +106,9 @@ __device__ inline bool map_block(const SolveArgs& a, int& p, int& i) {
   return true;
 }
 
-__device__ inline int steps_of(const SolveArgs& a, int p) { return a.Lp ? a.Lp[p] : a.L; }
+// (A: SolveArgs, or a struct holding the fields these read.)
+template <class A>
+__device__ inline int steps_of(const A& a, int p) { return a.Lp ? a.Lp[p] : a.L; }
 
 // Forward launch s runs histogram p when p has an iteration s and reaches rmin.
 __device__ inline bool fwd_runs(const SolveArgs& a, int p, int s) {
@@ -118,12 +120,14 @@ __device__ inline bool fwd_runs(const SolveArgs& a, int p, int s) {
 // that many iterations: a histogram with L_p < L starts its backward at launch
 // L_p, so histogram p's history entry of step s is written and read at the
 // same launch index s in both directions.  Returns 0 when p has no step t.
-__device__ inline int bwd_step(const SolveArgs& a, int p, int t) {
+template <class A>
+__device__ inline int bwd_step(const A& a, int p, int t) {
   return t <= steps_of(a, p) ? t : 0;
 }
 
 // History slot of entry j (0-based: entry s-1 belongs to iteration s).
-__host__ __device__ inline long long hslot(const SolveArgs& a, int j) {
+template <class A>
+__host__ __device__ inline long long hslot(const A& a, int j) {
   return a.ring ? (long long)(j % a.ring) : (long long)j;
 }
 

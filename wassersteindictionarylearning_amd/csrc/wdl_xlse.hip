@@ -497,11 +497,28 @@ struct XEpiFwd1 {
   }
 };
 
+// The fields of SolveArgs the backward output stages read: the stages carry
+// this instead of the whole SolveArgs, which a group launch would otherwise
+// copy out of its member table into scratch memory.
+struct XBwdArgs {
+  long long N, stage;
+  int K, p0, L, ring;
+  const int* Lp;
+  const float *Ch, *Fh, *Bh, *logA;
+  float *GD, *V0, *V1, *R0, *R1;
+  double* xa;
+  unsigned char* sga;
+};
+__host__ __device__ inline XBwdArgs xbwd_args(const SolveArgs& a) {
+  return XBwdArgs{a.N,  a.stage, a.K,  a.p0, a.L,  a.ring, a.Lp,  a.Ch,  a.Fh,
+                  a.Bh, a.logA,  a.GD, a.V0, a.V1, a.R0,   a.R1,  a.xa,  a.sga};
+}
+
 // backward step s, after the first convolution: resultDic (chain 0, :750-752)
 // and, for s >= 2, the second convolution's input log p + c - 2 log c^{s-1},
 // sign of c (:754-756, :796)
 struct XEpiBwd1 {
-  SolveArgs a;
+  XBwdArgs a;
   int t, nch;
   long long off;  // scratch offset of the first image
   struct Line {
@@ -546,7 +563,7 @@ struct XEpiBwd1 {
 // backward step s >= 2, after the second convolution: -v = sign exp(res - log
 // phi^{s-1}) (:757-762), -r = sign exp(res + log b^{s-1}) (:797-805)
 struct XEpiBwd2 {
-  SolveArgs a;
+  XBwdArgs a;
   int t, nch;
   struct Line {
     const double* src;  // log phi^{s-1} (chain 0) or log b^{s-1} (chain 1) of the image
@@ -818,10 +835,10 @@ int exact_backward(const SolveArgs& a, hipStream_t st, int t) {
   }
   hipLaunchKernelGGL(k_xdot_fin, ew_grid((long long)a.np * a.K), dim3(256), 0, st, a, t);
   // histogram p runs step t when L_p >= t (the second convolution for t >= 2)
-  if ((e = xlse_conv_e(a.xa + off, a.sga + off, XEpiBwd1{a, t, nch, off}, a.xt + off, a.sgt + off,
+  if ((e = xlse_conv_e(a.xa + off, a.sga + off, XEpiBwd1{xbwd_args(a), t, nch, off}, a.xt + off, a.sgt + off,
                        W, H, nimg, g, a.Lp, a.p0, nch * a.K, t, st, a.xwpl, a.xlw)))
     return e;
-  if (t >= 2 && (e = xlse_conv_e(a.xa + off, a.sga + off, XEpiBwd2{a, t, nch}, a.xt + off,
+  if (t >= 2 && (e = xlse_conv_e(a.xa + off, a.sga + off, XEpiBwd2{xbwd_args(a), t, nch}, a.xt + off,
                                  a.sgt + off, W, H, nimg, g, a.Lp, a.p0, nch * a.K, t, st,
                                  a.xwpl, a.xlw)))
     return e;
@@ -842,18 +859,18 @@ __global__ void k_xf1_group(GroupArgs g, int s) {
   int lb, nb;
   const GroupMember& gm = g.m[group_member(g, GK_XF1, blockIdx.x, lb, nb)];
   if (*gm.lmax < s) return;
-  const SolveArgs a = gm.a;
+  const SolveArgs& a = gm.a;
   xf1_body(a, s, lb, nb);
 }
 
 // Forward passes of iteration s: STAGE 0 / 1 the first convolution along x /
 // y (output stage XEpiFwd1), 2 / 3 the second (into the history Fh).
 template <int STAGE>
-__global__ __launch_bounds__(kXMaxThreads) void k_xlse_group_fwd(GroupArgs g, int s) {
+__global__ __launch_bounds__(kXMaxThreads) __attribute__((amdgpu_waves_per_eu(5))) void k_xlse_group_fwd(GroupArgs g, int s) {
   int lb, nb;
   const GroupMember& gm =
       g.m[group_member(g, (STAGE & 1) ? GK_FB : GK_FA, blockIdx.x, lb, nb)];
-  const SolveArgs a = gm.a;
+  const SolveArgs& a = gm.a;
   const double ig = gm.ig;
   const int thr = max(s, a.rmin);
   if (*gm.lmax < thr) return;
@@ -878,11 +895,11 @@ __global__ __launch_bounds__(kXMaxThreads) void k_xlse_group_fwd(GroupArgs g, in
 // Backward passes of step t: STAGE 0 / 1 the first convolution along x / y
 // (output stage XEpiBwd1), 2 / 3 the second (XEpiBwd2; launched for t >= 2).
 template <int STAGE>
-__global__ __launch_bounds__(kXMaxThreads) void k_xlse_group_bwd(GroupArgs g, int t) {
+__global__ __launch_bounds__(kXMaxThreads) __attribute__((amdgpu_waves_per_eu(5))) void k_xlse_group_bwd(GroupArgs g, int t) {
   int lb, nb;
   const GroupMember& gm =
       g.m[group_member(g, (STAGE & 1) ? GK_BB : GK_BA, blockIdx.x, lb, nb)];
-  const SolveArgs a = gm.a;
+  const SolveArgs& a = gm.a;
   const double ig = gm.ig;
   if (*gm.lmax < t) return;
   const int W = a.G.W, H = a.G.H, nch = a.weightsOnly ? 1 : 2;
@@ -891,10 +908,10 @@ __global__ __launch_bounds__(kXMaxThreads) void k_xlse_group_bwd(GroupArgs g, in
     xlse_pass_body<true, XStore, kXWaves>(a.xa + off, a.sga + off, XStore{a.xt + off, a.sgt + off},
                                           H, W, nimg * H, ig, a.Lp, a.p0, nch * a.K, t, lb, nb);
   } else if constexpr (STAGE == 1) {
-    xlse_pass_body<true, XEpiBwd1, kXWaves>(a.xt + off, a.sgt + off, XEpiBwd1{a, t, nch, off}, W,
+    xlse_pass_body<true, XEpiBwd1, kXWaves>(a.xt + off, a.sgt + off, XEpiBwd1{xbwd_args(a), t, nch, off}, W,
                                             H, nimg * W, ig, a.Lp, a.p0, nch * a.K, t, lb, nb);
   } else {
-    xlse_pass_body<true, XEpiBwd2, kXWaves>(a.xt + off, a.sgt + off, XEpiBwd2{a, t, nch}, W, H,
+    xlse_pass_body<true, XEpiBwd2, kXWaves>(a.xt + off, a.sgt + off, XEpiBwd2{xbwd_args(a), t, nch}, W, H,
                                             nimg * W, ig, a.Lp, a.p0, nch * a.K, t, lb, nb);
   }
 }
@@ -904,7 +921,7 @@ __global__ __launch_bounds__(kThreads) void k_xdot_chunk_group(GroupArgs g, int 
   int lb, nb;
   const GroupMember& gm = g.m[group_member(g, GK_DOT, blockIdx.x, lb, nb)];
   if (*gm.lmax < t) return;
-  const SolveArgs a = gm.a;
+  const SolveArgs& a = gm.a;
   xdot_chunk_body<FUSED>(a, t, lb);
 }
 
@@ -912,7 +929,7 @@ __global__ void k_xb1_group(GroupArgs g, int t) {
   int lb, nb;
   const GroupMember& gm = g.m[group_member(g, GK_XB1, blockIdx.x, lb, nb)];
   if (*gm.lmax < t) return;
-  const SolveArgs a = gm.a;
+  const SolveArgs& a = gm.a;
   xb1_body(a, t, a.weightsOnly ? 1 : 2, lb, nb);
 }
 
@@ -920,7 +937,7 @@ __global__ void k_xdot_fin_group(GroupArgs g, int t) {
   int lb, nb;
   const GroupMember& gm = g.m[group_member(g, GK_DOTFIN, blockIdx.x, lb, nb)];
   if (*gm.lmax < t) return;
-  const SolveArgs a = gm.a;
+  const SolveArgs& a = gm.a;
   xdot_fin_body(a, t, lb, nb);
 }
 
