@@ -118,8 +118,10 @@ def projection_leg(ctx, requests=4096, chunk=512, one_launch=True):
         s.set_observed(c["obs"][sl])
         s.set_problem(atoms, c["weights"][sl])
         s.set_iterations(c["iterations"][sl])
-        if tuning is None:  # the launch shape, tuned on the first chunk
-            s.autotune(1)
+        if tuning is None:
+            # the launch shape, tuned on the first chunk; 3 timed runs per
+            # candidate (with one, 2 and 4 groups fell within its noise)
+            s.autotune(3)
             tuning = s.tuning()
             log("projection tuning: %s" % tuning)
         ctx.synchronize()
