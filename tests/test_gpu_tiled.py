@@ -182,8 +182,9 @@ def test_tiled_uneven_iterations_weights_only_warm(ctx, logd):
 def test_tiled_errors(ctx):
     with pytest.raises(wdl.WdlError):  # the dense flag is for the resident path only
         wdl.SinkhornSolver(ctx, 256, 256, 2, 1, 3, 1.0, 2, wdl.FLAG_DENSE)
-    with pytest.raises(wdl.WdlError):  # log domain at gamma 8 needs a halo above 32
-        wdl.SinkhornSolver(ctx, 256, 256, 2, 1, 3, 8.0, 2, wdl.FLAG_LOG_DOMAIN)
+    for flags in (0, wdl.FLAG_LOG_DOMAIN):  # gamma 12: factors reach 35 pixels, halo above 32
+        with pytest.raises(wdl.WdlError):
+            wdl.SinkhornSolver(ctx, 256, 256, 2, 1, 3, 12.0, 2, flags)
 
 
 def test_log_config_full_size_against_reference(ctx):

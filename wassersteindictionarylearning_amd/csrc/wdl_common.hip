@@ -64,13 +64,14 @@ ConvGeom make_geom(int W, int H, int D, int pad) {
 
 int make_tile_geom(int W, int H, int D, double gamma, bool logdom, TileGeom& t) {
   // Halo HB >= D: every output of the 64 x 64 tile sees all its non-zero
-  // factors.  In the log domain terms beyond the halo are dropped although
-  // their exact factor is not 0: with HB chosen so that (HB+1)^2 / gamma >= 264,
-  // they weigh below e^-264 times a term inside the checked spread (two
-  // neighbouring windows of at most kLogRange each), i.e. nothing in FP64.
-  int hb = round_up(D > 0 ? D : 1, 16);
-  if (logdom)
-    while ((double)(hb + 1) * (hb + 1) / gamma < 264.0) hb += 16;
+  // factors, and a multiple of 8, so R = 64 + 2 HB is a multiple of 16 (the
+  // MFMA row blocks, the region rows of the 16 half-waves) and S / 4 is odd.
+  // Positions further out would meet only factors that are 0 in FP32 --
+  // also in the log domain, whose products are these factors times shifted
+  // values <= 1; there a wider region would only widen the rows' shifts.
+  (void)gamma;
+  (void)logdom;
+  const int hb = round_up(D > 0 ? D : 1, 8);
   if (hb > kHaloMax) {
     set_error("tiled convolution: gamma " + std::to_string(gamma) + " needs a halo of " +
               std::to_string(hb) + " pixels, above " + std::to_string(kHaloMax));
@@ -80,9 +81,9 @@ int make_tile_geom(int W, int H, int D, double gamma, bool logdom, TileGeom& t) 
   t.H = H;
   t.HB = hb;
   t.R = kTile + 2 * hb;
-  // R is a multiple of 32, so S / 4 = R / 4 + 1 is odd: the 16 rows of a
-  // quarter-wave's block start on 16 distinct groups of 4 banks (as the
-  // resident strides, which are congruent to 4 mod 64), with the least LDS
+  // R is a multiple of 16 with R / 4 even, so S / 4 = R / 4 + 1 is odd: the 16
+  // rows of a quarter-wave's block start on 16 distinct groups of 4 banks (as
+  // the resident strides, which are congruent to 4 mod 64), with the least LDS
   t.S = t.R + 4;
   t.D = D;
   t.nx = (W + kTile - 1) / kTile;

@@ -1,7 +1,7 @@
 // Tiled separable Gaussian convolution for images larger than the LDS holds.
 //
 // A workgroup computes one 64 x 64 output tile.  It loads the tile plus a halo
-// of HB pixels on every side (HB >= D, a multiple of 16) into LDS, runs pass 1
+// of HB pixels on every side (HB >= D, a multiple of 8) into LDS, runs pass 1
 // (along x) over all R = 64 + 2 HB haloed rows for the 64 inner columns,
 // writes the result transposed, and runs pass 2 (along y) over the R haloed
 // positions for the 64 inner rows.  Factors beyond D are exactly 0 in FP32,
@@ -130,7 +130,7 @@ __device__ inline void tile_log_conv(float* buf, const float* tabs, const LogLds
     // group of 4 pixels per lane (R / 4 <= 32 groups), so the row's maximum
     // and finite minimum are half-wave DPP reductions and each value is
     // shifted by its row's maximum as soon as it is made -- one round of
-    // loads and one make() per pixel.  R is a multiple of 32, so every wave
+    // loads and one make() per pixel.  R is a multiple of 16, so every wave
     // runs the same R / 16 rows' worth of steps with all lanes active.
     const int q = T.R / 4;
     const int lane = threadIdx.x & 63, l32 = lane & 31;
