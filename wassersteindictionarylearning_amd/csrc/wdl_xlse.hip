@@ -108,49 +108,82 @@ __device__ inline float xlse_window_sum(const double* X2, const float* SG, const
   return s0 + s1;
 }
 
-// Radius of output jc's window on one side (B: suffix maxima stepping right,
-// dir = 1, or prefix maxima stepping left, dir = -1; lim: positions available
-// on that side).  The bound B[jc + dir d] - d^2/gamma falls strictly with d,
-// so the distances 4, 8, 12, 16 that pass form a prefix: their count locates
-// the first failing multiple of 4, a test at 2 below it the first failing
-// even distance d, and one at d - 1 the first failing distance; the radius is
-// the distance before it.  Windows wider than 16 continue in steps of 2.  B
-// holds -inf for kXBPad positions past either end of the line, so a distance
-// beyond it fails as a distance whose bound is too low does: no clamping to
-// lim below 18 (the penalties are formed as the table's are, (float)(d d) / gamma).
-__device__ inline int xlse_radius(const float* B, int jc, int dir, int lim, float thr, float igf) {
-  auto pen = [&](int d) { return (float)(d * d) * igf; };
-  float b[4];  // independent reads, issued together
-#pragma unroll
-  for (int i = 0; i < 4; ++i) b[i] = B[jc + dir * (4 * i + 4)];
-  int cnt = 0;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) cnt += (int)(b[i] - pen(4 * i + 4) > thr);
-  int d;
-  if (cnt < 4) {
-    d = 4 * cnt + 4;  // first failing multiple of 4
-    const int h = d - 2;
-    if (!(B[jc + dir * h] - pen(h) > thr)) d = h;
-  } else {
-    d = 18;
-    while (d <= lim && B[jc + dir * d] - pen(d) > thr) d += 2;
-  }
-  // d fails and d - 2 passes (or is 0, the centre); d - 1 decides (d - 1 is at
-  // most lim + 1: within the padding)
-  const int o = d - 1;
-  if (!(B[jc + dir * o] - pen(o) > thr)) d = o;
-  return min(d - 1, lim);
-}
-
-// Maximum of two pairs of 16-bit counts packed in an int (the window's left
-// and right radii, scanned together).
-struct PkMaxU16 {
-  typedef unsigned short us2 __attribute__((ext_vector_type(2)));
-  __device__ int operator()(int a, int b) const {
-    return __builtin_bit_cast(int, __builtin_elementwise_max(__builtin_bit_cast(us2, a),
-                                                             __builtin_bit_cast(us2, b)));
-  }
+// The wave's window radii {DL, DR}: on each side the largest distance d at
+// which some lane's bound B[jc + dir d] - d^2/gamma still exceeds its
+// threshold (B: prefix maxima stepping left, dir = -1, suffix maxima stepping
+// right, dir = 1), i.e. the widest of the lanes' own radii.  A lane's bound
+// falls with d and B holds -inf for kXBPad positions past either end of the
+// line, so a lane's passing distances form a prefix ending within the line,
+// and so do the distances that pass for any lane: the distances 4, 8, 12, 16
+// are tested at once by ballot, further multiples of 4 while all of them
+// pass (positions past the line read from its first padding entry), then the
+// three distances below the first failing multiple.  Lanes past the line carry
+// thr = +inf and pass nowhere.  Penalties are formed as the tables' entries
+// are, (float)(d d) / gamma (CFs: the centred table).
+struct XRadii {
+  int dl, dr;
 };
+__device__ inline XRadii xlse_wave_radii(const float* PM, const float* SM, const float* CFs,
+                                         int jc, int len, float thr, float igf) {
+  auto pen = [&](int d) { return (float)(d * d) * igf; };
+  // 1 if some lane passes; formed on the 64-bit mask so that it stays scalar
+  auto any = [](bool p) {
+    const unsigned long long m = __builtin_amdgcn_ballot_w64(p);
+    return (int)((m | (0ull - m)) >> 63);
+  };
+  float r[4], l[4];  // independent reads, issued together
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    r[i] = SM[jc + 4 * i + 4];
+    l[i] = PM[jc - 4 * i - 4];
+  }
+  int cr = 0, cl = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    cr += any(r[i] - pen(4 * i + 4) > thr);
+    cl += any(l[i] - pen(4 * i + 4) > thr);
+  }
+  int br = 4 * cr, bl = 4 * cl;  // pass (or 0, the centre); br + 4, bl + 4 fail
+  for (int c = cr; c == 4;) {
+    c = 0;
+#pragma unroll
+    for (int i = 1; i <= 4; ++i) c += any(SM[min(jc + br + 4 * i, len)] - pen(br + 4 * i) > thr);
+    br += 4 * c;
+  }
+  for (int c = cl; c == 4;) {
+    c = 0;
+#pragma unroll
+    for (int i = 1; i <= 4; ++i) c += any(PM[max(jc - bl - 4 * i, -1)] - pen(bl + 4 * i) > thr);
+    bl += 4 * c;
+  }
+  // the three distances below the first failing multiple of 4: within the
+  // padding and the penalty table when it is at most 16
+  float r3[3], l3[3], pr[3], pl[3];
+  if (br <= 12 && bl <= 12) {
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+      r3[u] = SM[jc + br + 1 + u];
+      l3[u] = PM[jc - bl - 1 - u];
+      pr[u] = CFs[br + 1 + u];
+      pl[u] = CFs[bl + 1 + u];
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+      r3[u] = SM[min(jc + br + 1 + u, len)];
+      l3[u] = PM[max(jc - bl - 1 - u, -1)];
+      pr[u] = pen(br + 1 + u);
+      pl[u] = pen(bl + 1 + u);
+    }
+  }
+  XRadii w{bl, br};
+#pragma unroll
+  for (int u = 0; u < 3; ++u) {
+    w.dr += any(r3[u] - pr[u] > thr);
+    w.dl += any(l3[u] - pl[u] > thr);
+  }
+  return w;
+}
 
 constexpr int kXMaxThreads = 1024;
 
@@ -280,8 +313,20 @@ __device__ __forceinline__ void xlse_pass_body(const double* __restrict__ in,
     if (l0 + stride < nlines) fetch(l0 + stride);
     // prefix maxima (wave 0 of the line) and suffix maxima (its last wave):
     // inclusive max-scans across the wave (wave_scan), 64 positions per
-    // step, with a running carry from lane 63
-    if (active && sub == 0) {
+    // step, with a running carry from lane 63; a line of one wave runs both
+    // in one loop, so that the two chains of dependent steps interleave
+    if (active && wpl == 1) {
+      float cp = -INFINITY, cs = -INFINITY;
+      for (int base = 0; base < len; base += 64) {
+        const int k = base + lane, q = len - 1 - base - lane;
+        const float v = fmaxf(wave_scan(k < len ? XF[k] : -INFINITY, MaxOp()), cp);
+        const float u = fmaxf(wave_scan(q >= 0 ? XF[q] : -INFINITY, MaxOp()), cs);
+        if (k < len) PM[k] = v;
+        if (q >= 0) SM[q] = u;
+        cp = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+        cs = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(u), 63));
+      }
+    } else if (active && sub == 0) {
       float carry = -INFINITY;
       for (int base = 0; base < len; base += 64) {
         const int k = base + lane;
@@ -290,7 +335,7 @@ __device__ __forceinline__ void xlse_pass_body(const double* __restrict__ in,
         carry = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
       }
     }
-    if (active && sub == wpl - 1) {
+    if (active && wpl > 1 && sub == wpl - 1) {
       float carry = -INFINITY;
       for (int top = len - 1; top >= 0; top -= 64) {
         const int k = top - lane;
@@ -305,19 +350,12 @@ __device__ __forceinline__ void xlse_pass_body(const double* __restrict__ in,
         const int j = j0 + lane;
         const bool valid = j < len;
         const int jc = valid ? j : len - 1;
-        int dl = 0, dr = 0;
-        if (valid) {
-          float m0 = XF[jc];
+        float m0 = XF[jc];
 #pragma unroll
-          for (int t = 1; t <= 2; ++t) m0 = fmaxf(m0, fmaxf(XF[jc - t], XF[jc + t]) - CFs[t]);
-          const float thr = m0 - kXMargin;
-          dr = xlse_radius(SM, jc, 1, len - 1 - jc, thr, igf);
-          dl = xlse_radius(PM, jc, -1, jc, thr, igf);
-        }
-        // the wave's widest window: both radii (< 2^16) in one packed scan
-        const int dlr =
-            __builtin_amdgcn_readlane(wave_scan((dl << 16) | dr, PkMaxU16()), 63);
-        const int DL = dlr >> 16, DR = dlr & 0xffff;
+        for (int t = 1; t <= 2; ++t) m0 = fmaxf(m0, fmaxf(XF[jc - t], XF[jc + t]) - CFs[t]);
+        const XRadii wr =
+            xlse_wave_radii(PM, SM, CFs, jc, len, valid ? m0 - kXMargin : INFINITY, igf);
+        const int DL = wr.dl, DR = wr.dr;
         const bool clamp = DL > kXPad || DR > kXPad;
         const float m = clamp ? xlse_window_max<true>(XF, CFs, jc, len, DL, DR, igf)
                               : xlse_window_max<false>(XF, CFs, jc, len, DL, DR, igf);
