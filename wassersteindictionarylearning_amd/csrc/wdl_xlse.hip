@@ -390,13 +390,15 @@ __device__ __forceinline__ void xlse_pass_body(const double* __restrict__ in,
   }
 }
 
-// At least 5 waves per SIMD: the signed passes would take ~100 VGPRs and hold
-// 4; limited to 96 they spill a few prefetch addresses, once per line group
-// (5 waves: 59.1k against 57.8k Sinkhorn-iters/s on the 512-request
-// projection; 6 waves spill more and lose).  KP: values per lane prefetched
-// (2 when each wave of a line holds at most 128 positions).
+// Waves per SIMD.  KP: values per lane prefetched (2 when each wave of a line
+// holds at most 128 positions).  With 2, 7 for the unsigned passes (71 VGPRs)
+// and 6 for the signed ones (79 VGPRs; the backward output stages spill 12
+// bytes): each ~1% more Sinkhorn-iters/s on the 512-request 128x128
+// projection than 6 and 5.  With 4, 5: the signed passes would take ~100 VGPRs
+// and hold 4; limited to 96 they spill a few prefetch addresses, once per line
+// group (59.1k against 57.8k on the same projection).
 template <bool SIGNED, class Epi, int LW, int KP>
-__global__ __launch_bounds__(kXMaxThreads) __attribute__((amdgpu_waves_per_eu(5))) void k_xlse_pass(
+__global__ __launch_bounds__(kXMaxThreads) __attribute__((amdgpu_waves_per_eu(KP > 2 ? 5 : (SIGNED ? 6 : 7)))) void k_xlse_pass(
     const double* __restrict__ in, const unsigned char* __restrict__ sin, Epi epi, int rows,
     int len, long long nlines, double inv_gamma, const int* __restrict__ Lp, int p0, int ipp,
     int active_thr) {
